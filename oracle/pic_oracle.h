@@ -1,0 +1,121 @@
+/*
+ * pic_oracle.h -- CPU oracle for the PIC particle stage.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain-C, FP64, operation-for-operation
+ * restatement of the reference's particle stage (field interpolation, the
+ * Boris-Leapfrog push, TriaTracking, cell_volweight_mean and shape_function
+ * deposition, and the mesh tables those routines read).  It is the checker
+ * for the HIP implementation in piclas_amd/ and the CPU baseline timed
+ * beside it.  Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it; the product never links or calls it.
+ *
+ * Conventions follow the reference's Fortran arrays:
+ *   ElemInfo[nElems][6], SideInfo[nSides][5]   HOPR mesh arrays
+ *                                               (src/piclas.h:136-176)
+ *   NodeCoords[nNodes][3], GlobalNodeIDs[nNodes]
+ *   U_N%E(1:3,0:N,0:N,0:N)     -> E[nElems][(N+1)^3][3]   (i fastest)
+ *   PS_N%PartSource(1:4,...)   -> PS[nElems][(N+1)^3][4]
+ *   PartState(1:6,nPart)       -> PartState[nPart][6]
+ *   element ids at the API are 1-based global ids (0 = unknown / lost).
+ */
+#ifndef PIC_ORACLE_H
+#define PIC_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+  int32_t nElems, nSides, nNodes;
+  int32_t NGeo;                  /* geometry degree of NodeCoords (1 = linear hex) */
+  int32_t N;                     /* DG degree of E and PartSource                  */
+  const int32_t *ElemInfo;       /* [nElems][6]                                    */
+  const int32_t *SideInfo;       /* [nSides][5]                                    */
+  const double  *NodeCoords;     /* [nNodes][3]                                    */
+  const int32_t *GlobalNodeIDs;  /* [nNodes], 1-based unique node ids              */
+  int32_t nBCs;
+  const int32_t *BCType;         /* [nBCs][4] HOPR BCType rows (type,curve,state,alpha) */
+  const int32_t *PartBoundCond;  /* [nBCs] 1 open, 2 reflective, 3 periodic        */
+  int32_t nPeriodicVectors;
+  const double  *PeriodicVectors;/* [nPeriodicVectors][3]                          */
+} om_mesh_desc;
+
+enum { OM_TRACK_REFMAPPING = 1, OM_TRACK_TRIA = 3 };
+enum { OM_DEPO_NONE = 0, OM_DEPO_CVWM = 1, OM_DEPO_SF = 2 };
+
+typedef struct {
+  int32_t tracking;              /* OM_TRACK_*                                     */
+  int32_t deposition;            /* OM_DEPO_*                                      */
+  double  sf_radius;             /* PIC-shapefunction-radius                       */
+  int32_t sf_alpha;              /* PIC-shapefunction-alpha                        */
+  int32_t refmapping_guess;      /* RefMappingGuess (1..4)                         */
+  double  refmapping_eps;        /* RefMappingEps                                  */
+  double  external_field[6];     /* PIC-externalField * PIC-scaleexternalField     */
+  int32_t nSpecies;
+  double  charge[8];             /* Species%ChargeIC                               */
+  double  mass[8];               /* Species%MassIC                                 */
+  double  mpf[8];                /* Species%MacroParticleFactor                    */
+} om_params;
+
+typedef struct om_state om_state;
+
+om_state *om_create(const om_mesh_desc *mesh, const om_params *par);
+void      om_destroy(om_state *s);
+const char *om_last_error(void);
+void      om_set_threads(om_state *s, int nthreads);
+
+int  om_set_field(om_state *s, const double *E);
+int  om_set_particles(om_state *s, int32_t n, const double *PartState,
+                      const int32_t *elem, const int32_t *species, const uint8_t *is_new);
+/* SinglePointToElement for every particle whose element is 0 */
+int  om_locate_particles(om_state *s);
+
+/* The stage, by reference name. */
+int  om_deposition(om_state *s);                     /* Deposition()                  */
+int  om_interpolate_field_to_particle(om_state *s);  /* InterpolateFieldToParticle()  */
+int  om_push_boris_leapfrog(om_state *s, double dt); /* push loop of the 508 stepper  */
+int  om_perform_tracking(om_state *s);               /* PerformTracking()             */
+/* nsteps of: Deposition, LastPartPos <- PartState, interpolate, push, track */
+int  om_time_step(om_state *s, double dt, int32_t nsteps);
+
+int32_t om_get_nparticles(const om_state *s);
+int  om_get_particles(const om_state *s, double *PartState, int32_t *elem, uint8_t *inside);
+int  om_get_last_part_pos(const om_state *s, double *LastPartPos);
+int  om_get_field_at_particle(const om_state *s, double *FieldAtParticle);
+int  om_get_part_source(const om_state *s, double *PS);
+int64_t om_get_lost_count(const om_state *s);
+
+/* Derived tables, for checking the product's own mesh producers. */
+int  om_get_elem_xgp(const om_state *s, double *xgp);        /* [nElems][(N+1)^3][3] */
+int  om_get_sj(const om_state *s, double *sJ);               /* [nElems][(N+1)^3]    */
+int  om_get_elem_side_node_id(const om_state *s, int32_t *id); /* [nElems][6][4] 0-based node */
+int  om_get_concave(const om_state *s, uint8_t *c);          /* [nElems][6]          */
+int  om_get_node_volume(const om_state *s, double *nv);      /* [nUniqueNodes]       */
+int32_t om_get_n_unique_nodes(const om_state *s);
+int  om_get_elem_bary(const om_state *s, double *bary, double *radius);
+int  om_get_xieta_basis(const om_state *s, double *basis, double *slen); /* [nElems][6][3], [nElems][6] */
+
+/* Reference-coordinate mapping of single points (GetPositionInRefElem). */
+int  om_position_in_ref_elem(om_state *s, int32_t n, const double *x, const int32_t *elem,
+                             double *xi, uint8_t *success);
+/* Field evaluation at reference positions (EvaluateFieldAtRefPos) */
+int  om_eval_field_at_ref_pos(om_state *s, int32_t n, const double *xi, const int32_t *elem,
+                              double *U);
+
+/* Analysis: integral of rho (CalcDepositedCharge) and electric field energy (CalcPotentialEnergy) */
+double om_deposited_charge(const om_state *s);
+double om_particle_charge(const om_state *s);
+double om_field_energy(const om_state *s);
+
+/* 1D node / basis helpers (basis.f90), exposed for closed-form checks */
+void om_legendre_gauss_nodes(int32_t N, double *x, double *w);
+void om_cheby_gauss_lobatto_nodes(int32_t N, double *x);
+void om_barycentric_weights(int32_t N, const double *x, double *wb);
+void om_lagrange_polys(double x, int32_t N, const double *xGP, const double *wb, double *L);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
