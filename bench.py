@@ -1,0 +1,125 @@
+"""Throughput of the PIC particle stage on one MI355X.
+
+Workload (the throughput configuration): periodic unit cube of 48^3 linear
+hexahedra, N=3, 200 electrons per element (22,118,400), Maxwellian 2 eV,
+frozen smooth E (8 Fourier modes, |E| <= 1e3 V/m), uniform B = 0.01 T,
+shape_function deposition R = 1.5h, alpha = 2, TriaTracking, FP64.
+A step = Deposition + interpolate + Boris push + TriaTracking (+ element sort),
+inputs resident in HBM.  Prints one JSON line.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "particle-steps/s (interp+push+track+deposit) FP64 vs mixed; % HBM peak"
+PEAK_FP64_TFLOPS = 78.6
+PEAK_HBM_GBS = 8000.0
+
+
+def sf_flops_per_particle(R_over_h, rad_over_h, dofs_per_elem):
+    """Algorithmic FP64 flops of one shape-function deposit of one particle
+    (the reference's algorithm, depoChargeOnDOFsSF): every DOF of every element
+    whose barycentre lies within R + ElemRadius is distance-tested (8 flops),
+    and every DOF within R receives S^2 * (q v, q) (4 flops)."""
+    cand = dofs_per_elem * 4.0 / 3.0 * math.pi * (R_over_h + rad_over_h) ** 3
+    inrad = dofs_per_elem * 4.0 / 3.0 * math.pi * R_over_h ** 3
+    return 8.0 * cand + 4.0 * inrad, cand, inrad
+
+
+def cpu_baseline(w, fraction, steps, threads):
+    """The FP64 CPU oracle (the reference's algorithm restated in C, OpenMP)
+    on a bounded sample of the same workload."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import oracle as O
+    n = int(len(w.PartState) * fraction)
+    idx = np.arange(n)
+    o = O.Oracle(w.mesh, w.N, deposition=w.deposition, sf_radius=w.sf_radius, sf_alpha=w.sf_alpha,
+                 external_field=w.external_field, charge=w.charge, mass=w.mass, mpf=w.mpf)
+    o.set_threads(threads)
+    o.set_field(w.E)
+    o.set_particles(w.PartState[idx], elem=w.elem[idx], species=w.species[idx])
+    t0 = time.perf_counter()
+    o.time_step(w.dt, steps)
+    t = time.perf_counter() - t0
+    return {"value": n * steps / t, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "sample": "%d of %d particles (every element's mesh, same field), %d steps, %.1f s wall"
+                      % (n, len(w.PartState), steps, t)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--nx", type=int, default=48)
+    ap.add_argument("--ppc", type=int, default=200)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-fraction", type=float, default=1.0 / 32)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU; multi-GPU runs are not part of it (got --gpus %d)" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+
+    from piclas_amd import stage, workloads as W
+    w = W.config2(nx=args.nx, ppc=args.ppc)
+    n = len(w.PartState)
+    print("bench: %s, %d particles, %d warmup + %d timed steps" % (w.name, n, args.warmup, args.steps),
+          file=sys.stderr, flush=True)
+    s = stage.PICStage(w.mesh, w.N, max_particles=n, **W.stage_kwargs(w))
+    s.set_field(w.E)
+    s.set_particles(w.PartState, elem=w.elem, species=w.species)
+    s.synchronize()
+    if args.warmup:
+        s.time_step(w.dt, args.warmup)
+    s.synchronize()
+    s.reset_stats()
+    t0 = time.perf_counter()
+    s.time_step(w.dt, args.steps)
+    s.synchronize()
+    t = time.perf_counter() - t0
+    st = s.stats()
+    n_now = st["n_particles"]
+    value = n * args.steps / t
+
+    # roofline of the dominant kernel (shape-function gather), timed with HIP
+    # events on the stage's own stream
+    h = 1.0 / args.nx
+    fpp, cand, inrad = sf_flops_per_particle(w.sf_radius / h, math.sqrt(3) / 2, (w.N + 1) ** 3)
+    avg_ms = st["ms_deposit_main"] / max(st["n_deposit_launches"], 1)
+    achieved = fpp * n_now / (avg_ms * 1e-3) / 1e12
+    roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": None,
+            "kernel": "k_sf_gather<3>", "avg_launch_ms": round(avg_ms, 4),
+            "algorithmic_flops_per_particle": round(fpp, 1),
+            "note": "FP64 vector ALU bound (no MFMA); peak = MI355X FP64 vector peak"}
+    ms_step = t / args.steps * 1e3
+    out = {"metric": METRIC, "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": w.name, "elements": args.nx ** 3, "N": w.N, "particles": n,
+                      "deposition": "shape_function R=1.5h alpha=2", "tracking": "triatracking",
+                      "parallelism": "1 GPU"},
+           "stage_ms_per_step": {"deposit": st["ms_deposit"] / args.steps,
+                                 "interp_push_track": st["ms_push_track"] / args.steps,
+                                 "sort": st["ms_sort"] / args.steps},
+           "roofline": roof}
+    s.close()
+    if not args.no_cpu_baseline:
+        threads = min(16, os.cpu_count() or 1)
+        out["cpu_baseline"] = cpu_baseline(w, args.cpu_fraction, args.cpu_steps, threads)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,148 @@
+/*
+ * pic_stage.h -- C ABI of the MI355X PIC particle stage.
+ *
+ * The stage replaces, for the Boris-Leapfrog / HDG-Poisson time stepper
+ * (reference: src/timedisc/timedisc_TimeStepPoissonByBorisLeapfrog.f90:32-286),
+ * the four particle routines the stepper calls:
+ *
+ *   Deposition()                    src/particles/pic/deposition/pic_depo.f90:944-1018
+ *   InterpolateFieldToParticle()    src/particles/pic/interpolation/pic_interpolation.f90:248-322
+ *   push loop (inline in stepper)   timedisc_TimeStepPoissonByBorisLeapfrog.f90:128-198
+ *   PerformTracking()               src/particles/tracking/particle_tracking.f90:32-70
+ *
+ * In the reference these are Fortran module procedures that communicate
+ * through module arrays (PartState, PEM, PDM, U_N, PS_N, FieldAtParticle).
+ * Here the same arrays cross the boundary explicitly: plain pointers with the
+ * reference's Fortran memory layout, so a Fortran host binds them with
+ * ISO_C_BINDING and passes its own arrays unchanged:
+ *
+ *   mesh: ElemInfo(6,nElems), SideInfo(5,nSides), NodeCoords(3,nNodes),
+ *         GlobalNodeIDs(nNodes), BCType(4,nBCs)   (HOPR file arrays)
+ *   U_N(iElem)%E(1:3,0:N,0:N,0:N)        -> double E[nElems][(N+1)^3][3]
+ *   PS_N(iElem)%PartSource(1:4,0:N,...)  -> double PS[nElems][(N+1)^3][4]
+ *   PartState(1:6,nPart)                 -> double PartState[nPart][6]
+ *   FieldAtParticle(1:6,nPart)           -> double F[nPart][6]
+ *   PEM%GlobalElemID(nPart)              -> int32_t elem[nPart]  (1-based; 0 = unknown)
+ *   PartSpecies(nPart)                   -> int32_t species[nPart] (1-based)
+ *   PDM%IsNewPart / ParticleInside       -> uint8_t flags
+ *
+ * The library owns its device memory; host buffers stay the caller's and are
+ * copied in and out.  All calls are synchronous on the handle's HIP stream
+ * unless the name says _async.  Every entry point returns 0 on success and a
+ * negative PIC_E* code on failure; pic_last_error() returns the message (the
+ * reference aborts instead, src/globals/globals.f90:435).  A handle must not
+ * be used from two threads at once (the reference procedures are not
+ * reentrant either).
+ */
+#ifndef PIC_STAGE_H
+#define PIC_STAGE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  PIC_OK = 0,
+  PIC_EINVAL = -1,     /* invalid argument or unsupported configuration   */
+  PIC_EHIP = -2,       /* HIP runtime error                               */
+  PIC_ECAPACITY = -3,  /* more particles than the handle was created for  */
+  PIC_ESTATE = -4,     /* call out of order (e.g. no particles set)       */
+  PIC_ETRACK = -5      /* tracking hit a state the reference aborts on    */
+};
+
+enum { PIC_TRACK_REFMAPPING = 1, PIC_TRACK_TRIA = 3 };         /* TrackingMethod, src/piclas.h:357-359 */
+enum { PIC_DEPO_NONE = 0, PIC_DEPO_CVWM = 1, PIC_DEPO_SF = 2 }; /* PIC-Deposition-Type                  */
+enum { PIC_BC_OPEN = 1, PIC_BC_REFLECTIVE = 2, PIC_BC_PERIODIC = 3 }; /* PartBound%TargetBoundCond      */
+enum { PIC_PREC_FP64 = 0, PIC_PREC_MIXED = 1 };
+
+/* Mesh in HOPR layout, as the reference's mesh reader holds it
+ * (src/mesh/mesh_readin.f90; index constants src/piclas.h:136-176). */
+typedef struct {
+  int32_t nElems, nSides, nNodes;
+  int32_t NGeo;                   /* geometry degree (1: linear hexahedra)        */
+  int32_t N;                      /* DG polynomial degree of E and PartSource     */
+  const int32_t *ElemInfo;        /* [nElems][6]                                  */
+  const int32_t *SideInfo;        /* [nSides][5]                                  */
+  const double  *NodeCoords;      /* [nNodes][3]                                  */
+  const int32_t *GlobalNodeIDs;   /* [nNodes] 1-based unique node ids             */
+  int32_t nBCs;
+  const int32_t *BCType;          /* [nBCs][4] (type, curve, state, alpha)        */
+  const int32_t *PartBoundCond;   /* [nBCs] PIC_BC_*  (Part-BoundaryX-Condition)  */
+  int32_t nPeriodicVectors;
+  const double  *PeriodicVectors; /* [nPeriodicVectors][3]  (GEO%PeriodicVectors) */
+} pic_mesh_desc;
+
+/* Run-time parameters the reference reads from parameter.ini. */
+typedef struct {
+  int32_t tracking;               /* TrackingMethod (PIC_TRACK_TRIA)              */
+  int32_t deposition;             /* PIC-Deposition-Type                          */
+  double  sf_radius;              /* PIC-shapefunction-radius                     */
+  int32_t sf_alpha;               /* PIC-shapefunction-alpha                      */
+  int32_t refmapping_guess;       /* RefMappingGuess                              */
+  double  refmapping_eps;         /* RefMappingEps                                */
+  double  external_field[6];      /* PIC-externalField * PIC-scaleexternalField   */
+  int32_t nSpecies;               /* <= 8                                         */
+  double  charge[8];              /* Part-SpeciesX-ChargeIC                       */
+  double  mass[8];                /* Part-SpeciesX-MassIC                         */
+  double  mpf[8];                 /* Part-SpeciesX-MacroParticleFactor            */
+  int32_t precision;              /* PIC_PREC_*                                   */
+  int64_t max_particles;          /* Part-maxParticleNumber                       */
+  int32_t device;                 /* HIP device ordinal                           */
+} pic_params;
+
+typedef struct {
+  int64_t n_particles;            /* particles currently inside the domain        */
+  int64_t n_lost;                 /* particles removed by tracking (cumulative)    */
+  int64_t n_steps;                /* time steps performed (cumulative)            */
+  double  ms_deposit;             /* device time in deposition kernels            */
+  double  ms_push_track;          /* device time in interpolate+push+track        */
+  double  ms_sort;                /* device time in the element sort              */
+  int32_t n_deposit_launches;     /* launches of the dominant deposition kernel   */
+  double  ms_deposit_main;        /* device time of that kernel alone             */
+} pic_stats;
+
+typedef struct pic_handle pic_handle;
+
+/* Set-up / tear-down (InitParticleMesh, InitializeDeposition, InitPIC). */
+int  pic_create(const pic_mesh_desc *mesh, const pic_params *par, pic_handle **out);
+int  pic_destroy(pic_handle *h);
+const char *pic_last_error(void);
+
+/* Field: U_N%E (host array).  pic_set_field_device takes a device pointer of
+ * the same layout and copies device-to-device on the handle's stream. */
+int  pic_set_field(pic_handle *h, const double *E);
+int  pic_set_field_device(pic_handle *h, const double *E_dev);
+
+/* Particles (PartState, PEM%GlobalElemID, PartSpecies, PDM%IsNewPart).
+ * elem may be NULL or hold zeros: the stage then locates the particles
+ * (SinglePointToElement, src/particles/tracking/particle_localization.f90:81-203). */
+int  pic_set_particles(pic_handle *h, int64_t n, const double *PartState, const int32_t *elem,
+                       const int32_t *species, const uint8_t *is_new);
+
+/* The four stage routines, by reference name. */
+int  pic_deposition(pic_handle *h);                          /* Deposition()                 */
+int  pic_interpolate_field_to_particle(pic_handle *h);       /* InterpolateFieldToParticle() */
+int  pic_push_boris_leapfrog(pic_handle *h, double dt);      /* stepper push loop            */
+int  pic_perform_tracking(pic_handle *h);                    /* PerformTracking()            */
+
+/* nsteps of TimeStepPoissonByBorisLeapfrog with the field held fixed:
+ * Deposition; LastPartPos <- PartState; interpolate; push; track. */
+int  pic_time_step(pic_handle *h, double dt, int32_t nsteps);
+
+/* Results, in the caller's original particle order. */
+int  pic_get_particles(pic_handle *h, double *PartState, int32_t *elem, uint8_t *inside);
+int  pic_get_last_part_pos(pic_handle *h, double *LastPartPos);
+int  pic_get_field_at_particle(pic_handle *h, double *FieldAtParticle);
+int  pic_get_part_source(pic_handle *h, double *PS);
+/* Device pointer to PS[nElems][(N+1)^3][4], valid until the next deposition. */
+int  pic_part_source_device(pic_handle *h, const double **PS_dev);
+int  pic_get_stats(pic_handle *h, pic_stats *st);
+int  pic_reset_stats(pic_handle *h);
+int  pic_synchronize(pic_handle *h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -503,7 +503,7 @@ static int build_mesh(om_state *s, const om_mesh_desc *m) {
     }
     for (int u = 0; u < nu; u++) {
       s->perOff[u] = cnt;
-      for (int comb = 1; comb < ncomb; comb++) {
+      for (int comb = 0; comb < ncomb; comb++) {
         int cc = comb, coef[3] = {0, 0, 0};
         for (int v = 0; v < s->nPV; v++) { coef[v] = cc % 3 - 1; cc /= 3; }
         if (coef[0] == 0 && coef[1] == 0 && coef[2] == 0) continue;
@@ -801,23 +801,25 @@ static void eval_field_at_ref_pos(const om_state *s, const double xi[3], int e, 
   }
 }
 
-/* GetEMFieldDW, pic_interpolation_tools.f90:458-572 (E only) */
+/* GetEMFieldDW, pic_interpolation_tools.f90:458-572 (E only).  The triple
+ * loop is written DO k; DO l; DO m, so the EXIT taken when the particle sits
+ * on a Gauss point leaves only the m-loop: the weights of the points visited
+ * before are zeroed, that point gets 1, and the loop goes on with the next l. */
 static void em_field_dw(const om_state *s, int e, const double x[3], double U[3]) {
   const int n1 = s->N + 1, nd = s->nDof;
   double dist[MAXN * MAXN * MAXN];
   double sum = 0.0;
   for (int q = 0; q < nd; q++) dist[q] = 0.0;
   const double *xg = &s->xGPe[(size_t)e * nd * 3];
-  /* loops k,l,m over Elem_xGP(1:3,k,l,m): k is the first (fastest) index */
-  int done = 0;
-  for (int k = 0; k < n1 && !done; k++) for (int l = 0; l < n1 && !done; l++) for (int m = 0; m < n1; m++) {
+  for (int k = 0; k < n1; k++) for (int l = 0; l < n1; l++) for (int m = 0; m < n1; m++) {
     int r = (m * n1 + l) * n1 + k;
     double d[3] = {xg[r * 3] - x[0], xg[r * 3 + 1] - x[1], xg[r * 3 + 2] - x[2]};
     double norm = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
     if (norm > 0.) { dist[r] = 1. / norm; }
     else {
       for (int q = 0; q < nd; q++) dist[q] = 0.0;
-      dist[r] = 1.0; sum = 1.0; done = 1; break;
+      dist[r] = 1.0; sum = 1.0;
+      break;
     }
     sum = sum + dist[r];
   }
@@ -1333,10 +1335,17 @@ static void push_one(om_state *s, int i, double dt) {
   st[2] = st[2] + st[5] * dtFrac;
 }
 
+/* LastPartPos <- PartState and LastGlobalElemID <- GlobalElemID (stepper
+ * lines 109-110), then the push loop (lines 128-198). */
 int om_push_boris_leapfrog(om_state *s, double dt) {
 #pragma omp parallel for num_threads(s->nthreads) schedule(static)
-  for (int i = 0; i < s->n; i++)
+  for (int i = 0; i < s->n; i++) {
+    s->LastPartPos[i * 3 + 0] = s->PartState[i * 6 + 0];
+    s->LastPartPos[i * 3 + 1] = s->PartState[i * 6 + 1];
+    s->LastPartPos[i * 3 + 2] = s->PartState[i * 6 + 2];
+    s->lastElem[i] = s->elem[i];
     if (s->inside[i]) push_one(s, i, dt);
+  }
   return 0;
 }
 
@@ -1540,12 +1549,6 @@ int om_time_step(om_state *s, double dt, int32_t nsteps) {
   for (int it = 0; it < nsteps; it++) {
     if (om_deposition(s) != 0) return -1;
     /* (HDG field solve: outside the stage, E frozen) */
-    for (int i = 0; i < s->n; i++) {
-      s->LastPartPos[i * 3 + 0] = s->PartState[i * 6 + 0];
-      s->LastPartPos[i * 3 + 1] = s->PartState[i * 6 + 1];
-      s->LastPartPos[i * 3 + 2] = s->PartState[i * 6 + 2];
-      s->lastElem[i] = s->elem[i];
-    }
     om_interpolate_field_to_particle(s);
     om_push_boris_leapfrog(s, dt);
     int err = 0;

@@ -1,0 +1,566 @@
+// Device-side views and per-particle routines of the particle stage
+// (gfx950).  Every routine restates one routine of the reference (paths
+// relative to the reference's src/) with the same operation order; the
+// whole translation unit is compiled without floating-point contraction so
+// that the geometric predicates of the tracking round exactly like the
+// reference's expressions.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace pic {
+
+struct DevMesh {
+  int nElems, N, nDof, deposition, refGuess, nSpecies;
+  double refEps;
+  // tracking
+  const double *corner;     // [e][8][3] tensor corner order
+  const int32_t *sideNb;    // [e][6]
+  const int32_t *sideGID;   // [e][6]
+  const uint8_t *sideNode;  // [e][6][4]
+  const uint8_t *sideBC;    // [e][6]
+  const int8_t *sidePV;     // [e][6]
+  const uint8_t *concave;   // [e]
+  // reference mapping
+  const double *dXCL;       // [e][8][3][3]
+  const double *bary;       // [e][3]
+  const double *radius;     // [e]
+  const double *xiBasis;    // [e][6][3]
+  const double *slen;       // [e][6]
+  const double *xGPe;       // [e][nDof][3]
+  const double *E;          // [e][nDof][3]
+  double xiCL[2], wBaryCL[2];
+  double xGP[8], wBaryGP[8];
+  double PV[3][3];
+  double ext[6];
+  double charge[8], mass[8], mpf[8];
+};
+
+struct DevParts {
+  double *x, *y, *z, *vx, *vy, *vz;
+  int32_t *elem;    // owner element (0-based), -1 = lost
+  int32_t *id;      // caller's particle index
+  uint8_t *spec;    // species (0-based)
+  uint8_t *isNew;   // PDM%IsNewPart
+  double *lx, *ly, *lz;   // LastPartPos
+  int32_t *lastElem;      // PEM%LastGlobalElemID
+  double *F;              // FieldAtParticle [n][6]
+};
+
+constexpr double kEpsMach = 2.220446049250313e-16;
+
+// c^2 and 1/c^2 of globals_vars.f90:87-89
+__device__ inline double c2_inv() {
+  const double c = 299792458.0;
+  return 1.0 / (c * c);
+}
+
+// ALMOSTEQUAL_UNITY, interpolation/basis.f90:1011-1022
+__device__ inline bool almost_equal_unity(double x, double y) {
+  if (x == 0.0 || y == 0.0) return fabs(x - y) <= 2.0 * kEpsMach;
+  return (fabs(x - y) <= kEpsMach * fabs(x)) && (fabs(x - y) <= kEpsMach * fabs(y));
+}
+
+// LagrangeInterpolationPolys, interpolation/basis.f90:1223-1264
+template <int NP>
+__device__ inline void lagrange(double x, const double *xg, const double *wb, double *L) {
+  bool eq = false;
+#pragma unroll
+  for (int i = 0; i < NP; ++i) {
+    L[i] = 0.0;
+    if (almost_equal_unity(x, xg[i])) { L[i] = 1.0; eq = true; }
+  }
+  if (eq) return;
+  double sum = 0.0;
+#pragma unroll
+  for (int i = 0; i < NP; ++i) { L[i] = wb[i] / (x - xg[i]); sum = sum + L[i]; }
+#pragma unroll
+  for (int i = 0; i < NP; ++i) L[i] = L[i] / sum;
+}
+
+// getDet / getInv, interpolation/eval_xyz.f90:448-495
+__device__ inline double get_det(const double M[3][3]) {
+  return (M[0][0] * M[1][1] - M[0][1] * M[1][0]) * M[2][2] + (M[0][1] * M[1][2] - M[0][2] * M[1][1]) * M[2][0] +
+         (M[0][2] * M[1][0] - M[0][0] * M[1][2]) * M[2][1];
+}
+
+// RefElemNewton for NGeo=1, interpolation/eval_xyz.f90:298-445.
+// mode 1 = ForceMode; with wantSuccess the routine reports failure instead
+// of aborting.  Returns isSuccessful.
+__device__ inline bool ref_elem_newton(const DevMesh &M, int e, const double xin[3], double xi[3], bool wantSuccess) {
+  const double *X = M.corner + (size_t)e * 24;
+  const double *dX = M.dXCL + (size_t)e * 72;
+  double L[3][2], F[3], xo[3];
+  bool ok = true;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) lagrange<2>(xi[d], M.xiCL, M.wBaryCL, L[d]);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) F[c] = -xin[c];
+#pragma unroll
+  for (int k = 0; k < 2; ++k)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const double buff = L[1][j] * L[2][k];
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) F[c] = F[c] + X[(k * 4 + j * 2 + i) * 3 + c] * L[0][i] * buff;
+    }
+  double deltaXi2 = (fabs(F[0]) < kEpsMach && fabs(F[1]) < kEpsMach && fabs(F[2]) < kEpsMach) ? 0.0 : 1.0;
+  double NormF = F[0] * F[0] + F[1] * F[1] + F[2] * F[2];
+  double NormF_old;
+  int it = 0;
+  while (deltaXi2 > M.refEps && it < 100) {
+    ++it;
+    double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+#pragma unroll
+    for (int k = 0; k < 2; ++k)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const double buff = L[1][j] * L[2][k];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+          const double b2 = L[0][i] * buff;
+          const double *dp = dX + (k * 4 + j * 2 + i) * 9;
+#pragma unroll
+          for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int b = 0; b < 3; ++b) J[a][b] = J[a][b] + dp[a * 3 + b] * b2;
+        }
+      }
+    double sdet = get_det(J);
+    if (sdet > 0.) sdet = 1. / sdet;
+    double S[3][3];
+    S[0][0] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]) * sdet;
+    S[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * sdet;
+    S[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * sdet;
+    S[1][0] = (J[1][2] * J[2][0] - J[1][0] * J[2][2]) * sdet;
+    S[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * sdet;
+    S[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * sdet;
+    S[2][0] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) * sdet;
+    S[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * sdet;
+    S[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * sdet;
+    double dxi[3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) dxi[r] = S[r][0] * F[0] + S[r][1] * F[1] + S[r][2] * F[2];
+    deltaXi2 = dxi[0] * dxi[0] + dxi[1] * dxi[1] + dxi[2] * dxi[2];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) xo[d] = xi[d];
+    NormF_old = NormF;
+    NormF = NormF * 2.;
+    double lambda = 1.0;
+    int iArmijo = 1;
+    while (NormF > NormF_old * (1. - 0.0001 * lambda) && iArmijo <= 8) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) xi[d] = xo[d] - lambda * dxi[d];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) lagrange<2>(xi[d], M.xiCL, M.wBaryCL, L[d]);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) F[c] = -xin[c];
+#pragma unroll
+      for (int k = 0; k < 2; ++k)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          const double buff = L[1][j] * L[2][k];
+#pragma unroll
+          for (int i = 0; i < 2; ++i) {
+            const double b2 = L[0][i] * buff;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) F[c] = F[c] + X[(k * 4 + j * 2 + i) * 3 + c] * b2;
+          }
+        }
+      lambda = 0.2 * lambda;
+      iArmijo = iArmijo + 1;
+      NormF = F[0] * F[0] + F[1] * F[1] + F[2] * F[2];
+    }
+    if (fabs(xi[0]) > 1.5 || fabs(xi[1]) > 1.5 || fabs(xi[2]) > 1.5) {
+      if (wantSuccess) ok = false;
+      break;
+    }
+  }
+  return ok;
+}
+
+// GetRefNewtonStartValue, interpolation/eval_xyz.f90:498-612 (guesses 1, 2, 4)
+template <int N>
+__device__ inline void ref_newton_start(const DevMesh &M, int e, const double x[3], double xi[3]) {
+  if (M.refGuess == 1) {
+    const double *b0 = M.bary + (size_t)e * 3;
+    const double P[3] = {x[0] - b0[0], x[1] - b0[1], x[2] - b0[2]};
+    double XL[6];
+#pragma unroll
+    for (int d = 0; d < 6; ++d) {
+      const double *b = M.xiBasis + ((size_t)e * 6 + d) * 3;
+      XL[d] = (P[0] * b[0] + P[1] * b[1] + P[2] * b[2]) * M.slen[(size_t)e * 6 + d];
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) xi[d] = 0.5 * (XL[d] - XL[d + 3]);
+    const double mx = fmax(fabs(xi[0]), fmax(fabs(xi[1]), fabs(xi[2])));
+    if (mx > 1.0 + M.refEps)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) xi[d] = fmax(fmin(1.0, xi[d]), -1.0);
+  } else if (M.refGuess == 2) {
+    constexpr int n1 = N + 1;
+    const double *xg = M.xGPe + (size_t)e * n1 * n1 * n1 * 3;
+    const double d0[3] = {x[0] - xg[0], x[1] - xg[1], x[2] - xg[2]};
+    double win = sqrt(d0[0] * d0[0] + d0[1] * d0[1] + d0[2] * d0[2]);
+    xi[0] = xi[1] = xi[2] = M.xGP[0];
+    for (int i = 0; i < n1; ++i)
+      for (int j = 0; j < n1; ++j)
+        for (int k = 0; k < n1; ++k) {
+          const int r = (k * n1 + j) * n1 + i;
+          const double dX = fabs(x[0] - xg[r * 3]);
+          if (dX > win) continue;
+          const double dY = fabs(x[1] - xg[r * 3 + 1]);
+          if (dY > win) continue;
+          const double dZ = fabs(x[2] - xg[r * 3 + 2]);
+          if (dZ > win) continue;
+          const double dist = sqrt(dX * dX + dY * dY + dZ * dZ);
+          if (dist < win) { win = dist; xi[0] = M.xGP[i]; xi[1] = M.xGP[j]; xi[2] = M.xGP[k]; }
+        }
+  } else {
+    xi[0] = xi[1] = xi[2] = 0.0;
+  }
+}
+
+// EvaluateFieldAtRefPos with NVar=3 (interpolation/eval_xyz.f90:167-295)
+template <int N>
+__device__ inline void eval_field(const DevMesh &M, int e, const double xi[3], double U[3]) {
+  constexpr int n1 = N + 1;
+  double L[3][n1];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) lagrange<n1>(xi[d], M.xGP, M.wBaryGP, L[d]);
+  const double *Ue = M.E + (size_t)e * n1 * n1 * n1 * 3;
+  U[0] = U[1] = U[2] = 0.0;
+#pragma unroll
+  for (int k = 0; k < n1; ++k)
+#pragma unroll
+    for (int j = 0; j < n1; ++j) {
+      const double Lez = L[1][j] * L[2][k];
+#pragma unroll
+      for (int i = 0; i < n1; ++i) {
+        const double *u = Ue + ((k * n1 + j) * n1 + i) * 3;
+        U[0] = U[0] + u[0] * L[0][i] * Lez;
+        U[1] = U[1] + u[1] * L[0][i] * Lez;
+        U[2] = U[2] + u[2] * L[0][i] * Lez;
+      }
+    }
+}
+
+// GetEMFieldDW (E only), particles/pic/interpolation/pic_interpolation_tools.f90:458-572.
+// The reference's loop nest is DO k; DO l; DO m, and the EXIT taken when the
+// particle sits on a Gauss point leaves only the m-loop: weights of points
+// visited before are zeroed, that point weighs 1, the rest of its m-row is
+// skipped and the loop continues with the next l.  Two passes restate that
+// without a per-point weight array.
+template <int N>
+__device__ inline void em_field_dw(const DevMesh &M, int e, const double x[3], double U[3]) {
+  constexpr int n1 = N + 1, nd = n1 * n1 * n1;
+  const double *xg = M.xGPe + (size_t)e * nd * 3;
+  const double *Ue = M.E + (size_t)e * nd * 3;
+  auto norm_of = [&](int r) {
+    const double d0 = xg[r * 3] - x[0], d1 = xg[r * 3 + 1] - x[1], d2 = xg[r * 3 + 2] - x[2];
+    return sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+  };
+  // pass 1: DistSum and the (last) Gauss point hit, in loop order
+  double sum = 0.0;
+  int hitK = -1, hitL = -1, hitM = -1;
+  for (int k = 0; k < n1; ++k)
+    for (int l = 0; l < n1; ++l)
+      for (int m = 0; m < n1; ++m) {
+        const double nrm = norm_of((m * n1 + l) * n1 + k);
+        if (nrm > 0.) sum = sum + 1. / nrm;
+        else { sum = 1.0; hitK = k; hitL = l; hitM = m; break; }
+      }
+  // pass 2: weights; before the hit they are zero, in the hit's m-row after it too
+  U[0] = U[1] = U[2] = 0.0;
+  for (int k = 0; k < n1; ++k)
+    for (int l = 0; l < n1; ++l)
+      for (int m = 0; m < n1; ++m) {
+        const int r = (m * n1 + l) * n1 + k;
+        double w;
+        if (hitK < 0) w = 1. / norm_of(r);
+        else {
+          const int ord = (k * n1 + l) * n1 + m, hord = (hitK * n1 + hitL) * n1 + hitM;
+          if (ord < hord) w = 0.0;
+          else if (ord == hord) w = 1.0;
+          else if (k == hitK && l == hitL) w = 0.0;
+          else { const double nrm = norm_of(r); w = nrm > 0. ? 1. / nrm : 1.0; }
+        }
+        U[0] = U[0] + w / sum * Ue[r * 3 + 0];
+        U[1] = U[1] + w / sum * Ue[r * 3 + 1];
+        U[2] = U[2] + w / sum * Ue[r * 3 + 2];
+      }
+}
+
+// InterpolateFieldToSingleParticle for the 508 stepper
+// (particles/pic/interpolation/pic_interpolation.f90:325-369):
+// external field + E(xi) with xi from GetPositionInRefElem.
+template <int N>
+__device__ inline void field_at_particle(const DevMesh &M, int e, const double x[3], double F[6]) {
+#pragma unroll
+  for (int c = 0; c < 6; ++c) F[c] = M.ext[c];
+  double xi[3];
+  ref_newton_start<N>(M, e, x, xi);
+  const bool ok = ref_elem_newton(M, e, x, xi, true);
+  double U[3];
+  if (!ok && M.deposition == 1) em_field_dw<N>(M, e, x, U);
+  else eval_field<N>(M, e, xi, U);
+  F[0] = F[0] + U[0];
+  F[1] = F[1] + U[1];
+  F[2] = F[2] + U[2];
+}
+
+// Boris-Leapfrog push of one particle,
+// timedisc/timedisc_TimeStepPoissonByBorisLeapfrog.f90:128-198 (with the
+// IsNewPart half step through PartRHS_NR, particles/particle_rhs.f90:203-253)
+__device__ inline void boris_push(double q, double m, bool isNew, const double F[6], double dt, double x[3], double v[3]) {
+  const bool push = fabs(q) > 0.0;
+  if (isNew && push) {
+    const double qmt = q / m;
+    const double Pt[3] = {F[0] * qmt, F[1] * qmt, F[2] * qmt};
+    v[0] = v[0] - Pt[0] * dt * 0.5;
+    v[1] = v[1] - Pt[1] * dt * 0.5;
+    v[2] = v[2] - Pt[2] * dt * 0.5;
+  }
+  if (push) {
+    const double c2i = c2_inv();
+    const double c_1 = (q * dt) / (m * 2.);
+    const double gamma = 1. / sqrt(1 - ((v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) * c2i));
+    const double vmo[3] = {v[0] * gamma, v[1] * gamma, v[2] * gamma};
+    const double vm[3] = {vmo[0] + c_1 * F[0], vmo[1] + c_1 * F[1], vmo[2] + c_1 * F[2]};
+    const double gm = sqrt(1 + (vm[0] * vm[0] + vm[1] * vm[1] + vm[2] * vm[2]) * c2i);
+    const double Bn = sqrt(F[3] * F[3] + F[4] * F[4] + F[5] * F[5]);
+    double ub[3] = {0.0, 0.0, 0.0};
+    double invL = sqrt(F[3] * F[3] + F[4] * F[4] + F[5] * F[5]);
+    if (fabs(invL) > 0.0) { invL = 1. / invL; ub[0] = F[3] * invL; ub[1] = F[4] * invL; ub[2] = F[5] * invL; }
+    const double tf = tan(c_1 / gm * Bn);
+    const double t[3] = {tf * ub[0], tf * ub[1], tf * ub[2]};
+    const double vp[3] = {vm[0] + (vm[1] * t[2] - vm[2] * t[1]), vm[1] + (vm[2] * t[0] - vm[0] * t[2]),
+                          vm[2] + (vm[0] * t[1] - vm[1] * t[0])};
+    const double f2 = 2.0 / (1. + (t[0] * t[0] + t[1] * t[1] + t[2] * t[2]));
+    const double cr[3] = {vp[1] * t[2] - vp[2] * t[1], vp[2] * t[0] - vp[0] * t[2], vp[0] * t[1] - vp[1] * t[0]};
+    const double vpl[3] = {vm[0] + f2 * cr[0], vm[1] + f2 * cr[1], vm[2] + f2 * cr[2]};
+    const double vn1[3] = {vpl[0] + c_1 * F[0], vpl[1] + c_1 * F[1], vpl[2] + c_1 * F[2]};
+    const double den = sqrt(1 + (vn1[0] * vn1[0] + vn1[1] * vn1[1] + vn1[2] * vn1[2]) * c2i);
+    v[0] = vn1[0] / den;
+    v[1] = vn1[1] / den;
+    v[2] = vn1[2] / den;
+  }
+  x[0] = x[0] + v[0] * dt;
+  x[1] = x[1] + v[1] * dt;
+  x[2] = x[2] + v[2] * dt;
+}
+
+// ParticleInsideQuad3D, particles/particle_mesh/particle_mesh_tools.f90:78-222
+__device__ inline bool inside_quad(const DevMesh &M, int e, const double x[3], double det[6][2]) {
+  const double *C = M.corner + (size_t)e * 24;
+  const uint8_t cc = M.concave[e];
+  bool in = true;
+#pragma unroll
+  for (int l = 0; l < 6; ++l) {
+    const uint8_t *sn = M.sideNode + ((size_t)e * 6 + l) * 4;
+    double A[3][4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) A[c][q] = C[sn[q] * 3 + c] - x[c];
+    const double cp0 = A[1][0] * A[2][2] - A[2][0] * A[1][2];
+    const double cp1 = A[2][0] * A[0][2] - A[0][0] * A[2][2];
+    const double cp2 = A[0][0] * A[1][2] - A[1][0] * A[0][2];
+    double d1 = cp0 * A[0][1] + cp1 * A[1][1] + cp2 * A[2][1];
+    d1 = -d1;
+    const double d2 = cp0 * A[0][3] + cp1 * A[1][3] + cp2 * A[2][3];
+    det[l][0] = d1;
+    det[l][1] = d2;
+    const bool neg = (d1 < 0) || (d2 < 0);
+    const bool pos = !(d1 < 0) || !(d2 < 0);
+    if ((cc >> l) & 1) { if (!pos) in = false; }
+    else { if (neg) in = false; }
+  }
+  return in;
+}
+
+// ParticleThroughSideCheck3DFast, particles/tracking/particle_intersection.f90:167-280
+__device__ inline bool through_side_fast(const DevMesh &M, int e, int l, int tri, const double P[3], const double V[3]) {
+  const double *C = M.corner + (size_t)e * 24;
+  const uint8_t *sn = M.sideNode + ((size_t)e * 6 + l) * 4;
+  double Ax[3], Ay[3], Az[3];
+  Ax[0] = C[sn[0] * 3 + 0] - P[0];
+  Ay[0] = C[sn[0] * 3 + 1] - P[1];
+  Az[0] = C[sn[0] * 3 + 2] - P[2];
+#pragma unroll
+  for (int n = 1; n < 3; ++n) {
+    const int node = n + tri - 1;  // 0-based side node for n = 1,2
+    Ax[n] = C[sn[node] * 3 + 0] - P[0];
+    Ay[n] = C[sn[node] * 3 + 1] - P[1];
+    Az[n] = C[sn[node] * 3 + 2] - P[2];
+  }
+  double dc[3][3];
+  dc[0][0] = (Ay[0] * V[2] - Az[0] * V[1]) * Ax[2];
+  dc[0][1] = (Az[0] * V[0] - Ax[0] * V[2]) * Ay[2];
+  dc[0][2] = (Ax[0] * V[1] - Ay[0] * V[0]) * Az[2];
+  dc[1][0] = (Ay[1] * V[2] - Az[1] * V[1]) * Ax[0];
+  dc[1][1] = (Az[1] * V[0] - Ax[1] * V[2]) * Ay[0];
+  dc[1][2] = (Ax[1] * V[1] - Ay[1] * V[0]) * Az[0];
+  dc[2][0] = (Ay[2] * V[2] - Az[2] * V[1]) * Ax[1];
+  dc[2][1] = (Az[2] * V[0] - Ax[2] * V[2]) * Ay[1];
+  dc[2][2] = (Ax[2] * V[1] - Ay[2] * V[0]) * Az[1];
+  bool through = true;
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    const double det = dc[r][0] + dc[r][1] + dc[r][2];
+    double mn = 1.7976931348623157e308;  // MINVAL over an empty mask is HUGE
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const double a = fabs(dc[r][c]);
+      if (a > 0.0 && a < mn) mn = a;
+    }
+    if (!(det >= -kEpsMach * mn)) through = false;
+  }
+  return through;
+}
+
+// ParticleThroughSideLastPosCheck (regular side), particle_intersection.f90:430-512
+__device__ inline bool through_side_last_pos(const DevMesh &M, int e, int l, int tri, const double P[3], double &det) {
+  const double *C = M.corner + (size_t)e * 24;
+  const uint8_t *sn = M.sideNode + ((size_t)e * 6 + l) * 4;
+  const int nodes[3] = {sn[0], sn[tri], sn[tri + 1]};
+  double Ax[3], Ay[3], Az[3];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    Ax[q] = C[nodes[q] * 3 + 0] - P[0];
+    Ay[q] = C[nodes[q] * 3 + 1] - P[1];
+    Az[q] = C[nodes[q] * 3 + 2] - P[2];
+  }
+  det = ((Ay[0] * Az[1] - Az[0] * Ay[1]) * Ax[2] + (Az[0] * Ax[1] - Ax[0] * Az[1]) * Ay[2] +
+         (Ax[0] * Ay[1] - Ay[0] * Ax[1]) * Az[2]);
+  return !((det < 0) || (det != det));
+}
+
+// IntersectionWithWall, particle_intersection.f90:79-164
+__device__ inline double intersection_with_wall(const DevMesh &M, int e, int l, int tri, const double P[3], const double traj[3]) {
+  const double *C = M.corner + (size_t)e * 24;
+  const uint8_t *sn = M.sideNode + ((size_t)e * 6 + l) * 4;
+  const double xN = C[sn[0] * 3], yN = C[sn[0] * 3 + 1], zN = C[sn[0] * 3 + 2];
+  const int n1 = sn[tri], n2 = sn[tri + 1];
+  const double V1[3] = {C[n1 * 3] - xN, C[n1 * 3 + 1] - yN, C[n1 * 3 + 2] - zN};
+  const double V2[3] = {C[n2 * 3] - xN, C[n2 * 3 + 1] - yN, C[n2 * 3 + 2] - zN};
+  double nx = V1[1] * V2[2] - V1[2] * V2[1];
+  double ny = V1[2] * V2[0] - V1[0] * V2[2];
+  double nz = V1[0] * V2[1] - V1[1] * V2[0];
+  const double nVal = sqrt(nx * nx + ny * ny + nz * nz);
+  nx = nx / nVal; ny = ny / nVal; nz = nz / nVal;
+  const double bx = P[0] - xN, by = P[1] - yN, bz = P[2] - zN;
+  const double ax = bx - nx * (bx * nx + by * ny + bz * nz);
+  const double ay = by - ny * (bx * nx + by * ny + bz * nz);
+  const double az = bz - nz * (bx * nx + by * ny + bz * nz);
+  double dist = sqrt(((ay * bz - az * by) * (ay * bz - az * by) + (az * bx - ax * bz) * (az * bx - ax * bz) +
+                      (ax * by - ay * bx) * (ax * by - ay * bx)) /
+                     (ax * ax + ay * ay + az * az));
+  if (dist != dist) dist = sqrt(bx * bx + by * by + bz * bz);
+  double alpha = traj[0] * nx + traj[1] * ny + traj[2] * nz;
+  if (fabs(alpha) > 0.) alpha = dist / alpha;
+  return alpha;
+}
+
+// SingleParticleTriaTracking3D, particles/tracking/particle_triatracking.f90:137-484
+// (regular sides; open and periodic particle boundaries, particle_boundary_condition.f90:35-284).
+// x and last are updated in place by periodic shifts.  Returns the new owner
+// element, or -1 when the particle is lost or removed; *removed_open tells
+// the two apart.
+__device__ inline int tria_track(const DevMesh &M, int elem, double x[3], double last[3], bool &lost, int &steps) {
+  int DoneLastElem[6][4];
+#pragma unroll
+  for (int a = 0; a < 6; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) DoneLastElem[a][b] = 0;
+  double traj[3] = {0, 0, 0}, len = 0.0;
+  lost = false;
+  steps = 0;
+  for (int guard = 0; guard < 4096; ++guard) {
+    ++steps;
+    double det[6][2];
+    if (inside_quad(M, elem, x, det)) return elem;
+    traj[0] = x[0] - last[0];
+    traj[1] = x[1] - last[1];
+    traj[2] = x[2] - last[2];
+    len = sqrt(traj[0] * traj[0] + traj[1] * traj[1] + traj[2] * traj[2]);
+    if (fabs(len) > 0.) { traj[0] = traj[0] / len; traj[1] = traj[1] / len; traj[2] = traj[2] / len; }
+    int nThrough = 0, locS[12], triS[12];
+    int side = -1, tri = 0;
+    for (int l = 0; l < 6; ++l)
+      for (int t = 1; t <= 2; ++t)
+        if (det[l][t - 1] <= -0.0 && through_side_fast(M, elem, l, t, last, traj)) {
+          locS[nThrough] = l;
+          triS[nThrough] = t;
+          ++nThrough;
+          side = l;
+        }
+    tri = nThrough > 0 ? triS[0] : 0;
+    if (nThrough != 1) {
+      if (nThrough == 0) { lost = true; return -1; }
+      int second = 0;
+      double minRatio = 0;
+      for (int q = 0; q < nThrough; ++q) {
+        const int gid = M.sideGID[(size_t)elem * 6 + locS[q]];
+        bool doCheck = true;
+        for (int is = 1; is < 6; ++is)
+          if (DoneLastElem[is][0] == elem + 1 && DoneLastElem[is][3] == gid && DoneLastElem[is][2] == triS[q]) doCheck = false;
+        if (!doCheck) continue;
+        double detM;
+        if (through_side_last_pos(M, elem, locS[q], triS[q], last, detM)) {
+          const double dP = det[locS[q]][triS[q] - 1];
+          if (detM == 0 && dP == 0) continue;
+          if (detM == 0 && minRatio == 0) {
+            ++second; side = locS[q]; tri = triS[q];
+          } else {
+            if (detM == 0) continue;
+            const double ratio = dP / detM;
+            if (ratio < minRatio) { minRatio = ratio; ++second; side = locS[q]; tri = triS[q]; }
+          }
+        }
+      }
+      if (second == 0) { lost = true; return -1; }
+    }
+    const size_t si = (size_t)elem * 6 + side;
+    const int bc = M.sideBC[si];
+    const int gid = M.sideGID[si];
+    if (bc != 0) {
+      const int oldElem = elem;
+      if (bc == 1) {  // open boundary: RemoveParticle
+        lost = false;
+        return -1;
+      }
+      // periodic: IntersectionWithWall then PeriodicBoundary
+      const double alpha = intersection_with_wall(M, elem, side, tri, last, traj);
+      const int pvid = M.sidePV[si];
+      const int v = (pvid < 0 ? -pvid : pvid) - 1;
+      for (int c = 0; c < 3; ++c) last[c] = last[c] + traj[c] * alpha;
+      for (int c = 0; c < 3; ++c) last[c] = last[c] + copysign(fabs(M.PV[v][c]), (double)pvid);
+      for (int c = 0; c < 3; ++c) x[c] = last[c] + (len - alpha) * traj[c];
+      elem = M.sideNb[si];
+      for (int k = 4; k >= 0; --k)
+        for (int b = 0; b < 4; ++b) DoneLastElem[k + 1][b] = DoneLastElem[k][b];
+      DoneLastElem[0][0] = oldElem + 1;
+      DoneLastElem[0][1] = side + 1;
+      DoneLastElem[0][2] = tri;
+      DoneLastElem[0][3] = gid;
+    } else {
+      for (int k = 4; k >= 0; --k)
+        for (int b = 0; b < 4; ++b) DoneLastElem[k + 1][b] = DoneLastElem[k][b];
+      DoneLastElem[0][0] = elem + 1;
+      DoneLastElem[0][1] = side + 1;
+      DoneLastElem[0][2] = tri;
+      DoneLastElem[0][3] = gid;
+      elem = M.sideNb[si];
+    }
+    if (elem < 0 || elem >= M.nElems) { lost = true; return -1; }
+  }
+  lost = true;
+  return -1;
+}
+
+}  // namespace pic

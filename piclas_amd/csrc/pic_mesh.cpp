@@ -1,0 +1,524 @@
+// Host-side producers of the particle-mesh tables the stage kernels read.
+//
+// Each table restates the reference's particle-mesh initialisation (paths
+// relative to the reference's src/):
+//   ElemSideNodeID, ConcaveElemSide   particles/particle_mesh/particle_mesh_tools.f90:1800-1930
+//   XCL_NGeo, dXCL_NGeo               mesh/metrics.f90:293-338
+//   ElemBaryNGeo, ElemRadiusNGeo      particles/particle_mesh/particle_mesh_build.f90:217-309
+//   XiEtaZetaBasis, slen...           particles/particle_mesh/particle_mesh_build.f90:386-444
+//   Elem_xGP, sJ                      mesh/metrics.f90:293-356 (evaluated at the Gauss points)
+//   NodeVolume + periodic twins       particles/pic/deposition/pic_depo_tools.f90:224-355,
+//                                     particles/pic/deposition/pic_depo.f90:1218-
+// plus the shape-function source lists, which have no counterpart in the
+// reference (it searches its FIBGM per particle, pic_depo_shapefunction_tools.f90:292-422;
+// the stage instead lists, per target element, every element whose particles
+// can reach it, so deposition is a gather).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <unordered_map>
+
+#include "stage_internal.h"
+
+namespace pic {
+namespace {
+
+// basis.f90:1011-1022
+bool almost_equal_unity(double x, double y) {
+  const double eps = 2.220446049250313e-16;
+  if (x == 0.0 || y == 0.0) return std::fabs(x - y) <= 2.0 * eps;
+  return std::fabs(x - y) <= eps * std::fabs(x) && std::fabs(x - y) <= eps * std::fabs(y);
+}
+
+// basis.f90:1223-1264
+void lagrange(double x, int N, const double *xg, const double *wb, double *L) {
+  bool eq = false;
+  for (int i = 0; i <= N; ++i) {
+    L[i] = 0.0;
+    if (almost_equal_unity(x, xg[i])) { L[i] = 1.0; eq = true; }
+  }
+  if (eq) return;
+  double sum = 0.0;
+  for (int i = 0; i <= N; ++i) { L[i] = wb[i] / (x - xg[i]); sum += L[i]; }
+  for (int i = 0; i <= N; ++i) L[i] = L[i] / sum;
+}
+
+// basis.f90:643-688, normalised Legendre polynomial and derivative
+void legendre(int N, double x, double &L, double &Ld) {
+  double l = 0, d = 0;
+  if (N == 0) { l = 1; d = 0; }
+  else if (N == 1) { l = x; d = 1; }
+  else {
+    double l2 = 1, l1 = x, d2 = 0, d1 = 1;
+    for (int i = 2; i <= N; ++i) {
+      l = (double(2 * i - 1) * x * l1 - double(i - 1) * l2) / double(i);
+      d = d2 + double(2 * i - 1) * l1;
+      l2 = l1; l1 = l; d2 = d1; d1 = d;
+    }
+  }
+  L = l * std::sqrt(double(N) + 0.5);
+  Ld = d * std::sqrt(double(N) + 0.5);
+}
+
+// basis.f90:758-835 Gauss-Legendre nodes by Newton from Chebyshev guesses
+void gauss_nodes(int N, double *x, double *w) {
+  double L, Ld;
+  if (N == 0) { x[0] = 0; w[0] = 2; return; }
+  if (N == 1) { x[0] = -std::sqrt(1.0 / 3.0); x[1] = -x[0]; w[0] = w[1] = 1.0; return; }
+  const double cheb = 2.0 * std::atan(1.0) / double(N + 1);
+  for (int i = 0; i <= (N + 1) / 2 - 1; ++i) {
+    x[i] = -std::cos(cheb * double(2 * i + 1));
+    for (int it = 0; it <= 10; ++it) {
+      legendre(N + 1, x[i], L, Ld);
+      double dx = -L / Ld;
+      x[i] += dx;
+      if (std::fabs(dx) < 1e-15 * std::fabs(x[i])) break;
+    }
+    legendre(N + 1, x[i], L, Ld);
+    x[N - i] = -x[i];
+    w[i] = (2.0 * N + 3) / ((1.0 - x[i] * x[i]) * Ld * Ld);
+    w[N - i] = w[i];
+  }
+  if (N % 2 == 0) {
+    x[N / 2] = 0.0;
+    legendre(N + 1, 0.0, L, Ld);
+    w[N / 2] = (2.0 * N + 3) / (Ld * Ld);
+  }
+}
+
+// basis.f90:952-976
+void bary_weights(int N, const double *x, double *wb) {
+  for (int i = 0; i <= N; ++i) wb[i] = 1.0;
+  for (int i = 1; i <= N; ++i)
+    for (int j = 0; j < i; ++j) { wb[j] *= (x[j] - x[i]); wb[i] *= (x[i] - x[j]); }
+  for (int i = 0; i <= N; ++i) wb[i] = 1.0 / wb[i];
+}
+
+// Sum-factorised tensor-product change of basis of a 3-vector field
+// (mesh/changeBasis.f90:54-111).
+void change_basis(int Nin, int Nout, const double *V, const double *in, double *out) {
+  const int a = Nin + 1, b = Nout + 1;
+  std::vector<double> b1(size_t(b) * a * a * 3, 0.0), b2(size_t(b) * b * a * 3, 0.0);
+  for (int k = 0; k < a; ++k) for (int j = 0; j < a; ++j) for (int i = 0; i < a; ++i)
+    for (int io = 0; io < b; ++io) for (int c = 0; c < 3; ++c)
+      b1[((k * a + j) * b + io) * 3 + c] += V[io * a + i] * in[((k * a + j) * a + i) * 3 + c];
+  for (int k = 0; k < a; ++k) for (int j = 0; j < a; ++j) for (int jo = 0; jo < b; ++jo)
+    for (int io = 0; io < b; ++io) for (int c = 0; c < 3; ++c)
+      b2[((k * b + jo) * b + io) * 3 + c] += V[jo * a + j] * b1[((k * a + j) * b + io) * 3 + c];
+  std::fill(out, out + size_t(b) * b * b * 3, 0.0);
+  for (int k = 0; k < a; ++k) for (int ko = 0; ko < b; ++ko) for (int jo = 0; jo < b; ++jo)
+    for (int io = 0; io < b; ++io) for (int c = 0; c < 3; ++c)
+      out[((ko * b + jo) * b + io) * 3 + c] += V[ko * a + k] * b2[((k * b + jo) * b + io) * 3 + c];
+}
+
+struct UnionFind {
+  std::vector<int32_t> p;
+  explicit UnionFind(int n) : p(n) { std::iota(p.begin(), p.end(), 0); }
+  int find(int x) { while (p[x] != x) { p[x] = p[p[x]]; x = p[x]; } return x; }
+  void unite(int a, int b) { a = find(a); b = find(b); if (a != b) p[std::max(a, b)] = std::min(a, b); }
+};
+
+}  // namespace
+
+std::string build_host_mesh(const pic_mesh_desc &m, const pic_params &par, HostMesh &hm) {
+  const int ne = m.nElems;
+  if (ne <= 0 || m.nSides != 6 * ne) return "mesh: expected 6 sides per element (no mortars)";
+  if (m.NGeo < 1 || m.NGeo > kMaxNGeo) return "mesh: NGeo > 1 (curved elements) is not supported by this build";
+  if (m.N < 1 || m.N > kMaxN) return "mesh: N out of range 1..7";
+  hm.nElems = ne; hm.nSides = m.nSides; hm.nNodes = m.nNodes; hm.NGeo = m.NGeo; hm.N = m.N;
+  hm.nDof = (m.N + 1) * (m.N + 1) * (m.N + 1);
+  hm.nGP = (m.NGeo + 1) * (m.NGeo + 1) * (m.NGeo + 1);
+  const int g1 = m.NGeo + 1, nGP = hm.nGP, nDof = hm.nDof;
+  auto EI = [&](int e, int c) { return m.ElemInfo[size_t(e) * 6 + c]; };
+  auto SI = [&](int s, int c) { return m.SideInfo[size_t(s) * 5 + c]; };
+
+  hm.nPV = m.nPeriodicVectors;
+  if (hm.nPV > 3) return "mesh: more than 3 periodic vectors";
+  for (int v = 0; v < hm.nPV; ++v)
+    for (int c = 0; c < 3; ++c) hm.PV[v][c] = m.PeriodicVectors[v * 3 + c];
+  for (int b = 0; b < m.nBCs; ++b) {
+    int t = m.PartBoundCond[b];
+    if (t != PIC_BC_OPEN && t != PIC_BC_PERIODIC)
+      return "mesh: particle boundary condition " + std::to_string(t) + " is not supported (open, periodic)";
+  }
+
+  // CGNS corners (mesh_tools.f90:740-769) as tensor corner indices t=i+2j+4k
+  const int cgns_t[8] = {0, 1, 3, 2, 4, 5, 7, 6};
+  const int side_cgns[6][4] = {{1, 4, 3, 2}, {1, 2, 6, 5}, {2, 3, 7, 6}, {3, 4, 8, 7}, {1, 5, 8, 4}, {5, 6, 7, 8}};
+  // tensor corner t -> element-local node number (0-based) for NGeo
+  auto corner_node = [&](int t) {
+    int i = (t & 1) * m.NGeo, j = ((t >> 1) & 1) * m.NGeo, k = ((t >> 2) & 1) * m.NGeo;
+    return (k * g1 + j) * g1 + i;
+  };
+
+  hm.corner.assign(size_t(ne) * 24, 0.0);
+  hm.sideNb.assign(size_t(ne) * 6, -1);
+  hm.sideGID.assign(size_t(ne) * 6, 0);
+  hm.sideNode.assign(size_t(ne) * 24, 0);
+  hm.sideBC.assign(size_t(ne) * 6, 0);
+  hm.sidePV.assign(size_t(ne) * 6, 0);
+  hm.concave.assign(ne, 0);
+  for (int c = 0; c < 3; ++c) { hm.lo[c] = 1e300; hm.hi[c] = -1e300; }
+
+  for (int e = 0; e < ne; ++e) {
+    const int fs = EI(e, 2), ls_ = EI(e, 3), fn = EI(e, 4), ln = EI(e, 5);
+    if (ls_ - fs != 6) return "mesh: element with != 6 sides";
+    if (ln - fn != nGP) return "mesh: element node count != (NGeo+1)^3";
+    for (int t = 0; t < 8; ++t)
+      for (int c = 0; c < 3; ++c) {
+        double v = m.NodeCoords[size_t(fn + corner_node(t)) * 3 + c];
+        hm.corner[size_t(e) * 24 + t * 3 + c] = v;
+        hm.lo[c] = std::min(hm.lo[c], v);
+        hm.hi[c] = std::max(hm.hi[c], v);
+      }
+    for (int l = 0; l < 6; ++l) {
+      const int sd = fs + l;  // 0-based side index; SIDE_LOCALID = l+1 for non-mortar hexahedra
+      if (SI(sd, 0) > 100) return "mesh: mortar sides are not supported";
+      const int nb = SI(sd, 2);
+      if (nb < 0) return "mesh: mortar neighbours are not supported";
+      hm.sideNb[e * 6 + l] = nb - 1;
+      hm.sideGID[e * 6 + l] = sd + 1;
+      const int bcid = SI(sd, 4);
+      if (bcid > 0) {
+        hm.sideBC[e * 6 + l] = uint8_t(m.PartBoundCond[bcid - 1]);
+        hm.sidePV[e * 6 + l] = int8_t(m.BCType[(bcid - 1) * 4 + 3]);
+        if (m.PartBoundCond[bcid - 1] == PIC_BC_PERIODIC && (hm.sidePV[e * 6 + l] == 0 || std::abs(hm.sidePV[e * 6 + l]) > hm.nPV))
+          return "mesh: periodic side without a valid periodic vector";
+      } else if (nb == 0) {
+        return "mesh: side without neighbour and without boundary condition";
+      }
+      // ElemSideNodeID with the flip rotation (particle_mesh_tools.f90:1874-1898)
+      const int flip = SI(sd, 3) % 10;
+      const int nStart = SI(sd, 1) > 0 ? 0 : std::max(0, flip - 1);
+      for (int k = 0; k < 4; ++k)
+        hm.sideNode[(e * 6 + l) * 4 + k] = uint8_t(cgns_t[side_cgns[l][(nStart + k) % 4] - 1]);
+    }
+    // ConcaveElemSide (particle_mesh_tools.f90:1904-1930)
+    for (int l = 0; l < 6; ++l) {
+      const uint8_t *sn = &hm.sideNode[(e * 6 + l) * 4];
+      const double *C = &hm.corner[size_t(e) * 24];
+      double A[3][3];
+      for (int q = 0; q < 3; ++q)
+        for (int c = 0; c < 3; ++c) A[c][q] = C[sn[q] * 3 + c] - C[sn[3] * 3 + c];
+      double det = ((A[1][0] * A[2][1] - A[2][0] * A[1][1]) * A[0][2] +
+                    (A[2][0] * A[0][1] - A[0][0] * A[2][1]) * A[1][2] +
+                    (A[0][0] * A[1][1] - A[1][0] * A[0][1]) * A[2][2]);
+      bool cc = false;
+      if (det < 0) cc = true;
+      else if (det == 0.0) {
+        // GlobalSideID < SIDE_NBSIDEID (1-based; 0 when there is no neighbour)
+        const int sd = fs + l, nb = SI(sd, 2);
+        int nbside = 0;
+        if (nb > 0) {
+          const int nfs = EI(nb - 1, 2);
+          for (int q = 0; q < 6; ++q)
+            if (std::abs(SI(nfs + q, 1)) == std::abs(SI(sd, 1))) { nbside = nfs + q + 1; break; }
+        }
+        cc = (sd + 1) < nbside;
+      }
+      if (cc) hm.concave[e] |= uint8_t(1u << l);
+    }
+  }
+
+  // 1D bases
+  for (int i = 0; i <= m.NGeo; ++i) hm.xiCL[i] = -std::cos(double(i) / double(m.NGeo) * std::acos(-1.0));
+  bary_weights(m.NGeo, hm.xiCL, hm.wBaryCL);
+  gauss_nodes(m.N, hm.xGP, hm.wGP);
+  bary_weights(m.N, hm.xGP, hm.wBaryGP);
+
+  // XCL_NGeo (equidistant -> CL change of basis) and dXCL_NGeo = D x XCL
+  double xEQ[8], wEQ[8], Vdm[64], D[64];
+  for (int i = 0; i <= m.NGeo; ++i) xEQ[i] = 2.0 * double(i) / double(m.NGeo) - 1.0;
+  bary_weights(m.NGeo, xEQ, wEQ);
+  for (int io = 0; io <= m.NGeo; ++io) lagrange(hm.xiCL[io], m.NGeo, xEQ, wEQ, &Vdm[io * g1]);
+  // PolynomialDerivativeMatrix on the CL nodes
+  std::fill(D, D + 64, 0.0);
+  for (int l = 0; l < g1; ++l)
+    for (int i = 0; i < g1; ++i)
+      if (l != i) {
+        D[i * g1 + l] = hm.wBaryCL[l] / (hm.wBaryCL[i] * (hm.xiCL[i] - hm.xiCL[l]));
+        D[i * g1 + i] -= D[i * g1 + l];
+      }
+  hm.XCL.assign(size_t(ne) * nGP * 3, 0.0);
+  hm.dXCL.assign(size_t(ne) * nGP * 9, 0.0);
+  for (int e = 0; e < ne; ++e) {
+    const double *nc = &m.NodeCoords[size_t(EI(e, 4)) * 3];
+    double *X = &hm.XCL[size_t(e) * nGP * 3];
+    change_basis(m.NGeo, m.NGeo, Vdm, nc, X);
+    double *dX = &hm.dXCL[size_t(e) * nGP * 9];
+    for (int k = 0; k < g1; ++k) for (int j = 0; j < g1; ++j) for (int i = 0; i < g1; ++i) {
+      const int p = (k * g1 + j) * g1 + i;
+      for (int ll = 0; ll < g1; ++ll)
+        for (int x = 0; x < 3; ++x) {
+          dX[p * 9 + x * 3 + 0] += D[i * g1 + ll] * X[((k * g1 + j) * g1 + ll) * 3 + x];
+          dX[p * 9 + x * 3 + 1] += D[j * g1 + ll] * X[((k * g1 + ll) * g1 + i) * 3 + x];
+          dX[p * 9 + x * 3 + 2] += D[k * g1 + ll] * X[((ll * g1 + j) * g1 + i) * 3 + x];
+        }
+    }
+  }
+
+  // ElemBaryNGeo / ElemRadiusNGeo (TriaTracking flavour: corner mean, max corner distance)
+  hm.bary.assign(size_t(ne) * 3, 0.0);
+  hm.radius.assign(ne, 0.0);
+  for (int e = 0; e < ne; ++e) {
+    const double *nc = &m.NodeCoords[size_t(EI(e, 4)) * 3];
+    double xp[3] = {0, 0, 0};
+    for (int q = 0; q < 8; ++q)
+      for (int c = 0; c < 3; ++c) xp[c] += nc[q * 3 + c];
+    for (int c = 0; c < 3; ++c) hm.bary[e * 3 + c] = xp[c] / 8.0;
+    double r = 0;
+    for (int q = 0; q < 8; ++q) {
+      double d0 = nc[q * 3] - hm.bary[e * 3], d1 = nc[q * 3 + 1] - hm.bary[e * 3 + 1], d2 = nc[q * 3 + 2] - hm.bary[e * 3 + 2];
+      r = std::max(r, std::sqrt(d0 * d0 + d1 * d1 + d2 * d2));
+    }
+    hm.radius[e] = r;
+  }
+
+  // XiEtaZetaBasis / slenXiEtaZetaBasis
+  hm.xiBasis.assign(size_t(ne) * 18, 0.0);
+  hm.slen.assign(size_t(ne) * 6, 0.0);
+  {
+    const double Xi[6][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {-1, 0, 0}, {0, -1, 0}, {0, 0, -1}};
+    double L[3][8];
+    for (int e = 0; e < ne; ++e) {
+      const double *X = &hm.XCL[size_t(e) * nGP * 3];
+      for (int d = 0; d < 6; ++d) {
+        for (int q = 0; q < 3; ++q) lagrange(Xi[d][q], m.NGeo, hm.xiCL, hm.wBaryCL, L[q]);
+        double xp[3] = {0, 0, 0};
+        for (int k = 0; k < g1; ++k) for (int j = 0; j < g1; ++j) for (int i = 0; i < g1; ++i)
+          for (int c = 0; c < 3; ++c) xp[c] = xp[c] + X[((k * g1 + j) * g1 + i) * 3 + c] * L[0][i] * L[1][j] * L[2][k];
+        double *b = &hm.xiBasis[(e * 6 + d) * 3];
+        for (int c = 0; c < 3; ++c) b[c] = xp[c] - hm.bary[e * 3 + c];
+        hm.slen[e * 6 + d] = 1.0 / (b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+      }
+    }
+  }
+
+  // Elem_xGP and sJ at the Gauss points (mapping and Jacobian of XCL_NGeo)
+  hm.xGPe.assign(size_t(ne) * nDof * 3, 0.0);
+  hm.sJ.assign(size_t(ne) * nDof, 0.0);
+  {
+    double Vg[8 * (kMaxN + 1)];
+    for (int io = 0; io <= m.N; ++io) lagrange(hm.xGP[io], m.NGeo, hm.xiCL, hm.wBaryCL, &Vg[io * g1]);
+    std::vector<double> din(size_t(nGP) * 3), dout(size_t(nDof) * 3), J(size_t(nDof) * 9);
+    for (int e = 0; e < ne; ++e) {
+      change_basis(m.NGeo, m.N, Vg, &hm.XCL[size_t(e) * nGP * 3], &hm.xGPe[size_t(e) * nDof * 3]);
+      for (int d = 0; d < 3; ++d) {
+        for (int p = 0; p < nGP; ++p)
+          for (int x = 0; x < 3; ++x) din[p * 3 + x] = hm.dXCL[(size_t(e) * nGP + p) * 9 + x * 3 + d];
+        change_basis(m.NGeo, m.N, Vg, din.data(), dout.data());
+        for (int p = 0; p < nDof; ++p)
+          for (int x = 0; x < 3; ++x) J[p * 9 + x * 3 + d] = dout[p * 3 + x];
+      }
+      for (int p = 0; p < nDof; ++p) {
+        const double *Jp = &J[p * 9];
+        auto dx = [&](int d, int x) { return Jp[x * 3 + d]; };
+        double det = dx(0, 0) * (dx(1, 1) * dx(2, 2) - dx(2, 1) * dx(1, 2)) +
+                     dx(1, 0) * (dx(2, 1) * dx(0, 2) - dx(0, 1) * dx(2, 2)) +
+                     dx(2, 0) * (dx(0, 1) * dx(1, 2) - dx(1, 1) * dx(0, 2));
+        if (!(det > 0)) return "mesh: non-positive Jacobian at a Gauss point of element " + std::to_string(e + 1);
+        hm.sJ[size_t(e) * nDof + p] = 1.0 / det;
+      }
+    }
+  }
+
+  // ---- cell_volweight_mean node classes ------------------------------------
+  int maxid = 0;
+  for (int i = 0; i < m.nNodes; ++i) maxid = std::max(maxid, m.GlobalNodeIDs[i]);
+  hm.nUnique = maxid;
+  std::vector<double> ux(size_t(maxid) * 3, 0.0);
+  for (int i = 0; i < m.nNodes; ++i)
+    for (int c = 0; c < 3; ++c) ux[size_t(m.GlobalNodeIDs[i] - 1) * 3 + c] = m.NodeCoords[size_t(i) * 3 + c];
+  UnionFind uf(maxid);
+  if (hm.nPV > 0) {
+    double scale = 0;
+    for (int e = 0; e < ne; ++e) scale = std::max(scale, hm.radius[e]);
+    const double tol = 1e-9 * (scale > 0 ? scale : 1.0), q = 1e-6 * (scale > 0 ? scale : 1.0);
+    auto key = [&](double x, double y, double z) {
+      int64_t a = int64_t(std::floor(x / q)), b = int64_t(std::floor(y / q)), c = int64_t(std::floor(z / q));
+      return (uint64_t(a) * 73856093ull) ^ (uint64_t(b) * 19349663ull) ^ (uint64_t(c) * 83492791ull);
+    };
+    std::unordered_multimap<uint64_t, int32_t> grid;
+    grid.reserve(size_t(maxid) * 2);
+    for (int u = 0; u < maxid; ++u) grid.emplace(key(ux[u * 3], ux[u * 3 + 1], ux[u * 3 + 2]), u);
+    int ncomb = 1;
+    for (int v = 0; v < hm.nPV; ++v) ncomb *= 3;
+    for (int u = 0; u < maxid; ++u)
+      for (int comb = 0; comb < ncomb; ++comb) {
+        int cc = comb, co[3] = {0, 0, 0};
+        for (int v = 0; v < hm.nPV; ++v) { co[v] = cc % 3 - 1; cc /= 3; }
+        double p[3];
+        for (int c = 0; c < 3; ++c) {
+          p[c] = ux[u * 3 + c];
+          for (int v = 0; v < hm.nPV; ++v) p[c] += co[v] * hm.PV[v][c];
+        }
+        for (int dz = -1; dz <= 1; ++dz) for (int dy = -1; dy <= 1; ++dy) for (int dx = -1; dx <= 1; ++dx) {
+          auto rg = grid.equal_range(key(p[0] + dx * q, p[1] + dy * q, p[2] + dz * q));
+          for (auto it = rg.first; it != rg.second; ++it) {
+            int w = it->second;
+            if (std::fabs(ux[w * 3] - p[0]) <= tol && std::fabs(ux[w * 3 + 1] - p[1]) <= tol &&
+                std::fabs(ux[w * 3 + 2] - p[2]) <= tol)
+              uf.unite(u, w);
+          }
+        }
+      }
+  }
+  std::vector<int32_t> cls(maxid, -1);
+  int ncl = 0;
+  for (int u = 0; u < maxid; ++u) {
+    int r = uf.find(u);
+    if (cls[r] < 0) cls[r] = ncl++;
+    cls[u] = cls[r];
+  }
+  hm.nClasses = ncl;
+  hm.elemClass.assign(size_t(ne) * 8, 0);
+  std::vector<double> nodeVol(maxid, 0.0);
+  {
+    const int n1 = m.N + 1;
+    const double *x = hm.xGP, *w = hm.wGP;
+    for (int e = 0; e < ne; ++e) {
+      const int fn = EI(e, 4);
+      int uid[8];
+      for (int c = 0; c < 8; ++c) {
+        uid[c] = m.GlobalNodeIDs[fn + corner_node(cgns_t[c])] - 1;
+        hm.elemClass[size_t(e) * 8 + c] = cls[uid[c]];
+      }
+      // CalcCellLocNodeVolumes (pic_depo_tools.f90:291-303)
+      for (int i = 0; i < n1; ++i) for (int j = 0; j < n1; ++j) for (int k = 0; k < n1; ++k) {
+        const double s = hm.sJ[size_t(e) * nDof + (k * n1 + j) * n1 + i];
+        const double wt = w[i] * w[j] * w[k] / 8.;
+        nodeVol[uid[0]] += 1. / s * ((1. - x[i]) * (1. - x[j]) * (1. - x[k]) * wt);
+        nodeVol[uid[1]] += 1. / s * ((1. + x[i]) * (1. - x[j]) * (1. - x[k]) * wt);
+        nodeVol[uid[2]] += 1. / s * ((1. + x[i]) * (1. + x[j]) * (1. - x[k]) * wt);
+        nodeVol[uid[3]] += 1. / s * ((1. - x[i]) * (1. + x[j]) * (1. - x[k]) * wt);
+        nodeVol[uid[4]] += 1. / s * ((1. - x[i]) * (1. - x[j]) * (1. + x[k]) * wt);
+        nodeVol[uid[5]] += 1. / s * ((1. + x[i]) * (1. - x[j]) * (1. + x[k]) * wt);
+        nodeVol[uid[6]] += 1. / s * ((1. + x[i]) * (1. + x[j]) * (1. + x[k]) * wt);
+        nodeVol[uid[7]] += 1. / s * ((1. - x[i]) * (1. + x[j]) * (1. + x[k]) * wt);
+      }
+    }
+  }
+  // class volume = own volume + twins' volumes (periodic merge, pic_depo_tools.f90:319-338)
+  hm.classVolume.assign(ncl, 0.0);
+  for (int u = 0; u < maxid; ++u) hm.classVolume[cls[u]] += nodeVol[u];
+  // incidence CSR, class -> (elem*8 + corner), in element order
+  hm.classOff.assign(ncl + 1, 0);
+  for (size_t q = 0; q < hm.elemClass.size(); ++q) hm.classOff[hm.elemClass[q] + 1]++;
+  for (int c = 0; c < ncl; ++c) hm.classOff[c + 1] += hm.classOff[c];
+  hm.classInc.assign(hm.elemClass.size(), 0);
+  {
+    std::vector<int32_t> fill(ncl, 0);
+    for (size_t q = 0; q < hm.elemClass.size(); ++q) {
+      int c = hm.elemClass[q];
+      hm.classInc[hm.classOff[c] + fill[c]++] = int32_t(q);
+    }
+  }
+
+  // ---- shape-function source lists -----------------------------------------
+  if (par.deposition == PIC_DEPO_SF) {
+    const double R = par.sf_radius;
+    if (!(R > 0)) return "shape_function: radius must be positive";
+    if (hm.nPV != 0 && hm.nPV != 3) return "shape_function: 3D deposition needs 0 or 3 periodic vectors";
+    // shift table: the 27 periodic cases (pic_depo.f90:858-888), or identity only
+    int nsh = hm.nPV == 3 ? 27 : 1;
+    for (int s = 0; s < 27; ++s) for (int c = 0; c < 3; ++c) hm.shift[s][c] = 0.0;
+    if (nsh == 27)
+      for (int s = 0; s < 27; ++s) {
+        int c0 = s / 9 - 1, c1 = (s / 3) % 3 - 1, c2 = s % 3 - 1;
+        for (int c = 0; c < 3; ++c) hm.shift[s][c] = c0 * hm.PV[0][c] + c1 * hm.PV[1][c] + c2 * hm.PV[2][c];
+      }
+    // element bounding boxes (sources: all nodes; targets: Gauss points)
+    std::vector<double> sbb(size_t(ne) * 6), tbb(size_t(ne) * 6);
+    double ext = 0;
+    for (int e = 0; e < ne; ++e) {
+      const double *nc = &m.NodeCoords[size_t(EI(e, 4)) * 3];
+      for (int c = 0; c < 3; ++c) { sbb[e * 6 + c] = 1e300; sbb[e * 6 + 3 + c] = -1e300; tbb[e * 6 + c] = 1e300; tbb[e * 6 + 3 + c] = -1e300; }
+      for (int q = 0; q < nGP; ++q)
+        for (int c = 0; c < 3; ++c) {
+          sbb[e * 6 + c] = std::min(sbb[e * 6 + c], nc[q * 3 + c]);
+          sbb[e * 6 + 3 + c] = std::max(sbb[e * 6 + 3 + c], nc[q * 3 + c]);
+        }
+      for (int q = 0; q < nDof; ++q)
+        for (int c = 0; c < 3; ++c) {
+          double v = hm.xGPe[(size_t(e) * nDof + q) * 3 + c];
+          tbb[e * 6 + c] = std::min(tbb[e * 6 + c], v);
+          tbb[e * 6 + 3 + c] = std::max(tbb[e * 6 + 3 + c], v);
+        }
+      for (int c = 0; c < 3; ++c) ext = std::max(ext, sbb[e * 6 + 3 + c] - sbb[e * 6 + c]);
+    }
+    // bins over source boxes
+    int nb[3];
+    double bd[3];
+    for (int c = 0; c < 3; ++c) {
+      double L = hm.hi[c] - hm.lo[c];
+      nb[c] = std::max(1, int(std::ceil(L / ext)));
+      bd[c] = L > 0 ? L / nb[c] : 1.0;
+    }
+    const size_t nbins = size_t(nb[0]) * nb[1] * nb[2];
+    std::vector<int32_t> boff(nbins + 1, 0), belem;
+    auto binr = [&](double lo_, double hi_, int c, int &a, int &b) {
+      a = int(std::floor((lo_ - hm.lo[c]) / bd[c]));
+      b = int(std::floor((hi_ - hm.lo[c]) / bd[c]));
+      a = std::max(a, 0); b = std::min(b, nb[c] - 1);
+    };
+    for (int pass = 0; pass < 2; ++pass) {
+      std::vector<int32_t> fill(pass ? nbins : 0, 0);
+      for (int e = 0; e < ne; ++e) {
+        int a[3], b[3];
+        for (int c = 0; c < 3; ++c) binr(sbb[e * 6 + c], sbb[e * 6 + 3 + c], c, a[c], b[c]);
+        for (int z = a[2]; z <= b[2]; ++z) for (int y = a[1]; y <= b[1]; ++y) for (int x = a[0]; x <= b[0]; ++x) {
+          size_t bi = (size_t(z) * nb[1] + y) * nb[0] + x;
+          if (!pass) boff[bi + 1]++;
+          else belem[boff[bi] + fill[bi]++] = e;
+        }
+      }
+      if (!pass) {
+        for (size_t q = 0; q < nbins; ++q) boff[q + 1] += boff[q];
+        belem.resize(boff[nbins]);
+      }
+    }
+    const double pad = 1e-9 * ext + 0.0;
+    hm.sfOff.assign(ne + 1, 0);
+    hm.sfSrc.clear();
+    hm.sfShift.clear();
+    std::vector<int64_t> cand;
+    for (int t = 0; t < ne; ++t) {
+      cand.clear();
+      for (int s = 0; s < nsh; ++s) {
+        // query box: target Gauss-point box minus shift, dilated by R
+        double qlo[3], qhi[3];
+        bool any = true;
+        for (int c = 0; c < 3; ++c) {
+          qlo[c] = tbb[t * 6 + c] - hm.shift[s][c] - R - pad;
+          qhi[c] = tbb[t * 6 + 3 + c] - hm.shift[s][c] + R + pad;
+          if (qhi[c] < hm.lo[c] || qlo[c] > hm.hi[c]) any = false;
+        }
+        if (!any) continue;
+        int a[3], b[3];
+        for (int c = 0; c < 3; ++c) binr(qlo[c], qhi[c], c, a[c], b[c]);
+        for (int z = a[2]; z <= b[2]; ++z) for (int y = a[1]; y <= b[1]; ++y) for (int x = a[0]; x <= b[0]; ++x) {
+          size_t bi = (size_t(z) * nb[1] + y) * nb[0] + x;
+          for (int q = boff[bi]; q < boff[bi + 1]; ++q) {
+            int se = belem[q];
+            // box-to-box distance between shifted source box and target Gauss-point box
+            double d2 = 0;
+            for (int c = 0; c < 3; ++c) {
+              double slo = sbb[se * 6 + c] + hm.shift[s][c], shi = sbb[se * 6 + 3 + c] + hm.shift[s][c];
+              double g = std::max(0.0, std::max(slo - tbb[t * 6 + 3 + c], tbb[t * 6 + c] - shi));
+              d2 += g * g;
+            }
+            if (d2 <= (R + pad) * (R + pad)) cand.push_back(int64_t(se) * 32 + s);
+          }
+        }
+      }
+      std::sort(cand.begin(), cand.end());
+      cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
+      for (int64_t v : cand) { hm.sfSrc.push_back(int32_t(v / 32)); hm.sfShift.push_back(uint8_t(v % 32)); }
+      hm.sfOff[t + 1] = int32_t(hm.sfSrc.size());
+    }
+  }
+  return "";
+}
+
+}  // namespace pic

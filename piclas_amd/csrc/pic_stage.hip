@@ -1,0 +1,1067 @@
+// MI355X (gfx950) particle stage: kernels and the C-ABI of include/pic_stage.h.
+//
+// Step structure (TimeStepPoissonByBorisLeapfrog without the field solve,
+// reference src/timedisc/timedisc_TimeStepPoissonByBorisLeapfrog.f90:32-286):
+//   1. deposition from the element-sorted particles (gather, no atomics)
+//   2. one fused kernel: LastPartPos <- x, interpolate E, Boris push, TriaTracking
+//   3. element re-sort: particles that stay keep their order, arrivals are
+//      appended in caller-id order, so every run is bit-for-bit repeatable.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "pic_device.h"
+#include "stage_internal.h"
+
+#pragma clang fp contract(off)
+
+using namespace pic;
+
+namespace {
+
+thread_local std::string g_err;
+int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+#define HIPCHK(x)                                                                                        \
+  do {                                                                                                   \
+    hipError_t e_ = (x);                                                                                 \
+    if (e_ != hipSuccess) return fail(PIC_EHIP, std::string(#x) + ": " + hipGetErrorString(e_));         \
+  } while (0)
+
+constexpr int kBlock = 256;
+constexpr int kWave = 64;
+
+inline unsigned grid_for(int64_t n, int block = kBlock) { return unsigned((n + block - 1) / block); }
+
+// Workgroups are handed to the 8 XCDs round-robin; this maps hardware block
+// b to a logical block so each XCD works on a contiguous range (its own L2
+// then holds the neighbourhood of the elements it deposits).  The grid is a
+// multiple of 8 blocks, which makes the map a permutation of [0, grid).
+__device__ inline int xcd_block(int b, int grid) {
+  const int per = grid / 8;
+  return (b % 8) * per + b / 8;
+}
+inline unsigned xcd_grid(int64_t nblk) { return unsigned((nblk + 7) / 8 * 8); }
+
+__device__ inline int lane_id() { return __lane_id(); }
+
+// ------------------------------------------------------------------ kernels
+
+// Interpolation only (InterpolateFieldToParticle): F[i] = ext + E(x_i)
+template <int N>
+__global__ __launch_bounds__(kBlock) void k_interp(DevMesh M, DevParts P, int64_t n) {
+  const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const int e = P.elem[i];
+  double F[6] = {0, 0, 0, 0, 0, 0};
+  const int sp = P.spec[i];
+  if (e >= 0 && fabs(M.charge[sp]) > 0.0) {
+    const double x[3] = {P.x[i], P.y[i], P.z[i]};
+    field_at_particle<N>(M, e, x, F);
+  }
+#pragma unroll
+  for (int c = 0; c < 6; ++c) P.F[i * 6 + c] = F[c];
+}
+
+// Push only: LastPartPos <- PartState, Boris push with the stored field
+__global__ __launch_bounds__(kBlock) void k_push(DevMesh M, DevParts P, int64_t n, double dt) {
+  const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const int e = P.elem[i];
+  double x[3] = {P.x[i], P.y[i], P.z[i]};
+  P.lx[i] = x[0]; P.ly[i] = x[1]; P.lz[i] = x[2];
+  P.lastElem[i] = e;
+  if (e < 0) return;
+  double v[3] = {P.vx[i], P.vy[i], P.vz[i]};
+  double F[6];
+#pragma unroll
+  for (int c = 0; c < 6; ++c) F[c] = P.F[i * 6 + c];
+  const int sp = P.spec[i];
+  boris_push(M.charge[sp], M.mass[sp], P.isNew[i] != 0, F, dt, x, v);
+  P.isNew[i] = 0;
+  P.x[i] = x[0]; P.y[i] = x[1]; P.z[i] = x[2];
+  P.vx[i] = v[0]; P.vy[i] = v[1]; P.vz[i] = v[2];
+}
+
+// Tracking only (PerformTracking): from lastElem / LastPartPos to the new owner
+__global__ __launch_bounds__(kBlock) void k_track(DevMesh M, DevParts P, int64_t n, int32_t *newElem,
+                                                  unsigned long long *lostCount, uint8_t *insideById, double *stateById) {
+  const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const int e0 = P.lastElem[i];
+  if (e0 < 0) { newElem[i] = -1; return; }
+  double x[3] = {P.x[i], P.y[i], P.z[i]};
+  double last[3] = {P.lx[i], P.ly[i], P.lz[i]};
+  bool lost;
+  int steps;
+  const int e = tria_track(M, e0, x, last, lost, steps);
+  P.x[i] = x[0]; P.y[i] = x[1]; P.z[i] = x[2];
+  P.lx[i] = last[0]; P.ly[i] = last[1]; P.lz[i] = last[2];
+  newElem[i] = e;
+  P.elem[i] = e;
+  if (e < 0) {
+    const int id = P.id[i];
+    insideById[id] = 0;
+    double *st = stateById + (size_t)id * 6;
+    st[0] = x[0]; st[1] = x[1]; st[2] = x[2]; st[3] = P.vx[i]; st[4] = P.vy[i]; st[5] = P.vz[i];
+    if (lost) atomicAdd(lostCount, 1ull);
+  }
+}
+
+// Fused step body: interpolate + push + track for one particle
+template <int N>
+__global__ __launch_bounds__(kBlock) void k_push_track(DevMesh M, DevParts P, int64_t n, double dt, int32_t *newElem,
+                                                       unsigned long long *lostCount, uint8_t *insideById,
+                                                       double *stateById) {
+  const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const int e0 = P.elem[i];
+  if (e0 < 0) { newElem[i] = -1; return; }
+  double x[3] = {P.x[i], P.y[i], P.z[i]};
+  double v[3] = {P.vx[i], P.vy[i], P.vz[i]};
+  const double last0[3] = {x[0], x[1], x[2]};
+  const int sp = P.spec[i];
+  double F[6] = {0, 0, 0, 0, 0, 0};
+  if (fabs(M.charge[sp]) > 0.0) field_at_particle<N>(M, e0, x, F);
+  boris_push(M.charge[sp], M.mass[sp], P.isNew[i] != 0, F, dt, x, v);
+  double last[3] = {last0[0], last0[1], last0[2]};
+  bool lost;
+  int steps;
+  const int e = tria_track(M, e0, x, last, lost, steps);
+  P.x[i] = x[0]; P.y[i] = x[1]; P.z[i] = x[2];
+  P.vx[i] = v[0]; P.vy[i] = v[1]; P.vz[i] = v[2];
+  P.isNew[i] = 0;
+  newElem[i] = e;
+  if (e < 0) {
+    const int id = P.id[i];
+    insideById[id] = 0;
+    double *st = stateById + (size_t)id * 6;
+    st[0] = x[0]; st[1] = x[1]; st[2] = x[2]; st[3] = v[0]; st[4] = v[1]; st[5] = v[2];
+    if (lost) atomicAdd(lostCount, 1ull);
+  }
+}
+
+// ---- sort by element --------------------------------------------------------
+// One wave per old element segment: stayers get their rank among the
+// stayers of the segment (order preserved); leavers take an arrival slot of
+// their new element (slot order fixed later by k_fix_arrivals).
+__global__ __launch_bounds__(kBlock) void k_seg_rank(int nElems, const int32_t *oldStart, const int32_t *newElem,
+                                                     int32_t *stayCnt, int32_t *arrCnt, int32_t *rank) {
+  const int w = (blockIdx.x * kBlock + threadIdx.x) / kWave;
+  const int lane = lane_id();
+  if (w >= nElems) return;
+  const int s0 = oldStart[w], s1 = oldStart[w + 1];
+  int stay = 0;
+  for (int base = s0; base < s1; base += kWave) {
+    const int i = base + lane;
+    const bool valid = i < s1;
+    const int ne = valid ? newElem[i] : -2;
+    const bool st = valid && ne == w;
+    const unsigned long long m = __ballot(st);
+    if (st) rank[i] = stay + __popcll(m & ((1ull << lane) - 1ull));
+    else if (valid && ne >= 0) rank[i] = atomicAdd(&arrCnt[ne], 1);
+    stay += __popcll(m);
+  }
+  if (lane == 0) stayCnt[w] = stay;
+}
+
+// exclusive scan of stayCnt+arrCnt into newStart[0..nElems], single workgroup
+__global__ __launch_bounds__(1024) void k_scan(int nElems, const int32_t *stayCnt, const int32_t *arrCnt, int32_t *newStart) {
+  __shared__ int32_t part[1024];
+  const int t = threadIdx.x;
+  const int per = (nElems + 1023) / 1024;
+  const int a = t * per, b = min(nElems, a + per);
+  int s = 0;
+  for (int e = a; e < b; ++e) s += stayCnt[e] + arrCnt[e];
+  part[t] = s;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    int v = t >= off ? part[t - off] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  int run = t > 0 ? part[t - 1] : 0;
+  for (int e = a; e < b; ++e) {
+    newStart[e] = run;
+    run += stayCnt[e] + arrCnt[e];
+  }
+  if (t == 1023) newStart[nElems] = part[1023];
+}
+
+__global__ __launch_bounds__(kBlock) void k_dest(int64_t n, const int32_t *oldElemOf, const int32_t *newElem,
+                                                 const int32_t *rank, const int32_t *newStart, const int32_t *stayCnt,
+                                                 int32_t *perm) {
+  const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const int ne = newElem[i];
+  if (ne < 0) return;
+  const int oe = oldElemOf[i];
+  int64_t d;
+  if (ne == oe) d = newStart[ne] + rank[i];
+  else d = int64_t(newStart[ne]) + stayCnt[ne] + rank[i];
+  perm[d] = int32_t(i);
+}
+
+// per element: order the arrival tail by caller id (insertion sort, tails are short)
+__global__ __launch_bounds__(kBlock) void k_fix_arrivals(int nElems, const int32_t *newStart, const int32_t *stayCnt,
+                                                         const int32_t *arrCnt, const int32_t *id, int32_t *perm) {
+  const int e = blockIdx.x * kBlock + threadIdx.x;
+  if (e >= nElems) return;
+  const int na = arrCnt[e];
+  if (na < 2) return;
+  int32_t *p = perm + newStart[e] + stayCnt[e];
+  for (int a = 1; a < na; ++a) {
+    const int32_t v = p[a];
+    const int32_t key = id[v];
+    int b = a - 1;
+    while (b >= 0 && id[p[b]] > key) { p[b + 1] = p[b]; --b; }
+    p[b + 1] = v;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_gather(const int32_t *total, const int32_t *perm, const T *__restrict__ src,
+                                                   T *__restrict__ dst, int width) {
+  const int64_t j = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (j >= int64_t(*total) * width) return;
+  const int64_t r = j / width, c = j % width;
+  dst[j] = src[int64_t(perm[r]) * width + c];
+}
+
+// six FP64 state arrays + id/spec/isNew/elem in one pass
+__global__ __launch_bounds__(kBlock) void k_gather_state(const int32_t *total, const int32_t *perm, DevParts s, DevParts d,
+                                                         const int32_t *newElem) {
+  const int64_t j = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (j >= *total) return;
+  const int32_t i = perm[j];
+  d.x[j] = s.x[i]; d.y[j] = s.y[i]; d.z[j] = s.z[i];
+  d.vx[j] = s.vx[i]; d.vy[j] = s.vy[i]; d.vz[j] = s.vz[i];
+  d.id[j] = s.id[i]; d.spec[j] = s.spec[i]; d.isNew[j] = s.isNew[i];
+  d.elem[j] = newElem[i];
+}
+
+// ---- cell_volweight_mean ----------------------------------------------------
+// One wave per element segment: trilinear weights of every particle (after
+// the ForceMode Newton of GetPositionInRefElem) summed per CGNS corner in a
+// fixed lane/butterfly order (DepositionMethod_CVWM, pic_depo_method.f90:465-544).
+template <int N>
+__global__ __launch_bounds__(kBlock) void k_cvwm_elem(DevMesh M, DevParts P, const int32_t *start, double *elemCorner) {
+  const int w = (blockIdx.x * kBlock + threadIdx.x) / kWave;
+  const int lane = lane_id();
+  if (w >= M.nElems) return;
+  double acc[8][4];
+#pragma unroll
+  for (int c = 0; c < 8; ++c)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) acc[c][k] = 0.0;
+  const int s0 = start[w], s1 = start[w + 1];
+  const double *C = M.corner + (size_t)w * 24;
+  for (int i = s0 + lane; i < s1; i += kWave) {
+    const int sp = P.spec[i];
+    const double q = M.charge[sp];
+    if (!(fabs(q) > 0.0)) continue;
+    const double Charge = q * M.mpf[sp];
+    const double x[3] = {P.x[i], P.y[i], P.z[i]};
+    double xi[3];
+    ref_newton_start<N>(M, w, x, xi);
+    const bool ok = ref_elem_newton(M, w, x, xi, true);
+    const double TS[4] = {P.vx[i] * Charge, P.vy[i] * Charge, P.vz[i] * Charge, Charge};
+    double wt[8];
+    if (ok) {
+      const double a1 = 0.5 * (xi[0] + 1.0), a2 = 0.5 * (xi[1] + 1.0), a3 = 0.5 * (xi[2] + 1.0);
+      wt[0] = (1 - a1) * (1 - a2) * (1 - a3);
+      wt[1] = (a1) * (1 - a2) * (1 - a3);
+      wt[2] = (a1) * (a2) * (1 - a3);
+      wt[3] = (1 - a1) * (a2) * (1 - a3);
+      wt[4] = (1 - a1) * (1 - a2) * (a3);
+      wt[5] = (a1) * (1 - a2) * (a3);
+      wt[6] = (a1) * (a2) * (a3);
+      wt[7] = (1 - a1) * (a2) * (a3);
+#pragma unroll
+      for (int c = 0; c < 8; ++c)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) acc[c][k] = acc[c][k] + (TS[k] * wt[c]);
+    } else {
+      // inverse-distance weights to the 8 corners (pic_depo_method.f90:512-539)
+      const int cgns_t[8] = {0, 1, 3, 2, 4, 5, 7, 6};
+      int hit = -1;
+      for (int c = 0; c < 8; ++c) {
+        const double *p = C + cgns_t[c] * 3;
+        const double d0 = p[0] - x[0], d1 = p[1] - x[1], d2 = p[2] - x[2];
+        const double nrm = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+        if (nrm > 0.) wt[c] = 1. / nrm;
+        else { hit = c; break; }
+      }
+      if (hit >= 0)
+        for (int c = 0; c < 8; ++c) wt[c] = c == hit ? 1.0 : 0.0;
+      double sum = 0.0;
+      for (int c = 0; c < 8; ++c) sum = sum + wt[c];
+#pragma unroll
+      for (int c = 0; c < 8; ++c)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) acc[c][k] = acc[c][k] + wt[c] / sum * TS[k];
+    }
+  }
+  // butterfly reduction over the wave
+#pragma unroll
+  for (int c = 0; c < 8; ++c)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      double v = acc[c][k];
+#pragma unroll
+      for (int off = 32; off >= 1; off >>= 1) v = v + __shfl_xor(v, off, kWave);
+      acc[c][k] = v;
+    }
+  if (lane < 32) {
+    double out = 0.0;
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        if (c * 4 + k == lane) out = acc[c][k];
+    elemCorner[(size_t)w * 32 + lane] = out;
+  }
+}
+
+// node classes (a node and its periodic twins): sum incidences, divide by
+// NodeVolume (pic_depo_method.f90:688-695)
+__global__ __launch_bounds__(kBlock) void k_cvwm_class(int nClasses, const int32_t *off, const int32_t *inc,
+                                                       const double *vol, const double *elemCorner, double *classSrc) {
+  const int c = blockIdx.x * kBlock + threadIdx.x;
+  if (c >= nClasses) return;
+  double s[4] = {0, 0, 0, 0};
+  for (int q = off[c]; q < off[c + 1]; ++q) {
+    const double *ec = elemCorner + (size_t)inc[q] * 4;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) s[k] = s[k] + ec[k];
+  }
+  const double v = vol[c];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) classSrc[(size_t)c * 4 + k] = v > 0. ? s[k] / v : s[k];
+}
+
+// node -> DOF trilinear interpolation (pic_depo_method.f90:704-733)
+template <int N>
+__global__ __launch_bounds__(kBlock) void k_cvwm_dof(DevMesh M, const int32_t *elemClass, const double *classSrc, double *PS) {
+  constexpr int n1 = N + 1, nd = n1 * n1 * n1;
+  const int64_t g = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (g >= int64_t(M.nElems) * nd) return;
+  const int e = int(g / nd), r = int(g % nd);
+  const int kk = r % n1, ll = (r / n1) % n1, mm = r / (n1 * n1);
+  const double a1 = (M.xGP[kk] + 1.0) / 2.0, a2 = (M.xGP[ll] + 1.0) / 2.0, a3 = (M.xGP[mm] + 1.0) / 2.0;
+  const int32_t *cl = elemClass + (size_t)e * 8;
+  double *ps = PS + g * 4;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const double *S0 = classSrc + (size_t)cl[0] * 4, *S1 = classSrc + (size_t)cl[1] * 4;
+    const double *S2 = classSrc + (size_t)cl[2] * 4, *S3 = classSrc + (size_t)cl[3] * 4;
+    const double *S4 = classSrc + (size_t)cl[4] * 4, *S5 = classSrc + (size_t)cl[5] * 4;
+    const double *S6 = classSrc + (size_t)cl[6] * 4, *S7 = classSrc + (size_t)cl[7] * 4;
+    ps[k] = S0[k] * (1 - a1) * (1 - a2) * (1 - a3) + S1[k] * (a1) * (1 - a2) * (1 - a3) +
+            S2[k] * (a1) * (a2) * (1 - a3) + S3[k] * (1 - a1) * (a2) * (1 - a3) +
+            S4[k] * (1 - a1) * (1 - a2) * (a3) + S5[k] * (a1) * (1 - a2) * (a3) +
+            S6[k] * (a1) * (a2) * (a3) + S7[k] * (1 - a1) * (a2) * (a3);
+  }
+}
+
+// ---- shape_function ---------------------------------------------------------
+// Gather form of depoChargeOnDOFsSF (pic_depo_shapefunction_tools.f90:292-422):
+// one wave per target element, one lane per DOF.  For every (source element,
+// periodic shift) of the target's list the wave streams the source segment,
+// keeps the particles the reference's element cull admits
+// (|x - bary| <= r_sf + ElemRadius), stages them in LDS and every lane adds
+// S^alpha * w_sf * q * (v,1) of the particles within r_sf of its DOF.
+struct SFArgs {
+  const int32_t *off, *src;
+  const uint8_t *shift;
+  double sh[27][3];
+  double r_sf, r2_sf, r2_sf_inv, w_sf;
+  int alpha;
+};
+
+template <int N>
+__global__ __launch_bounds__(kBlock) void k_sf_gather(DevMesh M, DevParts P, const int32_t *start, SFArgs A, double *PS) {
+  constexpr int n1 = N + 1, nd = n1 * n1 * n1, nch = (nd + kWave - 1) / kWave;
+  __shared__ double lds[kBlock / kWave][kWave][8];
+  const int wib = threadIdx.x / kWave;
+  const int lane = lane_id();
+  const int nblk = gridDim.x;
+  const int lb = xcd_block(blockIdx.x, nblk);
+  const int t = lb * (kBlock / kWave) + wib;
+  if (t >= M.nElems) return;
+  double xd[nch], yd[nch], zd[nch], acc[nch][4];
+  bool has[nch];
+#pragma unroll
+  for (int c = 0; c < nch; ++c) {
+    const int d = c * kWave + lane;
+    has[c] = d < nd;
+    const double *xg = M.xGPe + ((size_t)t * nd + (has[c] ? d : 0)) * 3;
+    xd[c] = xg[0]; yd[c] = xg[1]; zd[c] = xg[2];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) acc[c][k] = 0.0;
+  }
+  const double bx = M.bary[t * 3], by = M.bary[t * 3 + 1], bz = M.bary[t * 3 + 2];
+  const double thr = A.r_sf + M.radius[t];
+  double (*L)[8] = lds[wib];
+  for (int q = A.off[t]; q < A.off[t + 1]; ++q) {
+    const int s = A.src[q];
+    const int si = A.shift[q];
+    const double shx = A.sh[si][0], shy = A.sh[si][1], shz = A.sh[si][2];
+    const int s0 = start[s], s1 = start[s + 1];
+    for (int base = s0; base < s1; base += kWave) {
+      const int i = base + lane;
+      bool pass = false;
+      double px = 0, py = 0, pz = 0, f0 = 0, f1 = 0, f2 = 0, f3 = 0;
+      if (i < s1) {
+        const int sp = P.spec[i];
+        const double q0 = M.charge[sp];
+        // GetPartPosShifted: x + c1*V1 + c2*V2 + c3*V3 (the table holds the sum)
+        px = P.x[i] + shx; py = P.y[i] + shy; pz = P.z[i] + shz;
+        const double d0 = px - bx, d1 = py - by, d2 = pz - bz;
+        pass = (fabs(q0) > 0.0) && !(sqrt(d0 * d0 + d1 * d1 + d2 * d2) > thr);
+        const double Charge = q0 * M.mpf[sp];
+        f0 = P.vx[i] * Charge * A.w_sf;
+        f1 = P.vy[i] * Charge * A.w_sf;
+        f2 = P.vz[i] * Charge * A.w_sf;
+        f3 = Charge * A.w_sf;
+      }
+      const unsigned long long m = __ballot(pass);
+      const int cnt = __popcll(m);
+      if (pass) {
+        const int slot = __popcll(m & ((1ull << lane) - 1ull));
+        L[slot][0] = px; L[slot][1] = py; L[slot][2] = pz;
+        L[slot][3] = f0; L[slot][4] = f1; L[slot][5] = f2; L[slot][6] = f3;
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      for (int j = 0; j < cnt; ++j) {
+        const double qx = L[j][0], qy = L[j][1], qz = L[j][2];
+        const double g0 = L[j][3], g1 = L[j][4], g2 = L[j][5], g3 = L[j][6];
+#pragma unroll
+        for (int c = 0; c < nch; ++c) {
+          const double v0 = qx - xd[c], v1 = qy - yd[c], v2 = qz - zd[c];
+          const double r2 = v0 * v0 + v1 * v1 + v2 * v2;
+          if (has[c] && r2 <= A.r2_sf) {
+            const double S = 1. - A.r2_sf_inv * r2;
+            double S1 = S * S;
+            for (int ex = 3; ex <= A.alpha; ++ex) S1 = S * S1;
+            acc[c][0] = acc[c][0] + S1 * g0;
+            acc[c][1] = acc[c][1] + S1 * g1;
+            acc[c][2] = acc[c][2] + S1 * g2;
+            acc[c][3] = acc[c][3] + S1 * g3;
+          }
+        }
+      }
+      __builtin_amdgcn_wave_barrier();
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < nch; ++c)
+    if (has[c]) {
+      double *ps = PS + ((size_t)t * nd + c * kWave + lane) * 4;
+      ps[0] = acc[c][0]; ps[1] = acc[c][1]; ps[2] = acc[c][2]; ps[3] = acc[c][3];
+    }
+}
+
+__global__ void k_fill_u8(uint8_t *p, int64_t n, uint8_t v) {
+  const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+__global__ void k_fill_i32(int32_t *p, int64_t n, int32_t v) {
+  const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+// element id of each sorted slot, from the segment offsets
+__global__ void k_elem_of(int nElems, const int32_t *start, int32_t *elemOf) {
+  const int w = (blockIdx.x * kBlock + threadIdx.x) / kWave;
+  const int lane = lane_id();
+  if (w >= nElems) return;
+  for (int i = start[w] + lane; i < start[w + 1]; i += kWave) elemOf[i] = w;
+}
+
+}  // namespace
+
+// ======================================================================= handle
+struct pic_handle {
+  pic_mesh_desc meshDesc{};
+  pic_params par{};
+  HostMesh hm;
+  DevMesh M{};
+  hipStream_t stream = nullptr;
+  int device = 0;
+  int64_t cap = 0;      // particle capacity
+  int64_t n = 0;        // particles in the sorted arrays
+  int64_t nTotal = 0;   // caller's particle count (ids 0..nTotal-1)
+  // device buffers
+  std::vector<void *> allocs;
+  DevParts P{}, Q{};    // current and scratch particle arrays
+  int32_t *start = nullptr, *startNew = nullptr, *newElem = nullptr, *rank = nullptr, *perm = nullptr;
+  int32_t *stayCnt = nullptr, *arrCnt = nullptr, *elemOf = nullptr, *dTotal = nullptr;
+  uint8_t *insideById = nullptr;
+  double *stateById = nullptr;
+  unsigned long long *lostCount = nullptr;
+  double *E = nullptr, *PS = nullptr, *elemCorner = nullptr, *classSrc = nullptr;
+  int32_t *elemClass = nullptr, *classOff = nullptr, *classInc = nullptr;
+  double *classVol = nullptr;
+  SFArgs sf{};
+  bool stagedValid = false;   // LastPartPos/FieldAtParticle refer to the current order
+  std::vector<std::pair<int64_t, std::vector<double>>> unplaced;  // particles no element holds at set-up
+  // statistics
+  hipEvent_t ev[8] = {};
+  double msDeposit = 0, msPushTrack = 0, msSort = 0, msDepositMain = 0;
+  int nDepositLaunches = 0;
+  int64_t nSteps = 0;
+};
+
+namespace {
+
+template <typename T>
+int dalloc(pic_handle *h, T **p, size_t count) {
+  void *q = nullptr;
+  HIPCHK(hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T)));
+  h->allocs.push_back(q);
+  *p = static_cast<T *>(q);
+  return 0;
+}
+template <typename T>
+int upload(pic_handle *h, T **p, const std::vector<T> &v) {
+  if (int rc = dalloc(h, p, v.size())) return rc;
+  if (!v.empty()) HIPCHK(hipMemcpy(*p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  return 0;
+}
+
+int alloc_parts(pic_handle *h, DevParts &P, int64_t cap) {
+  int rc = 0;
+  rc |= dalloc(h, &P.x, cap); rc |= dalloc(h, &P.y, cap); rc |= dalloc(h, &P.z, cap);
+  rc |= dalloc(h, &P.vx, cap); rc |= dalloc(h, &P.vy, cap); rc |= dalloc(h, &P.vz, cap);
+  rc |= dalloc(h, &P.elem, cap); rc |= dalloc(h, &P.id, cap);
+  rc |= dalloc(h, &P.spec, cap); rc |= dalloc(h, &P.isNew, cap);
+  return rc;
+}
+
+#define LAUNCH_N(KERNEL, N, GRID, ...)                                                   \
+  switch (N) {                                                                           \
+    case 1: hipLaunchKernelGGL(KERNEL<1>, GRID, dim3(kBlock), 0, h->stream, __VA_ARGS__); break; \
+    case 2: hipLaunchKernelGGL(KERNEL<2>, GRID, dim3(kBlock), 0, h->stream, __VA_ARGS__); break; \
+    case 3: hipLaunchKernelGGL(KERNEL<3>, GRID, dim3(kBlock), 0, h->stream, __VA_ARGS__); break; \
+    case 4: hipLaunchKernelGGL(KERNEL<4>, GRID, dim3(kBlock), 0, h->stream, __VA_ARGS__); break; \
+    case 5: hipLaunchKernelGGL(KERNEL<5>, GRID, dim3(kBlock), 0, h->stream, __VA_ARGS__); break; \
+    case 6: hipLaunchKernelGGL(KERNEL<6>, GRID, dim3(kBlock), 0, h->stream, __VA_ARGS__); break; \
+    case 7: hipLaunchKernelGGL(KERNEL<7>, GRID, dim3(kBlock), 0, h->stream, __VA_ARGS__); break; \
+    default: break;                                                                      \
+  }
+
+float elapsed(hipEvent_t a, hipEvent_t b) {
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, a, b);
+  return ms;
+}
+
+int set_device(pic_handle *h) {
+  HIPCHK(hipSetDevice(h->device));
+  return 0;
+}
+
+// Deposition() on the current sorted particles
+int run_deposition(pic_handle *h, bool timed) {
+  const int ne = h->hm.nElems, N = h->hm.N;
+  const size_t nps = size_t(ne) * h->hm.nDof * 4;
+  if (timed) HIPCHK(hipEventRecord(h->ev[0], h->stream));
+  if (h->par.deposition == PIC_DEPO_NONE) {
+    HIPCHK(hipMemsetAsync(h->PS, 0, nps * sizeof(double), h->stream));
+  } else if (h->par.deposition == PIC_DEPO_CVWM) {
+    LAUNCH_N(k_cvwm_elem, N, dim3(grid_for(int64_t(ne) * kWave)), h->M, h->P, h->start, h->elemCorner);
+    if (timed) HIPCHK(hipEventRecord(h->ev[1], h->stream));
+    hipLaunchKernelGGL(k_cvwm_class, dim3(grid_for(h->hm.nClasses)), dim3(kBlock), 0, h->stream, h->hm.nClasses,
+                       h->classOff, h->classInc, h->classVol, h->elemCorner, h->classSrc);
+    LAUNCH_N(k_cvwm_dof, N, dim3(grid_for(int64_t(ne) * h->hm.nDof)), h->M, h->elemClass, h->classSrc, h->PS);
+  } else {
+    const int waves = kBlock / kWave;
+    const unsigned nblk = xcd_grid((ne + waves - 1) / waves);
+    LAUNCH_N(k_sf_gather, N, dim3(nblk), h->M, h->P, h->start, h->sf, h->PS);
+    if (timed) HIPCHK(hipEventRecord(h->ev[1], h->stream));
+  }
+  HIPCHK(hipGetLastError());
+  if (timed) HIPCHK(hipEventRecord(h->ev[2], h->stream));
+  return 0;
+}
+
+// element re-sort after tracking; newElem[i] holds the owner (-1 = gone)
+int run_sort(pic_handle *h, bool withStaged) {
+  const int ne = h->hm.nElems;
+  const int64_t n = h->n;
+  HIPCHK(hipMemsetAsync(h->arrCnt, 0, size_t(ne) * sizeof(int32_t), h->stream));
+  hipLaunchKernelGGL(k_elem_of, dim3(grid_for(int64_t(ne) * kWave)), dim3(kBlock), 0, h->stream, ne, h->start, h->elemOf);
+  hipLaunchKernelGGL(k_seg_rank, dim3(grid_for(int64_t(ne) * kWave)), dim3(kBlock), 0, h->stream, ne, h->start,
+                     h->newElem, h->stayCnt, h->arrCnt, h->rank);
+  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, h->stream, ne, h->stayCnt, h->arrCnt, h->startNew);
+  hipLaunchKernelGGL(k_dest, dim3(grid_for(n)), dim3(kBlock), 0, h->stream, n, h->elemOf, h->newElem, h->rank,
+                     h->startNew, h->stayCnt, h->perm);
+  hipLaunchKernelGGL(k_fix_arrivals, dim3(grid_for(ne)), dim3(kBlock), 0, h->stream, ne, h->startNew, h->stayCnt,
+                     h->arrCnt, h->P.id, h->perm);
+  const int32_t *total = h->startNew + ne;
+  hipLaunchKernelGGL(k_gather_state, dim3(grid_for(n)), dim3(kBlock), 0, h->stream, total, h->perm, h->P, h->Q, h->newElem);
+  if (withStaged) {
+    // keep LastPartPos / LastElem / FieldAtParticle in the new order
+    hipLaunchKernelGGL(k_gather<double>, dim3(grid_for(n)), dim3(kBlock), 0, h->stream, total, h->perm, h->P.lx, h->Q.lx, 1);
+    hipLaunchKernelGGL(k_gather<double>, dim3(grid_for(n)), dim3(kBlock), 0, h->stream, total, h->perm, h->P.ly, h->Q.ly, 1);
+    hipLaunchKernelGGL(k_gather<double>, dim3(grid_for(n)), dim3(kBlock), 0, h->stream, total, h->perm, h->P.lz, h->Q.lz, 1);
+    hipLaunchKernelGGL(k_gather<int32_t>, dim3(grid_for(n)), dim3(kBlock), 0, h->stream, total, h->perm, h->P.lastElem,
+                       h->Q.lastElem, 1);
+    hipLaunchKernelGGL(k_gather<double>, dim3(grid_for(n * 6)), dim3(kBlock), 0, h->stream, total, h->perm, h->P.F, h->Q.F, 6);
+    std::swap(h->P.lx, h->Q.lx); std::swap(h->P.ly, h->Q.ly); std::swap(h->P.lz, h->Q.lz);
+    std::swap(h->P.lastElem, h->Q.lastElem); std::swap(h->P.F, h->Q.F);
+  }
+  HIPCHK(hipGetLastError());
+  std::swap(h->P.x, h->Q.x); std::swap(h->P.y, h->Q.y); std::swap(h->P.z, h->Q.z);
+  std::swap(h->P.vx, h->Q.vx); std::swap(h->P.vy, h->Q.vy); std::swap(h->P.vz, h->Q.vz);
+  std::swap(h->P.elem, h->Q.elem); std::swap(h->P.id, h->Q.id);
+  std::swap(h->P.spec, h->Q.spec); std::swap(h->P.isNew, h->Q.isNew);
+  std::swap(h->start, h->startNew);
+  int32_t total_h = 0;
+  HIPCHK(hipMemcpyAsync(&total_h, h->start + ne, sizeof(int32_t), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  h->n = total_h;
+  return 0;
+}
+
+int require_particles(pic_handle *h) {
+  if (h->nTotal <= 0) return fail(PIC_ESTATE, "no particles set");
+  return 0;
+}
+
+}  // namespace
+
+// ========================================================================= C ABI
+extern "C" {
+
+const char *pic_last_error(void) { return g_err.c_str(); }
+
+int pic_create(const pic_mesh_desc *mesh, const pic_params *par, pic_handle **out) {
+  if (!mesh || !par || !out) return fail(PIC_EINVAL, "pic_create: null argument");
+  *out = nullptr;
+  if (par->tracking != PIC_TRACK_TRIA) return fail(PIC_EINVAL, "pic_create: only TriaTracking is implemented");
+  if (par->deposition < PIC_DEPO_NONE || par->deposition > PIC_DEPO_SF) return fail(PIC_EINVAL, "pic_create: deposition type");
+  if (par->nSpecies < 1 || par->nSpecies > kMaxSpecies) return fail(PIC_EINVAL, "pic_create: nSpecies must be 1..8");
+  if (par->refmapping_guess != 1 && par->refmapping_guess != 2 && par->refmapping_guess != 4)
+    return fail(PIC_EINVAL, "pic_create: RefMappingGuess must be 1, 2 or 4 on linear meshes");
+  if (par->deposition == PIC_DEPO_SF && par->sf_alpha < 2) return fail(PIC_EINVAL, "pic_create: PIC-shapefunction-alpha must be >= 2");
+  if (par->max_particles < 1 || par->max_particles > (int64_t(1) << 31) - 1)
+    return fail(PIC_EINVAL, "pic_create: max_particles must be in 1..2^31-1");
+  if (par->precision != PIC_PREC_FP64) return fail(PIC_EINVAL, "pic_create: only FP64 precision is implemented");
+  pic_handle *h = new pic_handle();
+  h->meshDesc = *mesh;
+  h->par = *par;
+  h->device = par->device;
+  std::string err = build_host_mesh(*mesh, *par, h->hm);
+  if (!err.empty()) { delete h; return fail(PIC_EINVAL, err); }
+  if (set_device(h)) { delete h; return PIC_EHIP; }
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) { delete h; return fail(PIC_EHIP, "stream"); }
+  for (auto &e : h->ev) (void)hipEventCreate(&e);
+  const HostMesh &hm = h->hm;
+  DevMesh &M = h->M;
+  M.nElems = hm.nElems; M.N = hm.N; M.nDof = hm.nDof; M.deposition = par->deposition;
+  M.refGuess = par->refmapping_guess; M.refEps = par->refmapping_eps; M.nSpecies = par->nSpecies;
+  int rc = 0;
+  rc |= upload(h, const_cast<double **>(&M.corner), hm.corner);
+  rc |= upload(h, const_cast<int32_t **>(&M.sideNb), hm.sideNb);
+  rc |= upload(h, const_cast<int32_t **>(&M.sideGID), hm.sideGID);
+  rc |= upload(h, const_cast<uint8_t **>(&M.sideNode), hm.sideNode);
+  rc |= upload(h, const_cast<uint8_t **>(&M.sideBC), hm.sideBC);
+  rc |= upload(h, const_cast<int8_t **>(&M.sidePV), hm.sidePV);
+  rc |= upload(h, const_cast<uint8_t **>(&M.concave), hm.concave);
+  rc |= upload(h, const_cast<double **>(&M.dXCL), hm.dXCL);
+  rc |= upload(h, const_cast<double **>(&M.bary), hm.bary);
+  rc |= upload(h, const_cast<double **>(&M.radius), hm.radius);
+  rc |= upload(h, const_cast<double **>(&M.xiBasis), hm.xiBasis);
+  rc |= upload(h, const_cast<double **>(&M.slen), hm.slen);
+  rc |= upload(h, const_cast<double **>(&M.xGPe), hm.xGPe);
+  rc |= dalloc(h, &h->E, size_t(hm.nElems) * hm.nDof * 3);
+  M.E = h->E;
+  for (int i = 0; i < 2; ++i) { M.xiCL[i] = hm.xiCL[i]; M.wBaryCL[i] = hm.wBaryCL[i]; }
+  for (int i = 0; i <= hm.N; ++i) { M.xGP[i] = hm.xGP[i]; M.wBaryGP[i] = hm.wBaryGP[i]; }
+  for (int v = 0; v < 3; ++v)
+    for (int c = 0; c < 3; ++c) M.PV[v][c] = hm.PV[v][c];
+  for (int c = 0; c < 6; ++c) M.ext[c] = par->external_field[c];
+  for (int s = 0; s < kMaxSpecies; ++s) { M.charge[s] = par->charge[s]; M.mass[s] = par->mass[s]; M.mpf[s] = par->mpf[s]; }
+  rc |= dalloc(h, &h->PS, size_t(hm.nElems) * hm.nDof * 4);
+  if (par->deposition == PIC_DEPO_CVWM) {
+    rc |= dalloc(h, &h->elemCorner, size_t(hm.nElems) * 32);
+    rc |= dalloc(h, &h->classSrc, size_t(hm.nClasses) * 4);
+    rc |= upload(h, &h->elemClass, hm.elemClass);
+    rc |= upload(h, &h->classOff, hm.classOff);
+    rc |= upload(h, &h->classInc, hm.classInc);
+    rc |= upload(h, &h->classVol, hm.classVolume);
+  }
+  if (par->deposition == PIC_DEPO_SF) {
+    rc |= upload(h, const_cast<int32_t **>(&h->sf.off), hm.sfOff);
+    rc |= upload(h, const_cast<int32_t **>(&h->sf.src), hm.sfSrc);
+    rc |= upload(h, const_cast<uint8_t **>(&h->sf.shift), hm.sfShift);
+    for (int s = 0; s < 27; ++s)
+      for (int c = 0; c < 3; ++c) h->sf.sh[s][c] = hm.shift[s][c];
+    const double a = par->sf_alpha, r = par->sf_radius;
+    h->sf.r_sf = r;
+    h->sf.r2_sf = r * r;
+    h->sf.r2_sf_inv = 1. / h->sf.r2_sf;
+    // InitShapeFunctionDimensionalty 3D branch (pic_depo_shapefunction_tools.f90:1243-1246)
+    const double BetaFac = std::tgamma(1.5) * std::tgamma(a + 1.) / std::tgamma(1.5 + (a + 1.));
+    h->sf.w_sf = 1. / (2. * BetaFac * a + 2 * BetaFac) * (a + 1.) / (M_PI * (r * r * r));
+    h->sf.alpha = par->sf_alpha;
+  }
+  // particles
+  const int64_t cap = par->max_particles;
+  h->cap = cap;
+  rc |= alloc_parts(h, h->P, cap);
+  rc |= alloc_parts(h, h->Q, cap);
+  rc |= dalloc(h, &h->P.lx, cap); rc |= dalloc(h, &h->P.ly, cap); rc |= dalloc(h, &h->P.lz, cap);
+  rc |= dalloc(h, &h->Q.lx, cap); rc |= dalloc(h, &h->Q.ly, cap); rc |= dalloc(h, &h->Q.lz, cap);
+  rc |= dalloc(h, &h->P.lastElem, cap); rc |= dalloc(h, &h->Q.lastElem, cap);
+  rc |= dalloc(h, &h->P.F, cap * 6); rc |= dalloc(h, &h->Q.F, cap * 6);
+  rc |= dalloc(h, &h->start, hm.nElems + 1);
+  rc |= dalloc(h, &h->startNew, hm.nElems + 1);
+  rc |= dalloc(h, &h->newElem, cap);
+  rc |= dalloc(h, &h->rank, cap);
+  rc |= dalloc(h, &h->perm, cap);
+  rc |= dalloc(h, &h->elemOf, cap);
+  rc |= dalloc(h, &h->stayCnt, hm.nElems);
+  rc |= dalloc(h, &h->arrCnt, hm.nElems);
+  rc |= dalloc(h, &h->insideById, cap);
+  rc |= dalloc(h, &h->stateById, cap * 6);
+  rc |= dalloc(h, &h->lostCount, 1);
+  if (rc) { pic_destroy(h); return PIC_EHIP; }
+  HIPCHK(hipMemset(h->lostCount, 0, sizeof(unsigned long long)));
+  HIPCHK(hipMemset(h->E, 0, size_t(hm.nElems) * hm.nDof * 3 * sizeof(double)));
+  HIPCHK(hipMemset(h->PS, 0, size_t(hm.nElems) * hm.nDof * 4 * sizeof(double)));
+  *out = h;
+  return PIC_OK;
+}
+
+int pic_destroy(pic_handle *h) {
+  if (!h) return PIC_OK;
+  (void)hipSetDevice(h->device);
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  for (void *p : h->allocs) (void)hipFree(p);
+  for (auto &e : h->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+  return PIC_OK;
+}
+
+int pic_set_field(pic_handle *h, const double *E) {
+  if (!h || !E) return fail(PIC_EINVAL, "pic_set_field: null argument");
+  if (set_device(h)) return PIC_EHIP;
+  HIPCHK(hipMemcpyAsync(h->E, E, size_t(h->hm.nElems) * h->hm.nDof * 3 * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return PIC_OK;
+}
+
+int pic_set_field_device(pic_handle *h, const double *E_dev) {
+  if (!h || !E_dev) return fail(PIC_EINVAL, "pic_set_field_device: null argument");
+  if (set_device(h)) return PIC_EHIP;
+  HIPCHK(hipMemcpyAsync(h->E, E_dev, size_t(h->hm.nElems) * h->hm.nDof * 3 * sizeof(double), hipMemcpyDeviceToDevice,
+                        h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return PIC_OK;
+}
+
+// SinglePointToElement on the host (TriaTracking inside test), used for
+// particles handed over without an element.
+static int locate_host(const HostMesh &hm, const double x[3]) {
+  // candidate elements by barycentre distance <= radius, nearest first
+  std::vector<std::pair<double, int>> cand;
+  for (int e = 0; e < hm.nElems; ++e) {
+    const double d0 = x[0] - hm.bary[e * 3], d1 = x[1] - hm.bary[e * 3 + 1], d2 = x[2] - hm.bary[e * 3 + 2];
+    const double d2s = d0 * d0 + d1 * d1 + d2 * d2;
+    if (d2s <= hm.radius[e] * hm.radius[e]) cand.emplace_back(d2s, e);
+  }
+  std::stable_sort(cand.begin(), cand.end(), [](auto &a, auto &b) { return a.first < b.first; });
+  for (auto &c : cand) {
+    const int e = c.second;
+    bool in = true;
+    for (int l = 0; l < 6 && in; ++l) {
+      const uint8_t *sn = &hm.sideNode[(e * 6 + l) * 4];
+      const double *C = &hm.corner[size_t(e) * 24];
+      double A[3][4];
+      for (int q = 0; q < 4; ++q)
+        for (int k = 0; k < 3; ++k) A[k][q] = C[sn[q] * 3 + k] - x[k];
+      double cp0 = A[1][0] * A[2][2] - A[2][0] * A[1][2];
+      double cp1 = A[2][0] * A[0][2] - A[0][0] * A[2][2];
+      double cp2 = A[0][0] * A[1][2] - A[1][0] * A[0][2];
+      double d1 = -(cp0 * A[0][1] + cp1 * A[1][1] + cp2 * A[2][1]);
+      double d2 = cp0 * A[0][3] + cp1 * A[1][3] + cp2 * A[2][3];
+      bool neg = d1 < 0 || d2 < 0, pos = !(d1 < 0) || !(d2 < 0);
+      if ((hm.concave[e] >> l) & 1) { if (!pos) in = false; }
+      else if (neg) in = false;
+    }
+    if (in) return e;
+  }
+  return -1;
+}
+
+int pic_set_particles(pic_handle *h, int64_t n, const double *PartState, const int32_t *elem, const int32_t *species,
+                      const uint8_t *is_new) {
+  if (!h || (n > 0 && !PartState)) return fail(PIC_EINVAL, "pic_set_particles: null argument");
+  if (n < 0 || n > h->cap) return fail(PIC_ECAPACITY, "pic_set_particles: more particles than max_particles");
+  if (set_device(h)) return PIC_EHIP;
+  const int ne = h->hm.nElems;
+  std::vector<int32_t> el(n), sp(n, 0);
+  std::vector<uint8_t> ins(n, 1);
+  for (int64_t i = 0; i < n; ++i) {
+    int e = elem ? elem[i] : 0;
+    if (e < 0 || e > ne) return fail(PIC_EINVAL, "pic_set_particles: element id out of range");
+    if (e == 0) e = locate_host(h->hm, &PartState[i * 6]) + 1;
+    el[i] = e - 1;
+    if (el[i] < 0) ins[i] = 0;
+    sp[i] = species ? species[i] - 1 : 0;
+    if (sp[i] < 0 || sp[i] >= h->par.nSpecies) return fail(PIC_EINVAL, "pic_set_particles: species out of range");
+  }
+  // counting sort by element, stable in caller order
+  std::vector<int64_t> cnt(ne + 1, 0);
+  for (int64_t i = 0; i < n; ++i)
+    if (el[i] >= 0) cnt[el[i] + 1]++;
+  for (int e = 0; e < ne; ++e) cnt[e + 1] += cnt[e];
+  const int64_t nin = cnt[ne];
+  std::vector<int32_t> start(ne + 1);
+  for (int e = 0; e <= ne; ++e) start[e] = int32_t(cnt[e]);
+  std::vector<int64_t> order(nin);
+  {
+    std::vector<int64_t> fill(cnt.begin(), cnt.end() - 1);
+    for (int64_t i = 0; i < n; ++i)
+      if (el[i] >= 0) order[fill[el[i]]++] = i;
+  }
+  std::vector<double> buf(nin);
+  double *dst[6] = {h->P.x, h->P.y, h->P.z, h->P.vx, h->P.vy, h->P.vz};
+  for (int c = 0; c < 6; ++c) {
+    for (int64_t j = 0; j < nin; ++j) buf[j] = PartState[order[j] * 6 + c];
+    HIPCHK(hipMemcpy(dst[c], buf.data(), nin * sizeof(double), hipMemcpyHostToDevice));
+  }
+  std::vector<int32_t> ib(nin);
+  std::vector<uint8_t> ub(nin);
+  for (int64_t j = 0; j < nin; ++j) ib[j] = el[order[j]];
+  HIPCHK(hipMemcpy(h->P.elem, ib.data(), nin * sizeof(int32_t), hipMemcpyHostToDevice));
+  for (int64_t j = 0; j < nin; ++j) ib[j] = int32_t(order[j]);
+  HIPCHK(hipMemcpy(h->P.id, ib.data(), nin * sizeof(int32_t), hipMemcpyHostToDevice));
+  for (int64_t j = 0; j < nin; ++j) ub[j] = uint8_t(sp[order[j]]);
+  HIPCHK(hipMemcpy(h->P.spec, ub.data(), nin, hipMemcpyHostToDevice));
+  for (int64_t j = 0; j < nin; ++j) ub[j] = is_new ? is_new[order[j]] : 0;
+  HIPCHK(hipMemcpy(h->P.isNew, ub.data(), nin, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(h->start, start.data(), (ne + 1) * sizeof(int32_t), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(h->insideById, ins.data(), n, hipMemcpyHostToDevice));
+  HIPCHK(hipMemset(h->lostCount, 0, sizeof(unsigned long long)));
+  // particles no element holds keep their state for pic_get_particles
+  h->unplaced.clear();
+  for (int64_t i = 0; i < n; ++i)
+    if (!ins[i]) h->unplaced.emplace_back(i, std::vector<double>(PartState + i * 6, PartState + i * 6 + 6));
+  h->n = nin;
+  h->nTotal = n;
+  h->stagedValid = false;
+  return PIC_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int pic_deposition(pic_handle *h) {
+  if (!h) return fail(PIC_EINVAL, "pic_deposition: null handle");
+  if (set_device(h)) return PIC_EHIP;
+  if (int rc = run_deposition(h, true)) return rc;
+  HIPCHK(hipStreamSynchronize(h->stream));
+  h->msDeposit += elapsed(h->ev[0], h->ev[2]);
+  if (h->par.deposition != PIC_DEPO_NONE) { h->msDepositMain += elapsed(h->ev[0], h->ev[1]); h->nDepositLaunches++; }
+  return PIC_OK;
+}
+
+int pic_interpolate_field_to_particle(pic_handle *h) {
+  if (!h) return fail(PIC_EINVAL, "pic_interpolate_field_to_particle: null handle");
+  if (int rc = require_particles(h)) return rc;
+  if (set_device(h)) return PIC_EHIP;
+  if (h->n > 0) { LAUNCH_N(k_interp, h->hm.N, dim3(grid_for(h->n)), h->M, h->P, h->n); }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(h->stream));
+  h->stagedValid = true;
+  return PIC_OK;
+}
+
+int pic_push_boris_leapfrog(pic_handle *h, double dt) {
+  if (!h) return fail(PIC_EINVAL, "pic_push_boris_leapfrog: null handle");
+  if (int rc = require_particles(h)) return rc;
+  if (!h->stagedValid) return fail(PIC_ESTATE, "pic_push_boris_leapfrog: call pic_interpolate_field_to_particle first");
+  if (set_device(h)) return PIC_EHIP;
+  if (h->n > 0) hipLaunchKernelGGL(k_push, dim3(grid_for(h->n)), dim3(kBlock), 0, h->stream, h->M, h->P, h->n, dt);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return PIC_OK;
+}
+
+int pic_perform_tracking(pic_handle *h) {
+  if (!h) return fail(PIC_EINVAL, "pic_perform_tracking: null handle");
+  if (int rc = require_particles(h)) return rc;
+  if (!h->stagedValid) return fail(PIC_ESTATE, "pic_perform_tracking: call pic_push_boris_leapfrog first");
+  if (set_device(h)) return PIC_EHIP;
+  if (h->n > 0) {
+    hipLaunchKernelGGL(k_track, dim3(grid_for(h->n)), dim3(kBlock), 0, h->stream, h->M, h->P, h->n, h->newElem,
+                       h->lostCount, h->insideById, h->stateById);
+    HIPCHK(hipGetLastError());
+    if (int rc = run_sort(h, true)) return rc;
+  }
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return PIC_OK;
+}
+
+int pic_time_step(pic_handle *h, double dt, int32_t nsteps) {
+  if (!h) return fail(PIC_EINVAL, "pic_time_step: null handle");
+  if (int rc = require_particles(h)) return rc;
+  if (nsteps < 0) return fail(PIC_EINVAL, "pic_time_step: nsteps < 0");
+  if (set_device(h)) return PIC_EHIP;
+  for (int it = 0; it < nsteps; ++it) {
+    if (int rc = run_deposition(h, true)) return rc;
+    // LastPartPos <- PartState, interpolate, push, track (one kernel)
+    HIPCHK(hipEventRecord(h->ev[3], h->stream));
+    if (h->n > 0) {
+      LAUNCH_N(k_push_track, h->hm.N, dim3(grid_for(h->n)), h->M, h->P, h->n, dt, h->newElem, h->lostCount,
+               h->insideById, h->stateById);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(h->ev[4], h->stream));
+    if (h->n > 0)
+      if (int rc = run_sort(h, false)) return rc;  // synchronises the stream
+    HIPCHK(hipEventRecord(h->ev[5], h->stream));
+    HIPCHK(hipEventSynchronize(h->ev[5]));
+    h->msDeposit += elapsed(h->ev[0], h->ev[2]);
+    if (h->par.deposition != PIC_DEPO_NONE) { h->msDepositMain += elapsed(h->ev[0], h->ev[1]); h->nDepositLaunches++; }
+    h->msPushTrack += elapsed(h->ev[3], h->ev[4]);
+    h->msSort += elapsed(h->ev[4], h->ev[5]);
+    h->nSteps++;
+  }
+  h->stagedValid = false;
+  return PIC_OK;
+}
+
+int pic_get_particles(pic_handle *h, double *PartState, int32_t *elem, uint8_t *inside) {
+  if (!h) return fail(PIC_EINVAL, "pic_get_particles: null handle");
+  if (set_device(h)) return PIC_EHIP;
+  HIPCHK(hipStreamSynchronize(h->stream));
+  const int64_t n = h->n, nt = h->nTotal;
+  std::vector<double> a(n);
+  std::vector<int32_t> id(n), el(n);
+  std::vector<uint8_t> ins(nt);
+  HIPCHK(hipMemcpy(id.data(), h->P.id, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(el.data(), h->P.elem, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(ins.data(), h->insideById, nt, hipMemcpyDeviceToHost));
+  if (PartState) {
+    double *src[6] = {h->P.x, h->P.y, h->P.z, h->P.vx, h->P.vy, h->P.vz};
+    for (int c = 0; c < 6; ++c) {
+      HIPCHK(hipMemcpy(a.data(), src[c], n * sizeof(double), hipMemcpyDeviceToHost));
+      for (int64_t j = 0; j < n; ++j) PartState[int64_t(id[j]) * 6 + c] = a[j];
+    }
+    int64_t nlost = 0;
+    for (int64_t i = 0; i < nt; ++i) nlost += ins[i] ? 0 : 1;
+    if (nlost > 0) {
+      std::vector<double> st(size_t(nt) * 6);
+      HIPCHK(hipMemcpy(st.data(), h->stateById, size_t(nt) * 6 * sizeof(double), hipMemcpyDeviceToHost));
+      for (int64_t i = 0; i < nt; ++i)
+        if (!ins[i])
+          for (int c = 0; c < 6; ++c) PartState[i * 6 + c] = st[i * 6 + c];
+      for (auto &u : h->unplaced)
+        for (int c = 0; c < 6; ++c) PartState[u.first * 6 + c] = u.second[c];
+    }
+  }
+  if (elem) {
+    for (int64_t i = 0; i < nt; ++i) elem[i] = 0;
+    for (int64_t j = 0; j < n; ++j) elem[id[j]] = el[j] + 1;
+  }
+  if (inside)
+    for (int64_t i = 0; i < nt; ++i) inside[i] = ins[i];
+  return PIC_OK;
+}
+
+int pic_get_last_part_pos(pic_handle *h, double *L) {
+  if (!h || !L) return fail(PIC_EINVAL, "pic_get_last_part_pos: null argument");
+  if (!h->stagedValid) return fail(PIC_ESTATE, "pic_get_last_part_pos: no staged push since the last fused step");
+  if (set_device(h)) return PIC_EHIP;
+  const int64_t n = h->n;
+  std::vector<int32_t> id(n);
+  std::vector<double> a(n);
+  HIPCHK(hipMemcpy(id.data(), h->P.id, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+  double *src[3] = {h->P.lx, h->P.ly, h->P.lz};
+  for (int c = 0; c < 3; ++c) {
+    HIPCHK(hipMemcpy(a.data(), src[c], n * sizeof(double), hipMemcpyDeviceToHost));
+    for (int64_t j = 0; j < n; ++j) L[int64_t(id[j]) * 3 + c] = a[j];
+  }
+  return PIC_OK;
+}
+
+int pic_get_field_at_particle(pic_handle *h, double *F) {
+  if (!h || !F) return fail(PIC_EINVAL, "pic_get_field_at_particle: null argument");
+  if (!h->stagedValid) return fail(PIC_ESTATE, "pic_get_field_at_particle: call pic_interpolate_field_to_particle first");
+  if (set_device(h)) return PIC_EHIP;
+  const int64_t n = h->n;
+  std::vector<int32_t> id(n);
+  std::vector<double> a(size_t(n) * 6);
+  HIPCHK(hipMemcpy(id.data(), h->P.id, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(a.data(), h->P.F, size_t(n) * 6 * sizeof(double), hipMemcpyDeviceToHost));
+  for (int64_t j = 0; j < n; ++j)
+    for (int c = 0; c < 6; ++c) F[int64_t(id[j]) * 6 + c] = a[j * 6 + c];
+  return PIC_OK;
+}
+
+int pic_get_part_source(pic_handle *h, double *PS) {
+  if (!h || !PS) return fail(PIC_EINVAL, "pic_get_part_source: null argument");
+  if (set_device(h)) return PIC_EHIP;
+  HIPCHK(hipStreamSynchronize(h->stream));
+  HIPCHK(hipMemcpy(PS, h->PS, size_t(h->hm.nElems) * h->hm.nDof * 4 * sizeof(double), hipMemcpyDeviceToHost));
+  return PIC_OK;
+}
+
+int pic_part_source_device(pic_handle *h, const double **PS_dev) {
+  if (!h || !PS_dev) return fail(PIC_EINVAL, "pic_part_source_device: null argument");
+  *PS_dev = h->PS;
+  return PIC_OK;
+}
+
+int pic_get_stats(pic_handle *h, pic_stats *st) {
+  if (!h || !st) return fail(PIC_EINVAL, "pic_get_stats: null argument");
+  if (set_device(h)) return PIC_EHIP;
+  unsigned long long lost = 0;
+  HIPCHK(hipMemcpy(&lost, h->lostCount, sizeof(lost), hipMemcpyDeviceToHost));
+  st->n_particles = h->n;
+  st->n_lost = int64_t(lost);
+  st->n_steps = h->nSteps;
+  st->ms_deposit = h->msDeposit;
+  st->ms_push_track = h->msPushTrack;
+  st->ms_sort = h->msSort;
+  st->n_deposit_launches = h->nDepositLaunches;
+  st->ms_deposit_main = h->msDepositMain;
+  return PIC_OK;
+}
+
+int pic_reset_stats(pic_handle *h) {
+  if (!h) return fail(PIC_EINVAL, "pic_reset_stats: null handle");
+  h->msDeposit = h->msPushTrack = h->msSort = h->msDepositMain = 0;
+  h->nDepositLaunches = 0;
+  h->nSteps = 0;
+  return PIC_OK;
+}
+
+int pic_synchronize(pic_handle *h) {
+  if (!h) return fail(PIC_EINVAL, "pic_synchronize: null handle");
+  if (set_device(h)) return PIC_EHIP;
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return PIC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,67 @@
+// Internal declarations of the MI355X particle stage: the host-side mesh
+// tables (built once from the HOPR arrays) and the device views the kernels
+// read.  Memory layout in HBM:
+//   particles   structure of arrays, sorted by element (segment offsets
+//               elemStart[nElems+1]); x,y,z,vx,vy,vz FP64, elem/id int32
+//   per element corner coordinates, side topology, Newton tables, E field,
+//               Gauss-point positions: element-major, read by the particles
+//               of one element segment back to back.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "pic_stage.h"
+
+namespace pic {
+
+constexpr int kMaxN = 7;          // DG degree supported by the kernels
+constexpr int kMaxNGeo = 1;       // geometry degree supported by the tracking kernels
+constexpr int kMaxSpecies = 8;
+
+// Host-side mesh tables: the reference's particle-mesh arrays, derived from
+// the HOPR arrays exactly as its init routines derive them.
+struct HostMesh {
+  int nElems = 0, nSides = 0, nNodes = 0, NGeo = 1, N = 1, nDof = 8, nGP = 8;
+  int nUnique = 0, nClasses = 0;
+  // tracking (particle_mesh_tools.f90:1800-1930)
+  std::vector<double> corner;      // [e][8][3], tensor corner order t=i+2j+4k
+  std::vector<int32_t> sideNb;     // [e][6] 0-based neighbour element or -1
+  std::vector<int32_t> sideGID;    // [e][6] 1-based global side index
+  std::vector<uint8_t> sideNode;   // [e][6][4] tensor corner index of ElemSideNodeID
+  std::vector<uint8_t> sideBC;     // [e][6] 0 interior, PIC_BC_* on boundary sides
+  std::vector<int8_t> sidePV;      // [e][6] periodic vector id (BC_ALPHA)
+  std::vector<uint8_t> concave;    // [e] bit l set: ConcaveElemSide(l+1)
+  // reference mapping (metrics.f90, particle_mesh_build.f90)
+  std::vector<double> XCL;         // [e][nGP][3]
+  std::vector<double> dXCL;        // [e][nGP][3(x)][3(d)]
+  std::vector<double> bary;        // [e][3]  ElemBaryNGeo (corner mean)
+  std::vector<double> radius;      // [e]     ElemRadiusNGeo
+  std::vector<double> xiBasis;     // [e][6][3]
+  std::vector<double> slen;        // [e][6]
+  std::vector<double> xGPe;        // [e][nDof][3] Elem_xGP
+  std::vector<double> sJ;          // [e][nDof]
+  // 1D bases
+  double xiCL[8] = {0}, wBaryCL[8] = {0};
+  double xGP[kMaxN + 1] = {0}, wGP[kMaxN + 1] = {0}, wBaryGP[kMaxN + 1] = {0};
+  // cell_volweight_mean: node classes (a unique node and its periodic twins)
+  std::vector<int32_t> elemClass;  // [e][8] class of each CGNS corner
+  std::vector<int32_t> classOff;   // [nClasses+1] CSR over (e*8+corner) incidences
+  std::vector<int32_t> classInc;
+  std::vector<double> classVolume; // NodeVolume of the class
+  // shape function: target element -> (source element, periodic shift) list
+  std::vector<int32_t> sfOff;      // [nElems+1]
+  std::vector<int32_t> sfSrc;      // source element
+  std::vector<uint8_t> sfShift;    // index into shift table (27 combos)
+  double shift[27][3] = {{0}};
+  int nPV = 0;
+  double PV[3][3] = {{0}};
+  // domain bounding box
+  double lo[3] = {0}, hi[3] = {0};
+};
+
+// Builds every table above.  Returns empty string on success, else a message.
+std::string build_host_mesh(const pic_mesh_desc &m, const pic_params &p, HostMesh &hm);
+
+}  // namespace pic

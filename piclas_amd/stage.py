@@ -1,0 +1,232 @@
+"""Python mirror of the reference's particle-stage interface over the C ABI.
+
+The reference's stepper (src/timedisc/timedisc_TimeStepPoissonByBorisLeapfrog.f90)
+calls Deposition(), InterpolateFieldToParticle(), an inline Boris push and
+PerformTracking() on module arrays.  ``PICStage`` exposes the same four
+operations (under the reference names and snake_case aliases) on a handle of
+libpicstage.so, with the arrays passed in the reference's layout:
+
+  PartState (nPart, 6)   E (nElems, (N+1)^3, 3)   PartSource (nElems, (N+1)^3, 4)
+
+The HIP library is required: loading fails loudly when it is missing.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from .mesh import HoprMesh
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpicstage.so")
+
+TRACK_REFMAPPING, TRACK_TRIA = 1, 3
+DEPO_NONE, DEPO_CVWM, DEPO_SF = 0, 1, 2
+PREC_FP64, PREC_MIXED = 0, 1
+
+ELEMENTARY_CHARGE = 1.602176634e-19   # globals_vars.f90:54
+ELECTRON_MASS = 9.1093826e-31         # globals_vars.f90:57
+EPS0 = 8.8541878176e-12               # globals_vars.f90:84
+
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_f64p = ctypes.POINTER(ctypes.c_double)
+_u8p = ctypes.POINTER(ctypes.c_uint8)
+
+
+class PicMeshDesc(ctypes.Structure):
+    _fields_ = [("nElems", ctypes.c_int32), ("nSides", ctypes.c_int32), ("nNodes", ctypes.c_int32),
+                ("NGeo", ctypes.c_int32), ("N", ctypes.c_int32),
+                ("ElemInfo", _i32p), ("SideInfo", _i32p), ("NodeCoords", _f64p), ("GlobalNodeIDs", _i32p),
+                ("nBCs", ctypes.c_int32), ("BCType", _i32p), ("PartBoundCond", _i32p),
+                ("nPeriodicVectors", ctypes.c_int32), ("PeriodicVectors", _f64p)]
+
+
+class PicParams(ctypes.Structure):
+    _fields_ = [("tracking", ctypes.c_int32), ("deposition", ctypes.c_int32),
+                ("sf_radius", ctypes.c_double), ("sf_alpha", ctypes.c_int32),
+                ("refmapping_guess", ctypes.c_int32), ("refmapping_eps", ctypes.c_double),
+                ("external_field", ctypes.c_double * 6), ("nSpecies", ctypes.c_int32),
+                ("charge", ctypes.c_double * 8), ("mass", ctypes.c_double * 8), ("mpf", ctypes.c_double * 8),
+                ("precision", ctypes.c_int32), ("max_particles", ctypes.c_int64), ("device", ctypes.c_int32)]
+
+
+class PicStats(ctypes.Structure):
+    _fields_ = [("n_particles", ctypes.c_int64), ("n_lost", ctypes.c_int64), ("n_steps", ctypes.c_int64),
+                ("ms_deposit", ctypes.c_double), ("ms_push_track", ctypes.c_double), ("ms_sort", ctypes.c_double),
+                ("n_deposit_launches", ctypes.c_int32), ("ms_deposit_main", ctypes.c_double)]
+
+
+# Entry points declared in include/pic_stage.h: name -> (restype, argtypes)
+_H = ctypes.c_void_p
+EXPORTS = {
+    "pic_create": (ctypes.c_int, [ctypes.POINTER(PicMeshDesc), ctypes.POINTER(PicParams), ctypes.POINTER(ctypes.c_void_p)]),
+    "pic_destroy": (ctypes.c_int, [_H]),
+    "pic_last_error": (ctypes.c_char_p, []),
+    "pic_set_field": (ctypes.c_int, [_H, _f64p]),
+    "pic_set_field_device": (ctypes.c_int, [_H, ctypes.c_void_p]),
+    "pic_set_particles": (ctypes.c_int, [_H, ctypes.c_int64, _f64p, _i32p, _i32p, _u8p]),
+    "pic_deposition": (ctypes.c_int, [_H]),
+    "pic_interpolate_field_to_particle": (ctypes.c_int, [_H]),
+    "pic_push_boris_leapfrog": (ctypes.c_int, [_H, ctypes.c_double]),
+    "pic_perform_tracking": (ctypes.c_int, [_H]),
+    "pic_time_step": (ctypes.c_int, [_H, ctypes.c_double, ctypes.c_int32]),
+    "pic_get_particles": (ctypes.c_int, [_H, _f64p, _i32p, _u8p]),
+    "pic_get_last_part_pos": (ctypes.c_int, [_H, _f64p]),
+    "pic_get_field_at_particle": (ctypes.c_int, [_H, _f64p]),
+    "pic_get_part_source": (ctypes.c_int, [_H, _f64p]),
+    "pic_part_source_device": (ctypes.c_int, [_H, ctypes.POINTER(ctypes.c_void_p)]),
+    "pic_get_stats": (ctypes.c_int, [_H, ctypes.POINTER(PicStats)]),
+    "pic_reset_stats": (ctypes.c_int, [_H]),
+    "pic_synchronize": (ctypes.c_int, [_H]),
+}
+
+_lib = None
+
+
+def load_library(path=LIB_PATH):
+    """Load libpicstage.so; raises if it was not built (no fallback path exists)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(path):
+            raise RuntimeError("libpicstage.so not found at %s: build it with __graft_entry__.build()" % path)
+        L = ctypes.CDLL(path)
+        for name, (res, args) in EXPORTS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+class StageError(RuntimeError):
+    pass
+
+
+def _chk(rc, what):
+    if rc != 0:
+        raise StageError("%s failed (%d): %s" % (what, rc, load_library().pic_last_error().decode()))
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+class PICStage:
+    """One particle stage on one GPU (a pic_handle)."""
+
+    def __init__(self, mesh: HoprMesh, N, deposition=DEPO_NONE, sf_radius=0.0, sf_alpha=2, tracking=TRACK_TRIA,
+                 refmapping_guess=1, refmapping_eps=1e-4, external_field=(0, 0, 0, 0, 0, 0),
+                 charge=(-ELEMENTARY_CHARGE,), mass=(ELECTRON_MASS,), mpf=(1.0,), max_particles=1 << 20,
+                 precision=PREC_FP64, device=0):
+        L = load_library()
+        self._keep = [np.ascontiguousarray(mesh.ElemInfo, np.int32), np.ascontiguousarray(mesh.SideInfo, np.int32),
+                      np.ascontiguousarray(mesh.NodeCoords, np.float64),
+                      np.ascontiguousarray(mesh.GlobalNodeIDs, np.int32),
+                      np.ascontiguousarray(mesh.BCType, np.int32), np.ascontiguousarray(mesh.PartBoundCond, np.int32),
+                      np.ascontiguousarray(mesh.PeriodicVectors, np.float64).reshape(-1, 3)]
+        ei, si, nc, gid, bct, pbc, pv = self._keep
+        md = PicMeshDesc(ei.shape[0], si.shape[0], nc.shape[0], mesh.NGeo, N, _p(ei, _i32p), _p(si, _i32p),
+                         _p(nc, _f64p), _p(gid, _i32p), bct.shape[0], _p(bct, _i32p), _p(pbc, _i32p),
+                         pv.shape[0], _p(pv, _f64p))
+        par = PicParams()
+        par.tracking, par.deposition = tracking, deposition
+        par.sf_radius, par.sf_alpha = sf_radius, sf_alpha
+        par.refmapping_guess, par.refmapping_eps = refmapping_guess, refmapping_eps
+        for i in range(6):
+            par.external_field[i] = external_field[i]
+        par.nSpecies = len(charge)
+        for i in range(len(charge)):
+            par.charge[i], par.mass[i], par.mpf[i] = charge[i], mass[i], mpf[i]
+        par.precision, par.max_particles, par.device = precision, max_particles, device
+        h = ctypes.c_void_p()
+        _chk(L.pic_create(ctypes.byref(md), ctypes.byref(par), ctypes.byref(h)), "pic_create")
+        self.h = h
+        self.nElems, self.N, self.nDof = ei.shape[0], N, (N + 1) ** 3
+        self.nPart = 0
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            load_library().pic_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ---------------------------------------------------------------- inputs
+    def set_field(self, E):
+        E = np.ascontiguousarray(E, np.float64)
+        if E.size != self.nElems * self.nDof * 3:
+            raise ValueError("E must have nElems*(N+1)^3*3 entries")
+        _chk(load_library().pic_set_field(self.h, _p(E, _f64p)), "pic_set_field")
+
+    def set_particles(self, PartState, elem=None, species=None, is_new=None):
+        ps = np.ascontiguousarray(PartState, np.float64).reshape(-1, 6)
+        n = ps.shape[0]
+        el = None if elem is None else np.ascontiguousarray(elem, np.int32)
+        sp = None if species is None else np.ascontiguousarray(species, np.int32)
+        nw = None if is_new is None else np.ascontiguousarray(is_new, np.uint8)
+        _chk(load_library().pic_set_particles(self.h, n, _p(ps, _f64p), None if el is None else _p(el, _i32p),
+                                              None if sp is None else _p(sp, _i32p),
+                                              None if nw is None else _p(nw, _u8p)), "pic_set_particles")
+        self.nPart = n
+
+    # ---------------------------------------------------------------- the stage
+    def deposition(self):
+        _chk(load_library().pic_deposition(self.h), "pic_deposition")
+
+    def interpolate_field_to_particle(self):
+        _chk(load_library().pic_interpolate_field_to_particle(self.h), "pic_interpolate_field_to_particle")
+
+    def push_boris_leapfrog(self, dt):
+        _chk(load_library().pic_push_boris_leapfrog(self.h, dt), "pic_push_boris_leapfrog")
+
+    def perform_tracking(self):
+        _chk(load_library().pic_perform_tracking(self.h), "pic_perform_tracking")
+
+    def time_step(self, dt, nsteps=1):
+        _chk(load_library().pic_time_step(self.h, dt, nsteps), "pic_time_step")
+
+    # reference names
+    Deposition = deposition
+    InterpolateFieldToParticle = interpolate_field_to_particle
+    PerformTracking = perform_tracking
+    TimeStepPoissonByBorisLeapfrog = time_step
+
+    # ---------------------------------------------------------------- outputs
+    def particles(self):
+        ps = np.zeros((self.nPart, 6))
+        el = np.zeros(self.nPart, np.int32)
+        ins = np.zeros(self.nPart, np.uint8)
+        _chk(load_library().pic_get_particles(self.h, _p(ps, _f64p), _p(el, _i32p), _p(ins, _u8p)), "pic_get_particles")
+        return ps, el, ins.astype(bool)
+
+    def last_part_pos(self):
+        a = np.zeros((self.nPart, 3))
+        _chk(load_library().pic_get_last_part_pos(self.h, _p(a, _f64p)), "pic_get_last_part_pos")
+        return a
+
+    def field_at_particle(self):
+        a = np.zeros((self.nPart, 6))
+        _chk(load_library().pic_get_field_at_particle(self.h, _p(a, _f64p)), "pic_get_field_at_particle")
+        return a
+
+    def part_source(self):
+        a = np.zeros((self.nElems, self.nDof, 4))
+        _chk(load_library().pic_get_part_source(self.h, _p(a, _f64p)), "pic_get_part_source")
+        return a
+
+    def stats(self):
+        st = PicStats()
+        _chk(load_library().pic_get_stats(self.h, ctypes.byref(st)), "pic_get_stats")
+        return {k: getattr(st, k) for k, _ in PicStats._fields_}
+
+    def reset_stats(self):
+        _chk(load_library().pic_reset_stats(self.h), "pic_reset_stats")
+
+    def synchronize(self):
+        _chk(load_library().pic_synchronize(self.h), "pic_synchronize")

@@ -1,0 +1,20 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP particle stage)")
+
+
+@pytest.fixture(scope="session")
+def oracle_mod():
+    import oracle as O
+    if not os.path.exists(O.LIB_PATH):
+        O.build()
+    return O

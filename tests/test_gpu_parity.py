@@ -1,0 +1,210 @@
+"""Parity of the HIP particle stage with the CPU oracle (and with the
+reference's own golden deposition), through the C ABI.
+
+Bars (from the north-star parity definition):
+  element ownership   bit-exact, except particles within 1e-12 of a face
+  positions           |dx| <= 1e-12 * domain size          (FP64)
+  velocities          |dv| <= 1e-12 * max|v|               (FP64)
+  deposited source    |drho| <= 1e-12 * max|rho|            (FP64; the sums
+                      run in another order, so bit equality is not expected)
+"""
+import os
+
+import numpy as np
+import pytest
+
+from piclas_amd import mesh as M
+from piclas_amd import stage, workloads as W
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def run_pair(O, w, steps, deposition=None, staged=False, **over):
+    kw = W.stage_kwargs(w)
+    if deposition is not None:
+        kw["deposition"] = deposition
+    kw.update(over)
+    print("case %s steps=%d depo=%d n=%d staged=%s" % (w.name, steps, kw["deposition"], len(w.PartState), staged),
+          flush=True)
+    s = stage.PICStage(w.mesh, w.N, max_particles=len(w.PartState), **kw)
+    s.set_field(w.E)
+    s.set_particles(w.PartState, elem=w.elem, species=w.species)
+    o = O.Oracle(w.mesh, w.N, deposition=kw["deposition"], sf_radius=kw["sf_radius"], sf_alpha=kw["sf_alpha"],
+                 external_field=kw["external_field"], charge=kw["charge"], mass=kw["mass"], mpf=kw["mpf"])
+    o.set_field(w.E)
+    o.set_particles(w.PartState, elem=w.elem, species=w.species)
+    if staged:
+        for _ in range(steps):
+            s.deposition(); o.deposition()
+            s.interpolate_field_to_particle(); o.interpolate()
+            s.push_boris_leapfrog(w.dt); o.push(w.dt)
+            s.perform_tracking(); o.track()
+    else:
+        s.time_step(w.dt, steps)
+        o.time_step(w.dt, steps)
+    s.deposition()
+    o.deposition()
+    return s, o
+
+
+def compare(s, o, L, near_face_tol=1e-12, mesh=None):
+    ps, el, ins = s.particles()
+    ops, oel, oins = o.particles()
+    assert np.array_equal(ins, oins), "inside flags differ: %d" % (ins != oins).sum()
+    diff = el != oel
+    if diff.any():
+        # allowed only for particles within 1e-12 of a face (Cartesian boxes)
+        assert mesh is not None and mesh.shape is not None
+        h = (np.asarray(mesh.hi) - np.asarray(mesh.lo)) / np.asarray(mesh.shape)
+        r = np.mod(ops[diff, :3] - np.asarray(mesh.lo), h) / h
+        dist = np.minimum(r, 1 - r).min(1)
+        assert (dist <= near_face_tol).all(), "ownership differs away from faces"
+    keep = ~diff & ins
+    dx = np.abs(ps[keep, :3] - ops[keep, :3]).max() if keep.any() else 0.0
+    vs = max(np.abs(ops[:, 3:]).max(), 1e-300)
+    dv = np.abs(ps[keep, 3:] - ops[keep, 3:]).max() / vs if keep.any() else 0.0
+    PS, oPS = s.part_source(), o.part_source()
+    rs = max(np.abs(oPS).max(), 1e-300)
+    drho = np.abs(PS - oPS).max() / rs
+    print("near-face ownership differences %d, |dx|/L %.2e, |dv| %.2e, |drho| %.2e" % (diff.sum(), dx / L, dv, drho))
+    assert dx <= 1e-12 * L
+    assert dv <= 1e-12
+    assert drho <= 1e-12
+    return diff.sum()
+
+
+def test_cvwm_golden_reference_dg_source(oracle_mod):
+    """GPU cell_volweight_mean on the reference's HOPR mesh and particles vs its DG_Source."""
+    d = np.load(os.path.join(HERE, "golden", "cvwm_plasma_ball.npz"))
+    names = [str(x) for x in d["BCNames"]]
+    m = M.from_hopr_arrays(d["ElemInfo"], d["SideInfo"], d["NodeCoords"], d["GlobalNodeIDs"], d["BCType"],
+                           {nm: M.PART_BC_PERIODIC for nm in names}, names, d["PeriodicVectors"])
+    pd = d["PartData"]
+    print("case golden plasma ball CVWM, n=%d" % len(pd), flush=True)
+    with stage.PICStage(m, int(d["N"]), deposition=stage.DEPO_CVWM, charge=tuple(d["ChargeIC"]),
+                        mass=tuple(d["MassIC"]), mpf=tuple(d["MacroParticleFactor"]), max_particles=len(pd)) as s:
+        s.set_particles(pd[:, :6], elem=None, species=pd[:, 6].astype(np.int32))
+        _, el, ins = s.particles()
+        pi = d["PartInt"]
+        ref = np.zeros(len(pd), np.int32)
+        for e in range(pi.shape[1]):
+            ref[pi[0, e]:pi[1, e]] = e + 1
+        assert ins.all() and np.array_equal(el, ref)
+        s.deposition()
+        PS = s.part_source()
+    ref = d["DG_Source"].reshape(PS.shape)
+    assert np.abs(PS - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("depo", [stage.DEPO_SF, stage.DEPO_CVWM])
+def test_time_step_parity_small_cube(oracle_mod, depo):
+    w = W.config2(nx=6, ppc=24, deposition=depo, steps=10)
+    s, o = run_pair(oracle_mod, w, 10, sf_radius=w.sf_radius if depo == stage.DEPO_SF else 0.0)
+    compare(s, o, 1.0, mesh=w.mesh)
+    s.close()
+
+
+def test_time_step_parity_config1(oracle_mod):
+    w = W.config1(ppc=16)
+    s, o = run_pair(oracle_mod, w, 40)
+    compare(s, o, 4 * np.pi, mesh=w.mesh)
+    s.close()
+
+
+def test_staged_calls_match_oracle(oracle_mod):
+    """Deposition / InterpolateFieldToParticle / push / PerformTracking one by one."""
+    w = W.config2(nx=5, ppc=30, steps=1)
+    kw = W.stage_kwargs(w)
+    with stage.PICStage(w.mesh, w.N, max_particles=len(w.PartState), **kw) as s:
+        s.set_field(w.E)
+        s.set_particles(w.PartState, elem=w.elem, species=w.species)
+        o = oracle_mod.Oracle(w.mesh, w.N, deposition=kw["deposition"], sf_radius=kw["sf_radius"],
+                              sf_alpha=kw["sf_alpha"], external_field=kw["external_field"], charge=kw["charge"],
+                              mass=kw["mass"], mpf=kw["mpf"])
+        o.set_field(w.E)
+        o.set_particles(w.PartState, elem=w.elem, species=w.species)
+        for step in range(3):
+            s.deposition(); o.deposition()
+            PS, oPS = s.part_source(), o.part_source()
+            assert np.abs(PS - oPS).max() <= 1e-12 * np.abs(oPS).max()
+            s.interpolate_field_to_particle(); o.interpolate()
+            F, oF = s.field_at_particle(), o.field_at_particle()
+            assert np.abs(F - oF).max() <= 1e-12 * np.abs(oF).max()
+            s.push_boris_leapfrog(w.dt); o.push(w.dt)
+            assert np.array_equal(s.last_part_pos(), o.last_part_pos())
+            ps, _, _ = s.particles()
+            ops, _, _ = o.particles()
+            assert np.abs(ps[:, :3] - ops[:, :3]).max() <= 1e-12
+            assert np.abs(ps[:, 3:] - ops[:, 3:]).max() <= 1e-12 * np.abs(ops[:, 3:]).max()
+            s.perform_tracking(); o.track()
+            ps, el, ins = s.particles()
+            ops, oel, oins = o.particles()
+            assert ins.all() and oins.all()
+            assert np.array_equal(el, oel)
+            assert np.array_equal(el, w.mesh.element_of(np.mod(ps[:, :3], 1.0)))
+            assert np.abs(ps[:, :3] - ops[:, :3]).max() <= 1e-12
+
+
+def test_staged_time_step_equals_fused(oracle_mod):
+    w = W.config2(nx=6, ppc=20, steps=3)
+    s, o = run_pair(oracle_mod, w, 3, staged=True)
+    compare(s, o, 1.0, mesh=w.mesh)
+    s.close()
+
+
+def test_open_boundary_removal(oracle_mod):
+    m = M.cartesian_box((4, 4, 4), periodic=(False, False, False))
+    rng = np.random.default_rng(11)
+    n = 4000
+    x = rng.uniform(0.02, 0.98, (n, 3))
+    v = rng.normal(size=(n, 3)) * 2e7
+    w = W.Workload("open_box4", m, 2, np.hstack([x, v]), m.element_of(x), np.ones(n, np.int32),
+                   np.zeros((64, 27, 3)), (0, 0, 0, 0, 0, 0), stage.DEPO_CVWM, 0.0, 2, 1e-8, 3,
+                   (stage.ELEMENTARY_CHARGE * -1,), (stage.ELECTRON_MASS,), (1.0,))
+    s, o = run_pair(oracle_mod, w, 3)
+    ps, el, ins = s.particles()
+    assert (~ins).sum() > 100
+    compare(s, o, 1.0, mesh=m)
+    assert s.stats()["n_lost"] == 0
+    s.close()
+
+
+def test_deterministic_repeat(oracle_mod):
+    w = W.config2(nx=8, ppc=40, steps=4)
+    out = []
+    for _ in range(2):
+        with stage.PICStage(w.mesh, w.N, max_particles=len(w.PartState), **W.stage_kwargs(w)) as s:
+            s.set_field(w.E)
+            s.set_particles(w.PartState, elem=w.elem, species=w.species)
+            s.time_step(w.dt, 4)
+            s.deposition()
+            out.append((s.particles()[0], s.part_source()))
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])
+
+
+def test_config2_full_size_properties():
+    """48^3 elements, 22.1M particles: ownership equals the closed-form owner
+    after two steps, no particle lost, and the shape-function source is
+    reproducible bit for bit."""
+    w = W.config2()
+    print("case config2 full size n=%d" % len(w.PartState), flush=True)
+    with stage.PICStage(w.mesh, w.N, max_particles=len(w.PartState), **W.stage_kwargs(w)) as s:
+        s.set_field(w.E)
+        s.set_particles(w.PartState, elem=w.elem, species=w.species)
+        s.time_step(w.dt, 2)
+        ps, el, ins = s.particles()
+        assert ins.all()
+        own = w.mesh.element_of(ps[:, :3])
+        bad = own != el
+        if bad.any():
+            h = 1.0 / 48
+            r = np.mod(ps[bad, :3], h) / h
+            assert (np.minimum(r, 1 - r).min(1) <= 1e-12).all()
+        s.deposition()
+        a = s.part_source()
+        s.deposition()
+        b = s.part_source()
+        assert np.array_equal(a, b)
+        assert s.stats()["n_lost"] == 0
