@@ -52,7 +52,8 @@ def compare(s, o, L, near_face_tol=1e-12, mesh=None):
     ps, el, ins = s.particles()
     ops, oel, oins = o.particles()
     assert np.array_equal(ins, oins), "inside flags differ: %d" % (ins != oins).sum()
-    diff = el != oel
+    # removed particles have no owner (RemoveParticle only clears ParticleInside)
+    diff = (el != oel) & ins
     if diff.any():
         # allowed only for particles within 1e-12 of a face (Cartesian boxes)
         assert mesh is not None and mesh.shape is not None
