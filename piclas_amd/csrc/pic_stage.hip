@@ -390,29 +390,48 @@ struct SFArgs {
   int alpha;
 };
 
-template <int N>
+template <int N, int ALPHA>
 __global__ __launch_bounds__(kBlock) void k_sf_gather(DevMesh M, DevParts P, const int32_t *start, SFArgs A, double *PS) {
+  // Results need only agree with the reference to round-off (the sums run in
+  // another order anyway), so this kernel allows fused multiply-add.
+#pragma clang fp contract(fast)
   constexpr int n1 = N + 1, nd = n1 * n1 * n1, nch = (nd + kWave - 1) / kWave;
   __shared__ double lds[kBlock / kWave][kWave][8];
   const int wib = threadIdx.x / kWave;
   const int lane = lane_id();
-  const int nblk = gridDim.x;
-  const int lb = xcd_block(blockIdx.x, nblk);
+  const int lb = xcd_block(blockIdx.x, gridDim.x);
   const int t = lb * (kBlock / kWave) + wib;
   if (t >= M.nElems) return;
-  double xd[nch], yd[nch], zd[nch], acc[nch][4];
+  // DOFs of the target in coordinates centred on its barycentre:
+  //   r^2 = |d|^2 + |p|^2 - 2 d.p   with d, p relative to the barycentre
+  const double bx = M.bary[t * 3], by = M.bary[t * 3 + 1], bz = M.bary[t * 3 + 2];
+  double mx[nch], my[nch], mz[nch], dd[nch], acc[nch][4];
   bool has[nch];
+  double blo[3] = {1e300, 1e300, 1e300}, bhi[3] = {-1e300, -1e300, -1e300};
 #pragma unroll
   for (int c = 0; c < nch; ++c) {
     const int d = c * kWave + lane;
     has[c] = d < nd;
     const double *xg = M.xGPe + ((size_t)t * nd + (has[c] ? d : 0)) * 3;
-    xd[c] = xg[0]; yd[c] = xg[1]; zd[c] = xg[2];
+    const double dx = xg[0] - bx, dy = xg[1] - by, dz = xg[2] - bz;
+    mx[c] = -2.0 * dx; my[c] = -2.0 * dy; mz[c] = -2.0 * dz;
+    dd[c] = dx * dx + dy * dy + dz * dz;
+    blo[0] = fmin(blo[0], dx); blo[1] = fmin(blo[1], dy); blo[2] = fmin(blo[2], dz);
+    bhi[0] = fmax(bhi[0], dx); bhi[1] = fmax(bhi[1], dy); bhi[2] = fmax(bhi[2], dz);
 #pragma unroll
     for (int k = 0; k < 4; ++k) acc[c][k] = 0.0;
   }
-  const double bx = M.bary[t * 3], by = M.bary[t * 3 + 1], bz = M.bary[t * 3 + 2];
+  // bounding box of the target DOFs (wave min/max)
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1)
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      blo[a] = fmin(blo[a], __shfl_xor(blo[a], off, kWave));
+      bhi[a] = fmax(bhi[a], __shfl_xor(bhi[a], off, kWave));
+    }
   const double thr = A.r_sf + M.radius[t];
+  const double R2 = A.r2_sf, R2i = A.r2_sf_inv;
+  const double R2box = R2 * (1.0 + 1e-12);
   double (*L)[8] = lds[wib];
   for (int q = A.off[t]; q < A.off[t + 1]; ++q) {
     const int s = A.src[q];
@@ -422,48 +441,61 @@ __global__ __launch_bounds__(kBlock) void k_sf_gather(DevMesh M, DevParts P, con
     for (int base = s0; base < s1; base += kWave) {
       const int i = base + lane;
       bool pass = false;
-      double px = 0, py = 0, pz = 0, f0 = 0, f1 = 0, f2 = 0, f3 = 0;
+      double px = 0, py = 0, pz = 0, f = 0, vx = 0, vy = 0, vz = 0;
       if (i < s1) {
         const int sp = P.spec[i];
         const double q0 = M.charge[sp];
-        // GetPartPosShifted: x + c1*V1 + c2*V2 + c3*V3 (the table holds the sum)
-        px = P.x[i] + shx; py = P.y[i] + shy; pz = P.z[i] + shz;
-        const double d0 = px - bx, d1 = py - by, d2 = pz - bz;
-        pass = (fabs(q0) > 0.0) && !(sqrt(d0 * d0 + d1 * d1 + d2 * d2) > thr);
-        const double Charge = q0 * M.mpf[sp];
-        f0 = P.vx[i] * Charge * A.w_sf;
-        f1 = P.vy[i] * Charge * A.w_sf;
-        f2 = P.vz[i] * Charge * A.w_sf;
-        f3 = Charge * A.w_sf;
+        // shifted position (GetPartPosShifted) relative to the target barycentre
+        px = (P.x[i] + shx) - bx; py = (P.y[i] + shy) - by; pz = (P.z[i] + shz) - bz;
+        // the reference's element cull |x - bary| > r_sf + ElemRadius, and the
+        // exact test that some DOF of the target lies within r_sf
+        const double gx = fmax(0.0, fmax(blo[0] - px, px - bhi[0]));
+        const double gy = fmax(0.0, fmax(blo[1] - py, py - bhi[1]));
+        const double gz = fmax(0.0, fmax(blo[2] - pz, pz - bhi[2]));
+        pass = (fabs(q0) > 0.0) && !(sqrt(px * px + py * py + pz * pz) > thr) && (gx * gx + gy * gy + gz * gz <= R2box);
+        f = q0 * M.mpf[sp] * A.w_sf;
+        vx = P.vx[i]; vy = P.vy[i]; vz = P.vz[i];
       }
       const unsigned long long m = __ballot(pass);
       const int cnt = __popcll(m);
+      if (cnt == 0) continue;
       if (pass) {
         const int slot = __popcll(m & ((1ull << lane) - 1ull));
         L[slot][0] = px; L[slot][1] = py; L[slot][2] = pz;
-        L[slot][3] = f0; L[slot][4] = f1; L[slot][5] = f2; L[slot][6] = f3;
+        L[slot][3] = px * px + py * py + pz * pz;
+        L[slot][4] = vx * f; L[slot][5] = vy * f; L[slot][6] = vz * f; L[slot][7] = f;
       }
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      for (int j = 0; j < cnt; ++j) {
-        const double qx = L[j][0], qy = L[j][1], qz = L[j][2];
-        const double g0 = L[j][3], g1 = L[j][4], g2 = L[j][5], g3 = L[j][6];
+      auto deposit_one = [&](int j) {
+        const double qx = L[j][0], qy = L[j][1], qz = L[j][2], qq = L[j][3];
+        const double g0 = L[j][4], g1 = L[j][5], g2 = L[j][6], g3 = L[j][7];
 #pragma unroll
         for (int c = 0; c < nch; ++c) {
-          const double v0 = qx - xd[c], v1 = qy - yd[c], v2 = qz - zd[c];
-          const double r2 = v0 * v0 + v1 * v1 + v2 * v2;
-          if (has[c] && r2 <= A.r2_sf) {
-            const double S = 1. - A.r2_sf_inv * r2;
-            double S1 = S * S;
+          const double r2 = mx[c] * qx + (my[c] * qy + (mz[c] * qz + (dd[c] + qq)));
+          const double S = 1.0 - R2i * r2;
+          double S1 = S * S;
+          if (ALPHA == 0) {
             for (int ex = 3; ex <= A.alpha; ++ex) S1 = S * S1;
-            acc[c][0] = acc[c][0] + S1 * g0;
-            acc[c][1] = acc[c][1] + S1 * g1;
-            acc[c][2] = acc[c][2] + S1 * g2;
-            acc[c][3] = acc[c][3] + S1 * g3;
+          } else {
+#pragma unroll
+            for (int ex = 3; ex <= ALPHA; ++ex) S1 = S * S1;
           }
+          S1 = (r2 <= R2 && has[c]) ? S1 : 0.0;
+          acc[c][0] += S1 * g0;
+          acc[c][1] += S1 * g1;
+          acc[c][2] += S1 * g2;
+          acc[c][3] += S1 * g3;
         }
+      };
+      // two particles per trip so the LDS reads of the second overlap the first's arithmetic
+      int j = 0;
+      for (; j + 1 < cnt; j += 2) {
+        deposit_one(j);
+        deposit_one(j + 1);
       }
+      if (j < cnt) deposit_one(j);
       __builtin_amdgcn_wave_barrier();
     }
   }
@@ -563,6 +595,18 @@ int alloc_parts(pic_handle *h, DevParts &P, int64_t cap) {
     default: break;                                                                      \
   }
 
+#define LAUNCH_N2(KERNEL, N, A2, GRID, ...)                                                        \
+  switch (N) {                                                                                      \
+    case 1: hipLaunchKernelGGL((KERNEL<1, A2>), GRID, dim3(kBlock), 0, h->stream, __VA_ARGS__); break; \
+    case 2: hipLaunchKernelGGL((KERNEL<2, A2>), GRID, dim3(kBlock), 0, h->stream, __VA_ARGS__); break; \
+    case 3: hipLaunchKernelGGL((KERNEL<3, A2>), GRID, dim3(kBlock), 0, h->stream, __VA_ARGS__); break; \
+    case 4: hipLaunchKernelGGL((KERNEL<4, A2>), GRID, dim3(kBlock), 0, h->stream, __VA_ARGS__); break; \
+    case 5: hipLaunchKernelGGL((KERNEL<5, A2>), GRID, dim3(kBlock), 0, h->stream, __VA_ARGS__); break; \
+    case 6: hipLaunchKernelGGL((KERNEL<6, A2>), GRID, dim3(kBlock), 0, h->stream, __VA_ARGS__); break; \
+    case 7: hipLaunchKernelGGL((KERNEL<7, A2>), GRID, dim3(kBlock), 0, h->stream, __VA_ARGS__); break; \
+    default: break;                                                                                 \
+  }
+
 float elapsed(hipEvent_t a, hipEvent_t b) {
   float ms = 0.f;
   (void)hipEventElapsedTime(&ms, a, b);
@@ -590,7 +634,11 @@ int run_deposition(pic_handle *h, bool timed) {
   } else {
     const int waves = kBlock / kWave;
     const unsigned nblk = xcd_grid((ne + waves - 1) / waves);
-    LAUNCH_N(k_sf_gather, N, dim3(nblk), h->M, h->P, h->start, h->sf, h->PS);
+    if (h->sf.alpha == 2) {
+      LAUNCH_N2(k_sf_gather, N, 2, dim3(nblk), h->M, h->P, h->start, h->sf, h->PS);
+    } else {
+      LAUNCH_N2(k_sf_gather, N, 0, dim3(nblk), h->M, h->P, h->start, h->sf, h->PS);
+    }
     if (timed) HIPCHK(hipEventRecord(h->ev[1], h->stream));
   }
   HIPCHK(hipGetLastError());
