@@ -114,8 +114,11 @@ __device__ inline bool ref_elem_newton(const DevMesh &M, int e, const double xin
   int it = 0;
   while (deltaXi2 > M.refEps && it < 100) {
     ++it;
+    // keep the element tables in memory (cache) instead of pinning 96
+    // loop-invariant doubles in registers across the iteration
+    asm volatile("" ::: "memory");
     double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-#pragma unroll
+#pragma unroll 1
     for (int k = 0; k < 2; ++k)
 #pragma unroll
       for (int j = 0; j < 2; ++j) {
@@ -153,6 +156,7 @@ __device__ inline bool ref_elem_newton(const DevMesh &M, int e, const double xin
     double lambda = 1.0;
     int iArmijo = 1;
     while (NormF > NormF_old * (1. - 0.0001 * lambda) && iArmijo <= 8) {
+      asm volatile("" ::: "memory");
 #pragma unroll
       for (int d = 0; d < 3; ++d) xi[d] = xo[d] - lambda * dxi[d];
 #pragma unroll
@@ -207,8 +211,11 @@ __device__ inline void ref_newton_start(const DevMesh &M, int e, const double x[
     const double d0[3] = {x[0] - xg[0], x[1] - xg[1], x[2] - xg[2]};
     double win = sqrt(d0[0] * d0[0] + d0[1] * d0[1] + d0[2] * d0[2]);
     xi[0] = xi[1] = xi[2] = M.xGP[0];
+#pragma unroll 1
     for (int i = 0; i < n1; ++i)
+#pragma unroll 1
       for (int j = 0; j < n1; ++j)
+#pragma unroll 1
         for (int k = 0; k < n1; ++k) {
           const int r = (k * n1 + j) * n1 + i;
           const double dX = fabs(x[0] - xg[r * 3]);
@@ -234,9 +241,9 @@ __device__ inline void eval_field(const DevMesh &M, int e, const double xi[3], d
   for (int d = 0; d < 3; ++d) lagrange<n1>(xi[d], M.xGP, M.wBaryGP, L[d]);
   const double *Ue = M.E + (size_t)e * n1 * n1 * n1 * 3;
   U[0] = U[1] = U[2] = 0.0;
-#pragma unroll
+#pragma unroll 1
   for (int k = 0; k < n1; ++k)
-#pragma unroll
+#pragma unroll 1
     for (int j = 0; j < n1; ++j) {
       const double Lez = L[1][j] * L[2][k];
 #pragma unroll
@@ -359,7 +366,7 @@ __device__ inline bool inside_quad(const DevMesh &M, int e, const double x[3], d
   const double *C = M.corner + (size_t)e * 24;
   const uint8_t cc = M.concave[e];
   bool in = true;
-#pragma unroll
+#pragma unroll 1
   for (int l = 0; l < 6; ++l) {
     const uint8_t *sn = M.sideNode + ((size_t)e * 6 + l) * 4;
     double A[3][4];

@@ -151,6 +151,85 @@ __global__ __launch_bounds__(kBlock) void k_push_track(DevMesh M, DevParts P, in
   }
 }
 
+// Split form of the fused step for throughput.  Kernel A does, per particle,
+// interpolate + push and the first iteration of SingleParticleTriaTracking3D
+// (ParticleInsideQuad3D in the last element); particles still inside are
+// done.  The others (about one in a hundred) are appended to a list and
+// kernel B runs the full face walk for them from the same state, so the
+// result is the one of the single loop.  Splitting keeps kernel A's register
+// footprint low enough for several waves per SIMD.
+struct Movers {
+  int32_t *count;   // [1]
+  int32_t *idx;     // [cap]
+  double *last;     // [cap][3] LastPartPos
+};
+
+template <int N>
+__global__ __launch_bounds__(kBlock) void k_push_inside(DevMesh M, DevParts P, int64_t n, double dt, int32_t *newElem,
+                                                        Movers mv) {
+  const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  bool mover = false;
+  double last[3] = {0, 0, 0};
+  if (i < n) {
+    const int e0 = P.elem[i];
+    if (e0 < 0) {
+      newElem[i] = -1;
+    } else {
+      double x[3] = {P.x[i], P.y[i], P.z[i]};
+      double v[3] = {P.vx[i], P.vy[i], P.vz[i]};
+      last[0] = x[0]; last[1] = x[1]; last[2] = x[2];
+      const int sp = P.spec[i];
+      double F[6] = {0, 0, 0, 0, 0, 0};
+      if (fabs(M.charge[sp]) > 0.0) field_at_particle<N>(M, e0, x, F);
+      boris_push(M.charge[sp], M.mass[sp], P.isNew[i] != 0, F, dt, x, v);
+      P.x[i] = x[0]; P.y[i] = x[1]; P.z[i] = x[2];
+      P.vx[i] = v[0]; P.vy[i] = v[1]; P.vz[i] = v[2];
+      P.isNew[i] = 0;
+      double det[6][2];
+      if (inside_quad(M, e0, x, det)) newElem[i] = e0;
+      else mover = true;
+    }
+  }
+  // one atomic per wave for the mover slots
+  const unsigned long long m = __ballot(mover);
+  if (m == 0) return;
+  const int lane = lane_id();
+  const int leader = __ffsll((long long)m) - 1;
+  int base = 0;
+  if (lane == leader) base = atomicAdd(mv.count, __popcll(m));
+  base = __shfl(base, leader, kWave);
+  if (mover) {
+    const int slot = base + __popcll(m & ((1ull << lane) - 1ull));
+    mv.idx[slot] = int32_t(i);
+    mv.last[(size_t)slot * 3 + 0] = last[0];
+    mv.last[(size_t)slot * 3 + 1] = last[1];
+    mv.last[(size_t)slot * 3 + 2] = last[2];
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_track_movers(DevMesh M, DevParts P, int32_t *newElem, Movers mv,
+                                                         unsigned long long *lostCount, uint8_t *insideById,
+                                                         double *stateById) {
+  const int cnt = *mv.count;
+  for (int k = blockIdx.x * kBlock + threadIdx.x; k < cnt; k += gridDim.x * kBlock) {
+    const int i = mv.idx[k];
+    double x[3] = {P.x[i], P.y[i], P.z[i]};
+    double last[3] = {mv.last[(size_t)k * 3], mv.last[(size_t)k * 3 + 1], mv.last[(size_t)k * 3 + 2]};
+    bool lost;
+    int steps;
+    const int e = tria_track(M, P.elem[i], x, last, lost, steps);
+    P.x[i] = x[0]; P.y[i] = x[1]; P.z[i] = x[2];
+    newElem[i] = e;
+    if (e < 0) {
+      const int id = P.id[i];
+      insideById[id] = 0;
+      double *st = stateById + (size_t)id * 6;
+      st[0] = x[0]; st[1] = x[1]; st[2] = x[2]; st[3] = P.vx[i]; st[4] = P.vy[i]; st[5] = P.vz[i];
+      if (lost) atomicAdd(lostCount, 1ull);
+    }
+  }
+}
+
 // ---- sort by element --------------------------------------------------------
 // One wave per old element segment: stayers get their rank among the
 // stayers of the segment (order preserved); leavers take an arrival slot of
@@ -388,6 +467,7 @@ struct SFArgs {
   double sh[27][3];
   double r_sf, r2_sf, r2_sf_inv, w_sf;
   int alpha;
+  int linear;   // NGeo == 1: the barycentre cull is implied by the DOF-box test
 };
 
 template <int N, int ALPHA>
@@ -405,7 +485,10 @@ __global__ __launch_bounds__(kBlock) void k_sf_gather(DevMesh M, DevParts P, con
   // DOFs of the target in coordinates centred on its barycentre:
   //   r^2 = |d|^2 + |p|^2 - 2 d.p   with d, p relative to the barycentre
   const double bx = M.bary[t * 3], by = M.bary[t * 3 + 1], bz = M.bary[t * 3 + 2];
-  double mx[nch], my[nch], mz[nch], dd[nch], acc[nch][4];
+  //   S = 1 - r^2/R^2 = cd - (|p|^2/R^2) + (2/R^2) d.p   (one add, three FMA)
+  // lanes without a DOF get cd = -1e300 so that max(S,0) = 0.
+  const double R2i0 = A.r2_sf_inv;
+  double mx[nch], my[nch], mz[nch], cd[nch], acc[nch][4];
   bool has[nch];
   double blo[3] = {1e300, 1e300, 1e300}, bhi[3] = {-1e300, -1e300, -1e300};
 #pragma unroll
@@ -414,10 +497,12 @@ __global__ __launch_bounds__(kBlock) void k_sf_gather(DevMesh M, DevParts P, con
     has[c] = d < nd;
     const double *xg = M.xGPe + ((size_t)t * nd + (has[c] ? d : 0)) * 3;
     const double dx = xg[0] - bx, dy = xg[1] - by, dz = xg[2] - bz;
-    mx[c] = -2.0 * dx; my[c] = -2.0 * dy; mz[c] = -2.0 * dz;
-    dd[c] = dx * dx + dy * dy + dz * dz;
-    blo[0] = fmin(blo[0], dx); blo[1] = fmin(blo[1], dy); blo[2] = fmin(blo[2], dz);
-    bhi[0] = fmax(bhi[0], dx); bhi[1] = fmax(bhi[1], dy); bhi[2] = fmax(bhi[2], dz);
+    mx[c] = 2.0 * R2i0 * dx; my[c] = 2.0 * R2i0 * dy; mz[c] = 2.0 * R2i0 * dz;
+    cd[c] = has[c] ? 1.0 - R2i0 * (dx * dx + dy * dy + dz * dz) : -1e300;
+    if (has[c]) {
+      blo[0] = fmin(blo[0], dx); blo[1] = fmin(blo[1], dy); blo[2] = fmin(blo[2], dz);
+      bhi[0] = fmax(bhi[0], dx); bhi[1] = fmax(bhi[1], dy); bhi[2] = fmax(bhi[2], dz);
+    }
 #pragma unroll
     for (int k = 0; k < 4; ++k) acc[c][k] = 0.0;
   }
@@ -430,7 +515,7 @@ __global__ __launch_bounds__(kBlock) void k_sf_gather(DevMesh M, DevParts P, con
       bhi[a] = fmax(bhi[a], __shfl_xor(bhi[a], off, kWave));
     }
   const double thr = A.r_sf + M.radius[t];
-  const double R2 = A.r2_sf, R2i = A.r2_sf_inv;
+  const double R2 = A.r2_sf;
   const double R2box = R2 * (1.0 + 1e-12);
   double (*L)[8] = lds[wib];
   for (int q = A.off[t]; q < A.off[t + 1]; ++q) {
@@ -452,7 +537,10 @@ __global__ __launch_bounds__(kBlock) void k_sf_gather(DevMesh M, DevParts P, con
         const double gx = fmax(0.0, fmax(blo[0] - px, px - bhi[0]));
         const double gy = fmax(0.0, fmax(blo[1] - py, py - bhi[1]));
         const double gz = fmax(0.0, fmax(blo[2] - pz, pz - bhi[2]));
-        pass = (fabs(q0) > 0.0) && !(sqrt(px * px + py * py + pz * pz) > thr) && (gx * gx + gy * gy + gz * gz <= R2box);
+        // On linear elements the cull is implied by the box test (every DOF
+        // lies within ElemRadius of the barycentre), so only the box test runs.
+        pass = (fabs(q0) > 0.0) && (gx * gx + gy * gy + gz * gz <= R2box) &&
+               (A.linear || !(sqrt(px * px + py * py + pz * pz) > thr));
         f = q0 * M.mpf[sp] * A.w_sf;
         vx = P.vx[i]; vy = P.vy[i]; vz = P.vz[i];
       }
@@ -462,7 +550,7 @@ __global__ __launch_bounds__(kBlock) void k_sf_gather(DevMesh M, DevParts P, con
       if (pass) {
         const int slot = __popcll(m & ((1ull << lane) - 1ull));
         L[slot][0] = px; L[slot][1] = py; L[slot][2] = pz;
-        L[slot][3] = px * px + py * py + pz * pz;
+        L[slot][3] = R2i0 * (px * px + py * py + pz * pz);
         L[slot][4] = vx * f; L[slot][5] = vy * f; L[slot][6] = vz * f; L[slot][7] = f;
       }
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -473,8 +561,8 @@ __global__ __launch_bounds__(kBlock) void k_sf_gather(DevMesh M, DevParts P, con
         const double g0 = L[j][4], g1 = L[j][5], g2 = L[j][6], g3 = L[j][7];
 #pragma unroll
         for (int c = 0; c < nch; ++c) {
-          const double r2 = mx[c] * qx + (my[c] * qy + (mz[c] * qz + (dd[c] + qq)));
-          const double S = 1.0 - R2i * r2;
+          // radius2 <= r2_sf  <=>  S >= 0 (up to round-off, where S^alpha ~ 0)
+          const double S = fmax(mx[c] * qx + (my[c] * qy + (mz[c] * qz + (cd[c] - qq))), 0.0);
           double S1 = S * S;
           if (ALPHA == 0) {
             for (int ex = 3; ex <= A.alpha; ++ex) S1 = S * S1;
@@ -482,7 +570,6 @@ __global__ __launch_bounds__(kBlock) void k_sf_gather(DevMesh M, DevParts P, con
 #pragma unroll
             for (int ex = 3; ex <= ALPHA; ++ex) S1 = S * S1;
           }
-          S1 = (r2 <= R2 && has[c]) ? S1 : 0.0;
           acc[c][0] += S1 * g0;
           acc[c][1] += S1 * g1;
           acc[c][2] += S1 * g2;
@@ -548,6 +635,7 @@ struct pic_handle {
   int32_t *elemClass = nullptr, *classOff = nullptr, *classInc = nullptr;
   double *classVol = nullptr;
   SFArgs sf{};
+  Movers mv{};
   bool stagedValid = false;   // LastPartPos/FieldAtParticle refer to the current order
   std::vector<std::pair<int64_t, std::vector<double>>> unplaced;  // particles no element holds at set-up
   // statistics
@@ -767,6 +855,7 @@ int pic_create(const pic_mesh_desc *mesh, const pic_params *par, pic_handle **ou
     const double BetaFac = std::tgamma(1.5) * std::tgamma(a + 1.) / std::tgamma(1.5 + (a + 1.));
     h->sf.w_sf = 1. / (2. * BetaFac * a + 2 * BetaFac) * (a + 1.) / (M_PI * (r * r * r));
     h->sf.alpha = par->sf_alpha;
+    h->sf.linear = mesh->NGeo == 1;
   }
   // particles
   const int64_t cap = par->max_particles;
@@ -787,6 +876,9 @@ int pic_create(const pic_mesh_desc *mesh, const pic_params *par, pic_handle **ou
   rc |= dalloc(h, &h->arrCnt, hm.nElems);
   rc |= dalloc(h, &h->insideById, cap);
   rc |= dalloc(h, &h->stateById, cap * 6);
+  rc |= dalloc(h, &h->mv.count, 1);
+  rc |= dalloc(h, &h->mv.idx, cap);
+  rc |= dalloc(h, &h->mv.last, cap * 3);
   rc |= dalloc(h, &h->lostCount, 1);
   if (rc) { pic_destroy(h); return PIC_EHIP; }
   HIPCHK(hipMemset(h->lostCount, 0, sizeof(unsigned long long)));
@@ -980,8 +1072,10 @@ int pic_time_step(pic_handle *h, double dt, int32_t nsteps) {
     // LastPartPos <- PartState, interpolate, push, track (one kernel)
     HIPCHK(hipEventRecord(h->ev[3], h->stream));
     if (h->n > 0) {
-      LAUNCH_N(k_push_track, h->hm.N, dim3(grid_for(h->n)), h->M, h->P, h->n, dt, h->newElem, h->lostCount,
-               h->insideById, h->stateById);
+      HIPCHK(hipMemsetAsync(h->mv.count, 0, sizeof(int32_t), h->stream));
+      LAUNCH_N(k_push_inside, h->hm.N, dim3(grid_for(h->n)), h->M, h->P, h->n, dt, h->newElem, h->mv);
+      hipLaunchKernelGGL(k_track_movers, dim3(1024), dim3(kBlock), 0, h->stream, h->M, h->P, h->newElem, h->mv,
+                         h->lostCount, h->insideById, h->stateById);
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(h->ev[4], h->stream));
