@@ -471,7 +471,7 @@ struct SFArgs {
 };
 
 template <int N, int ALPHA>
-__global__ __launch_bounds__(kBlock) void k_sf_gather(DevMesh M, DevParts P, const int32_t *start, SFArgs A, double *PS) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) void k_sf_gather(DevMesh M, DevParts P, const int32_t *start, SFArgs A, double *PS) {
   // Results need only agree with the reference to round-off (the sums run in
   // another order anyway), so this kernel allows fused multiply-add.
 #pragma clang fp contract(fast)

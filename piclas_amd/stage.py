@@ -18,7 +18,7 @@ import numpy as np
 from .mesh import HoprMesh
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libpicstage.so")
+LIB_PATH = os.environ.get("PICLAS_AMD_LIB", os.path.join(HERE, "libpicstage.so"))
 
 TRACK_REFMAPPING, TRACK_TRIA = 1, 3
 DEPO_NONE, DEPO_CVWM, DEPO_SF = 0, 1, 2
