@@ -22,6 +22,10 @@ sys.path.insert(0, ROOT)
 METRIC = "particle-steps/s (interp+push+track+deposit) FP64 vs mixed; % HBM peak"
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
+# compulsory HBM bytes of one particle-step at 200 particles per element:
+# x,v read + written and the element id (104 B) plus the per-element tables
+# (E 1,536 B, Elem_xGP 1,536 B, source 2,048 B, tracking geometry 216 B) / 200
+STEP_BYTES_PER_PARTICLE = 104 + (1536 + 1536 + 2048 + 216) / 200.0
 
 
 def sf_flops_per_particle(R_over_h, rad_over_h, dofs_per_elem):
@@ -32,6 +36,19 @@ def sf_flops_per_particle(R_over_h, rad_over_h, dofs_per_elem):
     cand = dofs_per_elem * 4.0 / 3.0 * math.pi * (R_over_h + rad_over_h) ** 3
     inrad = dofs_per_elem * 4.0 / 3.0 * math.pi * R_over_h ** 3
     return 8.0 * cand + 4.0 * inrad, cand, inrad
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` for the 48^3 workload, as recorded by
+    the FETCH_SIZE / WRITE_SIZE counter passes in profiles/ (None if absent)."""
+    path = os.path.join(ROOT, "profiles", "k_sf_gather_cube48_N3_sf_hbm_traffic.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch %s:" % kernel):
+                return int(float(line.split(":")[1]))
+    except OSError:
+        pass
+    return None
 
 
 def cpu_baseline(w, fraction, steps, threads):
@@ -63,8 +80,8 @@ def main():
     ap.add_argument("--nx", type=int, default=48)
     ap.add_argument("--ppc", type=int, default=200)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-fraction", type=float, default=1.0 / 32)
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-fraction", type=float, default=1.0 / 16)
+    ap.add_argument("--cpu-steps", type=int, default=3)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; multi-GPU runs are not part of it (got --gpus %d)" % args.gpus,
@@ -72,7 +89,7 @@ def main():
         sys.exit(2)
 
     from piclas_amd import stage, workloads as W
-    w = W.config2(nx=args.nx, ppc=args.ppc)
+    w = W.thermal_cube(nx=args.nx, ppc=args.ppc)
     n = len(w.PartState)
     print("bench: %s, %d particles, %d warmup + %d timed steps" % (w.name, n, args.warmup, args.steps),
           file=sys.stderr, flush=True)
@@ -98,9 +115,11 @@ def main():
     fpp, cand, inrad = sf_flops_per_particle(w.sf_radius / h, math.sqrt(3) / 2, (w.N + 1) ** 3)
     avg_ms = st["ms_deposit_main"] / max(st["n_deposit_launches"], 1)
     achieved = fpp * n_now / (avg_ms * 1e-3) / 1e12
+    kname = "k_sf_gather<%d, %d>" % (w.N, w.sf_alpha)
+    traffic = measured_traffic(kname) if (args.nx, args.ppc) == (48, 200) else None
     roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-            "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": None,
-            "kernel": "k_sf_gather<3>", "avg_launch_ms": round(avg_ms, 4),
+            "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": traffic,
+            "kernel": kname, "avg_launch_ms": round(avg_ms, 4),
             "algorithmic_flops_per_particle": round(fpp, 1),
             "note": "FP64 vector ALU bound (no MFMA); peak = MI355X FP64 vector peak"}
     ms_step = t / args.steps * 1e3
@@ -110,6 +129,7 @@ def main():
            "config": {"workload": w.name, "elements": args.nx ** 3, "N": w.N, "particles": n,
                       "deposition": "shape_function R=1.5h alpha=2", "tracking": "triatracking",
                       "parallelism": "1 GPU"},
+           "step_hbm_frac": round(STEP_BYTES_PER_PARTICLE * value / (PEAK_HBM_GBS * 1e9), 5),
            "stage_ms_per_step": {"deposit": st["ms_deposit"] / args.steps,
                                  "interp_push_track": st["ms_push_track"] / args.steps,
                                  "sort": st["ms_sort"] / args.steps},

@@ -138,6 +138,14 @@ int  pic_get_field_at_particle(pic_handle *h, double *FieldAtParticle);
 int  pic_get_part_source(pic_handle *h, double *PS);
 /* Device pointer to PS[nElems][(N+1)^3][4], valid until the next deposition. */
 int  pic_part_source_device(pic_handle *h, const double **PS_dev);
+/* CalcDepositedCharge (src/particles/pic/analyze/pic_analyze.f90:136-210) on the
+ * current source: PartCharge[0] = sum_e sum_ijk w_i w_j w_k rho / sJ,
+ * PartCharge[1] = |Q_particles - PartCharge[0]|, PartCharge[2] = PartCharge[1] / Q_particles
+ * (0 when Q_particles is ALMOSTZERO), Q_particles = sum over particles inside of ChargeIC * MPF. */
+int  pic_deposited_charge(pic_handle *h, double *PartCharge);
+/* CalcPotentialEnergy (src/analyze/analyzefield_tools.f90:99-228, HDG with PP_nVar=1):
+ * WEl = eps0/2 sum_e sum_ijk w_i w_j w_k / sJ |E|^2 of the field last set. */
+int  pic_field_energy(pic_handle *h, double *WEl);
 int  pic_get_stats(pic_handle *h, pic_stats *st);
 int  pic_reset_stats(pic_handle *h);
 int  pic_synchronize(pic_handle *h);

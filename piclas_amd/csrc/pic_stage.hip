@@ -594,6 +594,66 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
     }
 }
 
+// ---- analysis integrals ------------------------------------------------------
+// Per element, in the reference's i-fastest DOF order:
+//   mode 0  sum_ijk (w_i w_j w_k * rho) * (1/sJ)       CalcDepositedCharge, pic_analyze.f90:172-179
+//   mode 1  sum_ijk  w_i w_j w_k / sJ * |E|^2           CalcPotentialEnergy, analyzefield_tools.f90:191,209
+struct QuadArgs {
+  double wGP[kMaxN + 1];
+};
+template <int N>
+__global__ __launch_bounds__(kBlock) void k_elem_integral(int nElems, int mode, const double *sJ, const double *src,
+                                                          QuadArgs Q, double *elemPart) {
+  constexpr int n1 = N + 1, nd = n1 * n1 * n1;
+  const int e = blockIdx.x * kBlock + threadIdx.x;
+  if (e >= nElems) return;
+  double acc = 0.0;
+  for (int k = 0; k < n1; ++k)
+    for (int j = 0; j < n1; ++j)
+#pragma unroll
+      for (int i = 0; i < n1; ++i) {
+        const int r = (k * n1 + j) * n1 + i;
+        const size_t g = size_t(e) * nd + r;
+        const double wijk = Q.wGP[i] * Q.wGP[j] * Q.wGP[k];
+        if (mode == 0) {
+          acc = acc + wijk * src[g * 4 + 3] * (1. / sJ[g]);
+        } else {
+          const double *E = src + g * 3;
+          const double Eabs = E[0] * E[0] + E[1] * E[1] + E[2] * E[2];
+          acc = acc + wijk / sJ[g] * Eabs;
+        }
+      }
+  elemPart[e] = acc;
+}
+
+// Sum of n values in a fixed order (strided per-thread sums, then a tree in
+// LDS): one block of 1024 threads, so the result does not depend on timing.
+__global__ __launch_bounds__(1024) void k_sum_fixed(int64_t n, const double *v, double *out) {
+  __shared__ double red[1024];
+  const int t = threadIdx.x;
+  double s = 0.0;
+  for (int64_t i = t; i < n; i += 1024) s = s + v[i];
+  red[t] = s;
+  __syncthreads();
+  for (int w = 512; w >= 1; w >>= 1) {
+    if (t < w) red[t] = red[t] + red[t + w];
+    __syncthreads();
+  }
+  if (t == 0) *out = red[0];
+}
+
+// Particles per species among the n element-sorted particles (all inside).
+__global__ __launch_bounds__(kBlock) void k_species_count(int64_t n, const uint8_t *spec,
+                                                          unsigned long long *cnt) {
+  __shared__ unsigned int c[kMaxSpecies];
+  if (threadIdx.x < kMaxSpecies) c[threadIdx.x] = 0;
+  __syncthreads();
+  for (int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x; i < n; i += int64_t(gridDim.x) * kBlock)
+    atomicAdd(&c[spec[i]], 1u);
+  __syncthreads();
+  if (threadIdx.x < kMaxSpecies && c[threadIdx.x]) atomicAdd(&cnt[threadIdx.x], (unsigned long long)c[threadIdx.x]);
+}
+
 __global__ void k_fill_u8(uint8_t *p, int64_t n, uint8_t v) {
   const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
   if (i < n) p[i] = v;
@@ -634,6 +694,8 @@ struct pic_handle {
   double *E = nullptr, *PS = nullptr, *elemCorner = nullptr, *classSrc = nullptr;
   int32_t *elemClass = nullptr, *classOff = nullptr, *classInc = nullptr;
   double *classVol = nullptr;
+  double *sJ = nullptr, *elemPart = nullptr, *sumOut = nullptr;
+  unsigned long long *specCnt = nullptr;
   SFArgs sf{};
   Movers mv{};
   bool stagedValid = false;   // LastPartPos/FieldAtParticle refer to the current order
@@ -833,6 +895,10 @@ int pic_create(const pic_mesh_desc *mesh, const pic_params *par, pic_handle **ou
   for (int c = 0; c < 6; ++c) M.ext[c] = par->external_field[c];
   for (int s = 0; s < kMaxSpecies; ++s) { M.charge[s] = par->charge[s]; M.mass[s] = par->mass[s]; M.mpf[s] = par->mpf[s]; }
   rc |= dalloc(h, &h->PS, size_t(hm.nElems) * hm.nDof * 4);
+  rc |= upload(h, &h->sJ, hm.sJ);
+  rc |= dalloc(h, &h->elemPart, hm.nElems);
+  rc |= dalloc(h, &h->sumOut, 1);
+  rc |= dalloc(h, &h->specCnt, kMaxSpecies);
   if (par->deposition == PIC_DEPO_CVWM) {
     rc |= dalloc(h, &h->elemCorner, size_t(hm.nElems) * 32);
     rc |= dalloc(h, &h->classSrc, size_t(hm.nClasses) * 4);
@@ -1172,6 +1238,55 @@ int pic_get_part_source(pic_handle *h, double *PS) {
 int pic_part_source_device(pic_handle *h, const double **PS_dev) {
   if (!h || !PS_dev) return fail(PIC_EINVAL, "pic_part_source_device: null argument");
   *PS_dev = h->PS;
+  return PIC_OK;
+}
+
+namespace {
+// element integral of the current PS (mode 0) or E (mode 1), summed on the device
+int elem_integral(pic_handle *h, int mode, double *out) {
+  const int ne = h->hm.nElems;
+  QuadArgs Q{};
+  for (int i = 0; i <= h->hm.N; ++i) Q.wGP[i] = h->hm.wGP[i];
+  const double *src = mode == 0 ? h->PS : h->E;
+  LAUNCH_N(k_elem_integral, h->hm.N, dim3(grid_for(ne)), ne, mode, h->sJ, src, Q, h->elemPart);
+  hipLaunchKernelGGL(k_sum_fixed, dim3(1), dim3(1024), 0, h->stream, int64_t(ne), h->elemPart, h->sumOut);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(out, h->sumOut, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return 0;
+}
+}  // namespace
+
+int pic_deposited_charge(pic_handle *h, double *part_charge) {
+  if (!h || !part_charge) return fail(PIC_EINVAL, "pic_deposited_charge: null argument");
+  if (set_device(h)) return PIC_EHIP;
+  double dep = 0.0;
+  if (int rc = elem_integral(h, 0, &dep)) return rc;
+  // charge of the particles inside: per-species counts (exact) times ChargeIC * MacroParticleFactor
+  unsigned long long cnt[kMaxSpecies] = {0};
+  HIPCHK(hipMemsetAsync(h->specCnt, 0, sizeof(cnt), h->stream));
+  if (h->n > 0)
+    hipLaunchKernelGGL(k_species_count, dim3(std::min<int64_t>(grid_for(h->n), 2048)), dim3(kBlock), 0, h->stream,
+                       h->n, h->P.spec, h->specCnt);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(cnt, h->specCnt, sizeof(cnt), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  double part = 0.0;
+  for (int s = 0; s < h->par.nSpecies; ++s) part = part + double(cnt[s]) * (h->par.charge[s] * h->par.mpf[s]);
+  part_charge[0] = dep;
+  part_charge[1] = std::fabs(part - dep);
+  // ALMOSTZERO (piclas.h:83): no relative error for a zero total
+  part_charge[2] = std::fabs(part) <= 2.22e-16 ? 0.0 : std::fabs(part - dep) / part;
+  return PIC_OK;
+}
+
+int pic_field_energy(pic_handle *h, double *w_el) {
+  if (!h || !w_el) return fail(PIC_EINVAL, "pic_field_energy: null argument");
+  if (set_device(h)) return PIC_EHIP;
+  double w = 0.0;
+  if (int rc = elem_integral(h, 1, &w)) return rc;
+  const double eps0 = 8.8541878176e-12;   // globals_vars.f90:84
+  *w_el = w * eps0 * 0.5;
   return PIC_OK;
 }
 

@@ -108,6 +108,16 @@ INTERFACE
     TYPE(C_PTR),VALUE          :: h
     REAL(C_DOUBLE),INTENT(OUT) :: F(*)
   END FUNCTION pic_get_field_at_particle
+  INTEGER(C_INT) FUNCTION pic_deposited_charge(h, PartCharge) BIND(C, NAME='pic_deposited_charge')
+    IMPORT :: C_INT, C_PTR, C_DOUBLE
+    TYPE(C_PTR),VALUE          :: h
+    REAL(C_DOUBLE),INTENT(OUT) :: PartCharge(3)
+  END FUNCTION pic_deposited_charge
+  INTEGER(C_INT) FUNCTION pic_field_energy(h, WEl) BIND(C, NAME='pic_field_energy')
+    IMPORT :: C_INT, C_PTR, C_DOUBLE
+    TYPE(C_PTR),VALUE          :: h
+    REAL(C_DOUBLE),INTENT(OUT) :: WEl
+  END FUNCTION pic_field_energy
   INTEGER(C_SIZE_T) FUNCTION c_strlen(s) BIND(C, NAME='strlen')
     IMPORT :: C_SIZE_T, C_PTR
     TYPE(C_PTR),VALUE :: s
@@ -118,7 +128,7 @@ TYPE(C_PTR),SAVE :: stage = C_NULL_PTR
 
 PUBLIC :: InitPICStage, FinalizePICStage, PICStage_SetField, PICStage_SetParticles, PICStage_GetParticles
 PUBLIC :: Deposition, InterpolateFieldToParticle, PushBorisLeapfrog, PerformTracking, TimeStepPoissonByBorisLeapfrog
-PUBLIC :: PICStage_GetPartSource, PICStage_GetFieldAtParticle
+PUBLIC :: PICStage_GetPartSource, PICStage_GetFieldAtParticle, CalcDepositedCharge, CalcPotentialEnergy
 
 CONTAINS
 
@@ -198,6 +208,18 @@ SUBROUTINE PICStage_GetFieldAtParticle(FieldAtParticle)
 REAL(C_DOUBLE),INTENT(OUT) :: FieldAtParticle(:,:)
 CALL Check(pic_get_field_at_particle(stage, FieldAtParticle), 'pic_get_field_at_particle')
 END SUBROUTINE PICStage_GetFieldAtParticle
+
+!> PartCharge(1:3) of src/particles/pic/analyze/pic_analyze.f90:136-210: deposited charge, absolute and relative error
+SUBROUTINE CalcDepositedCharge(PartCharge)
+REAL(C_DOUBLE),INTENT(OUT) :: PartCharge(3)
+CALL Check(pic_deposited_charge(stage, PartCharge), 'CalcDepositedCharge')
+END SUBROUTINE CalcDepositedCharge
+
+!> WEl of src/analyze/analyzefield_tools.f90:99-228 for the field last set
+SUBROUTINE CalcPotentialEnergy(WEl)
+REAL(C_DOUBLE),INTENT(OUT) :: WEl
+CALL Check(pic_field_energy(stage, WEl), 'CalcPotentialEnergy')
+END SUBROUTINE CalcPotentialEnergy
 
 SUBROUTINE Deposition()
 CALL Check(pic_h(stage), 'Deposition')

@@ -4,7 +4,7 @@
 ! Reads a case file (stream, little endian) holding the HOPR mesh arrays, the parameters, U_N%E and the particles, runs
 ! nSteps of Deposition / InterpolateFieldToParticle / push / PerformTracking (mode 0: stage by stage as the reference's
 ! TimeStepPoissonByBorisLeapfrog calls them; mode 1: the fused call), deposits once more and writes PartState,
-! PEM%GlobalElemID, PDM%ParticleInside and PS_N%PartSource to the result file.
+! PEM%GlobalElemID, PDM%ParticleInside, PS_N%PartSource, PartCharge(1:3) and WEl to the result file.
 !
 ! usage: pic_stage_driver <case.bin> <result.bin>
 !==================================================================================================================================
@@ -17,6 +17,7 @@ INTEGER(C_INT32_T)             :: hdr(16)
 INTEGER                        :: nElems, nSides, nNodes, NGeo, N, nBCs, nPV, nPart, nSteps, mode, nSpecies, iStep, nDof
 INTEGER(C_INT32_T),ALLOCATABLE :: ElemInfo(:,:), SideInfo(:,:), GlobalNodeIDs(:), BCType(:,:), PartBoundCond(:)
 INTEGER(C_INT32_T),ALLOCATABLE :: GlobalElemID(:), PartSpecies(:)
+REAL(C_DOUBLE)                 :: PartCharge(3), WEl
 REAL(C_DOUBLE),ALLOCATABLE     :: NodeCoords(:,:), PeriodicVectors(:,:), E(:,:,:,:,:), PartState(:,:), PS(:,:,:,:,:)
 LOGICAL,ALLOCATABLE            :: IsNewPart(:), ParticleInside(:)
 INTEGER(C_INT8_T),ALLOCATABLE  :: insideOut(:)
@@ -81,10 +82,12 @@ END DO
 CALL Deposition()
 CALL PICStage_GetParticles(PartState, GlobalElemID, ParticleInside)
 CALL PICStage_GetPartSource(PS)
+CALL CalcDepositedCharge(PartCharge)
+CALL CalcPotentialEnergy(WEl)
 CALL FinalizePICStage()
 insideOut = MERGE(1_C_INT8_T, 0_C_INT8_T, ParticleInside)
 OPEN(11, FILE=TRIM(outpath), ACCESS='STREAM', FORM='UNFORMATTED', STATUS='REPLACE')
-WRITE(11) PartState, GlobalElemID, insideOut, PS
+WRITE(11) PartState, GlobalElemID, insideOut, PS, PartCharge, WEl
 CLOSE(11)
 WRITE(*,'(A,I0,A,I0,A)') 'pic_stage_driver: ', nSteps, ' steps of ', nPart, ' particles done'
 END PROGRAM pic_stage_driver

@@ -1,0 +1,86 @@
+"""Summaries of rocprofv3 CSV output for the particle stage.
+
+  python -m piclas_amd.profiling stats  <kernel_stats.csv>
+      per-kernel calls / average / share table (from --kernel-trace --stats)
+  python -m piclas_amd.profiling traffic <fetch.csv> <write.csv> <copy_fetch.csv> <copy_write.csv> [copy_bytes]
+      HBM bytes per dispatch of every kernel from FETCH_SIZE / WRITE_SIZE
+      (KiB per dispatch), each scaled by the factor that makes the same
+      counter read the known size of a plain device copy (hbm_copy_probe).
+
+Both print Markdown tables.
+"""
+import csv
+import sys
+from collections import OrderedDict
+
+
+def short(name):
+    """Kernel name without namespace and argument list."""
+    n = name.strip('"')
+    n = n.replace("void ", "").replace("(anonymous namespace)::", "")
+    depth, out = 0, []
+    for ch in n:
+        if ch == "(" and depth == 0:
+            break
+        depth += ch == "<"
+        depth -= ch == ">"
+        out.append(ch)
+    return "".join(out)
+
+
+def kernel_stats(path, top=12):
+    rows = list(csv.DictReader(open(path)))
+    lines = ["| kernel | calls | avg (us) | min (us) | max (us) | share |", "|---|---|---|---|---|---|"]
+    for r in rows[:top]:
+        lines.append("| `%s` | %s | %.1f | %.1f | %.1f | %.2f %% |" % (
+            short(r["Name"]), r["Calls"], float(r["AverageNs"]) / 1e3, float(r["MinNs"]) / 1e3,
+            float(r["MaxNs"]) / 1e3, float(r["Percentage"])))
+    return "\n".join(lines)
+
+
+def counter_per_kernel(path):
+    out = OrderedDict()
+    for r in csv.DictReader(open(path)):
+        out.setdefault(short(r["Kernel_Name"]), []).append(float(r["Counter_Value"]))
+    return out
+
+
+def calibration(copy_csv, copy_bytes):
+    """bytes moved per counter unit for the plain copy (largest dispatches)."""
+    vals = [v for k, vs in counter_per_kernel(copy_csv).items() if "copyBuffer" in k for v in vs]
+    kib = max(vals)
+    return copy_bytes / (kib * 1024.0), kib
+
+
+def traffic(fetch_csv, write_csv, copy_fetch_csv, copy_write_csv, copy_bytes=1 << 30):
+    ff, fk = calibration(copy_fetch_csv, copy_bytes)
+    fw, wk = calibration(copy_write_csv, copy_bytes)
+    F = counter_per_kernel(fetch_csv)
+    W = counter_per_kernel(write_csv)
+    lines = ["Calibration: a %d-byte device copy reads FETCH_SIZE = %.0f KiB (factor %.3f) and WRITE_SIZE = %.0f KiB "
+             "(factor %.3f).\n" % (copy_bytes, fk, ff, wk, fw),
+             "| kernel | dispatches | FETCH_SIZE (KiB) | WRITE_SIZE (KiB) | HBM read (MB) | HBM write (MB) | traffic (MB) |",
+             "|---|---|---|---|---|---|---|"]
+    for k in F:
+        f = sum(F[k]) / len(F[k])
+        w = sum(W.get(k, [0.0])) / max(len(W.get(k, [0.0])), 1)
+        rb, wb = f * 1024 * ff, w * 1024 * fw
+        lines.append("| `%s` | %d | %.0f | %.0f | %.1f | %.1f | %.1f |" % (k, len(F[k]), f, w, rb / 1e6, wb / 1e6,
+                                                                        (rb + wb) / 1e6))
+    return "\n".join(lines)
+
+
+def main(argv):
+    if len(argv) >= 2 and argv[0] == "stats":
+        print(kernel_stats(argv[1]))
+    elif len(argv) >= 5 and argv[0] == "traffic":
+        nb = int(argv[5]) if len(argv) > 5 else (1 << 30)
+        print(traffic(argv[1], argv[2], argv[3], argv[4], nb))
+    else:
+        print(__doc__)
+        return 2
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main(sys.argv[1:]))

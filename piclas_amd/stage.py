@@ -75,6 +75,8 @@ EXPORTS = {
     "pic_get_field_at_particle": (ctypes.c_int, [_H, _f64p]),
     "pic_get_part_source": (ctypes.c_int, [_H, _f64p]),
     "pic_part_source_device": (ctypes.c_int, [_H, ctypes.POINTER(ctypes.c_void_p)]),
+    "pic_deposited_charge": (ctypes.c_int, [_H, _f64p]),
+    "pic_field_energy": (ctypes.c_int, [_H, _f64p]),
     "pic_get_stats": (ctypes.c_int, [_H, ctypes.POINTER(PicStats)]),
     "pic_reset_stats": (ctypes.c_int, [_H]),
     "pic_synchronize": (ctypes.c_int, [_H]),
@@ -219,6 +221,21 @@ class PICStage:
         a = np.zeros((self.nElems, self.nDof, 4))
         _chk(load_library().pic_get_part_source(self.h, _p(a, _f64p)), "pic_get_part_source")
         return a
+
+    def deposited_charge(self):
+        """CalcDepositedCharge: (deposited charge, absolute error, relative error)."""
+        a = np.zeros(3)
+        _chk(load_library().pic_deposited_charge(self.h, _p(a, _f64p)), "pic_deposited_charge")
+        return tuple(a)
+
+    def field_energy(self):
+        """CalcPotentialEnergy: eps0/2 * integral |E|^2 of the field last set."""
+        a = np.zeros(1)
+        _chk(load_library().pic_field_energy(self.h, _p(a, _f64p)), "pic_field_energy")
+        return float(a[0])
+
+    CalcDepositedCharge = deposited_charge
+    CalcPotentialEnergy = field_energy
 
     def stats(self):
         st = PicStats()

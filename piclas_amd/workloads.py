@@ -78,7 +78,7 @@ def maxwellian(rng, n, T_eV, m):
     return rng.normal(scale=vth, size=(n, 3))
 
 
-def config1(seed=20261203, ppc=64):
+def plasma_wave_box(seed=20261203, ppc=64):
     mesh = M.cartesian_box((16, 4, 4), lo=(0.0, 0.0, 0.0), hi=(4 * np.pi, 1.0, 1.0))
     N = 3
     rng = np.random.Generator(np.random.PCG64(seed))
@@ -99,12 +99,12 @@ def config1(seed=20261203, ppc=64):
     xg = dof_positions(mesh, N)
     E = np.zeros_like(xg)
     E[..., 0] = -ELEMENTARY_CHARGE * n0 * 0.01 / (EPS0 * 0.5) * np.sin(0.5 * xg[..., 0])
-    return Workload("config1_box16x4x4_N3_cvwm", mesh, N, np.hstack([pos, vel]), mesh.element_of(pos),
+    return Workload("plasma_wave_box16x4x4_N3_cvwm", mesh, N, np.hstack([pos, vel]), mesh.element_of(pos),
                     np.ones(n, np.int32), E, (0, 0, 0, 0, 0, 0), DEPO_CVWM, 0.0, 2, dt, 200,
                     (-ELEMENTARY_CHARGE,), (ELECTRON_MASS,), (mpf,))
 
 
-def config2(nx=48, ppc=200, seed=20261204, deposition=DEPO_SF, steps=100):
+def thermal_cube(nx=48, ppc=200, seed=20261204, deposition=DEPO_SF, steps=100):
     mesh = M.cartesian_box((nx, nx, nx), lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0))
     N = 3
     rng = np.random.Generator(np.random.PCG64(seed))
@@ -120,7 +120,7 @@ def config2(nx=48, ppc=200, seed=20261204, deposition=DEPO_SF, steps=100):
     xg = dof_positions(mesh, N)
     E = fourier_field(xg.reshape(-1, 3), (1.0, 1.0, 1.0), 8, 1e3, rng).reshape(xg.shape)
     h = 1.0 / nx
-    return Workload("config2_cube%d_N3_sf" % nx, mesh, N, np.hstack([pos, vel]), mesh.element_of(pos),
+    return Workload("thermal_cube%d_N3_sf" % nx, mesh, N, np.hstack([pos, vel]), mesh.element_of(pos),
                     np.ones(n, np.int32), E, (0, 0, 0, 0, 0, B), deposition, 1.5 * h, 2, dt, steps,
                     (-ELEMENTARY_CHARGE,), (ELECTRON_MASS,), (mpf,))
 

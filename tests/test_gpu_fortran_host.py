@@ -41,15 +41,16 @@ def read_result(path, n, nElems, nDof):
     ps = np.frombuffer(raw, np.float64, n * 6, o).reshape(n, 6); o += n * 48
     el = np.frombuffer(raw, np.int32, n, o); o += n * 4
     ins = np.frombuffer(raw, np.int8, n, o).astype(bool); o += n
-    PS = np.frombuffer(raw, np.float64, nElems * nDof * 4, o).reshape(nElems, nDof, 4)
-    return ps, el, ins, PS
+    PS = np.frombuffer(raw, np.float64, nElems * nDof * 4, o).reshape(nElems, nDof, 4); o += nElems * nDof * 32
+    tail = np.frombuffer(raw, np.float64, 4, o)
+    return ps, el, ins, PS, tail[:3], tail[3]
 
 
 @pytest.mark.parametrize("mode", [0, 1])
 @pytest.mark.parametrize("depo", [stage.DEPO_SF, stage.DEPO_CVWM])
 def test_fortran_host_matches_oracle(oracle_mod, tmp_path, mode, depo):
     assert os.path.exists(DRIVER), "build the Fortran driver with __graft_entry__.build()"
-    w = W.config2(nx=5, ppc=20, deposition=depo, steps=4)
+    w = W.thermal_cube(nx=5, ppc=20, deposition=depo, steps=4)
     if depo == stage.DEPO_CVWM:
         w.sf_radius = 0.0
     case, res = str(tmp_path / "case.bin"), str(tmp_path / "res.bin")
@@ -57,7 +58,7 @@ def test_fortran_host_matches_oracle(oracle_mod, tmp_path, mode, depo):
     print("case fortran host mode=%d depo=%d" % (mode, depo), flush=True)
     out = subprocess.run([DRIVER, case, res], capture_output=True, text=True, timeout=120)
     assert out.returncode == 0, out.stdout + out.stderr
-    ps, el, ins, PS = read_result(res, len(w.PartState), w.mesh.nElems, (w.N + 1) ** 3)
+    ps, el, ins, PS, part_charge, wel = read_result(res, len(w.PartState), w.mesh.nElems, (w.N + 1) ** 3)
     o = oracle_mod.Oracle(w.mesh, w.N, deposition=depo, sf_radius=w.sf_radius, sf_alpha=w.sf_alpha,
                           external_field=w.external_field, charge=w.charge, mass=w.mass, mpf=w.mpf)
     o.set_field(w.E)
@@ -70,3 +71,10 @@ def test_fortran_host_matches_oracle(oracle_mod, tmp_path, mode, depo):
     assert np.abs(ps[:, 3:] - ops[:, 3:]).max() <= 1e-12 * np.abs(ops[:, 3:]).max()
     oPS = o.part_source()
     assert np.abs(PS - oPS).max() <= 1e-12 * np.abs(oPS).max()
+    # CalcDepositedCharge / CalcPotentialEnergy through the Fortran wrappers
+    assert abs(part_charge[0] - o.deposited_charge()) <= 1e-12 * abs(o.deposited_charge())
+    qp = o.particle_charge()
+    assert abs(part_charge[1] - abs(qp - part_charge[0])) <= 1e-12 * abs(qp)
+    if depo == stage.DEPO_CVWM:   # node-volume weighting conserves charge to round-off
+        assert part_charge[2] <= 1e-12
+    assert abs(wel - o.field_energy()) <= 1e-12 * o.field_energy()

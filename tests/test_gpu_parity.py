@@ -100,22 +100,29 @@ def test_cvwm_golden_reference_dg_source(oracle_mod):
 
 @pytest.mark.parametrize("depo", [stage.DEPO_SF, stage.DEPO_CVWM])
 def test_time_step_parity_small_cube(oracle_mod, depo):
-    w = W.config2(nx=6, ppc=24, deposition=depo, steps=10)
+    w = W.thermal_cube(nx=6, ppc=24, deposition=depo, steps=10)
     s, o = run_pair(oracle_mod, w, 10, sf_radius=w.sf_radius if depo == stage.DEPO_SF else 0.0)
     compare(s, o, 1.0, mesh=w.mesh)
     s.close()
 
 
-def test_time_step_parity_config1(oracle_mod):
-    w = W.config1(ppc=16)
+def test_time_step_parity_plasma_wave_box(oracle_mod):
+    w = W.plasma_wave_box(ppc=16)
     s, o = run_pair(oracle_mod, w, 40)
     compare(s, o, 4 * np.pi, mesh=w.mesh)
+    # CalcDepositedCharge: node-volume weighting conserves the charge to round-off
+    dep, abs_err, rel_err = s.deposited_charge()
+    assert abs(dep - o.deposited_charge()) <= 1e-12 * abs(dep)
+    assert abs(abs_err - abs(o.particle_charge() - dep)) <= 1e-12 * abs(dep)
+    assert rel_err <= 1e-12
+    # CalcPotentialEnergy of the frozen field
+    assert abs(s.field_energy() - o.field_energy()) <= 1e-12 * o.field_energy()
     s.close()
 
 
 def test_staged_calls_match_oracle(oracle_mod):
     """Deposition / InterpolateFieldToParticle / push / PerformTracking one by one."""
-    w = W.config2(nx=5, ppc=30, steps=1)
+    w = W.thermal_cube(nx=5, ppc=30, steps=1)
     kw = W.stage_kwargs(w)
     with stage.PICStage(w.mesh, w.N, max_particles=len(w.PartState), **kw) as s:
         s.set_field(w.E)
@@ -148,7 +155,7 @@ def test_staged_calls_match_oracle(oracle_mod):
 
 
 def test_staged_time_step_equals_fused(oracle_mod):
-    w = W.config2(nx=6, ppc=20, steps=3)
+    w = W.thermal_cube(nx=6, ppc=20, steps=3)
     s, o = run_pair(oracle_mod, w, 3, staged=True)
     compare(s, o, 1.0, mesh=w.mesh)
     s.close()
@@ -172,7 +179,7 @@ def test_open_boundary_removal(oracle_mod):
 
 
 def test_deterministic_repeat(oracle_mod):
-    w = W.config2(nx=8, ppc=40, steps=4)
+    w = W.thermal_cube(nx=8, ppc=40, steps=4)
     out = []
     for _ in range(2):
         with stage.PICStage(w.mesh, w.N, max_particles=len(w.PartState), **W.stage_kwargs(w)) as s:
@@ -185,12 +192,12 @@ def test_deterministic_repeat(oracle_mod):
     assert np.array_equal(out[0][1], out[1][1])
 
 
-def test_config2_full_size_properties():
+def test_thermal_cube48_full_size_properties():
     """48^3 elements, 22.1M particles: ownership equals the closed-form owner
     after two steps, no particle lost, and the shape-function source is
     reproducible bit for bit."""
-    w = W.config2()
-    print("case config2 full size n=%d" % len(w.PartState), flush=True)
+    w = W.thermal_cube()
+    print("case thermal_cube48 full size n=%d" % len(w.PartState), flush=True)
     with stage.PICStage(w.mesh, w.N, max_particles=len(w.PartState), **W.stage_kwargs(w)) as s:
         s.set_field(w.E)
         s.set_particles(w.PartState, elem=w.elem, species=w.species)
@@ -209,3 +216,16 @@ def test_config2_full_size_properties():
         b = s.part_source()
         assert np.array_equal(a, b)
         assert s.stats()["n_lost"] == 0
+        # CalcDepositedCharge against the Gauss quadrature of the same source
+        # (affine cube: 1/sJ = (h/2)^3 at every DOF), summed exactly
+        import math
+        xg, wg = np.polynomial.legendre.leggauss(w.N + 1)
+        wijk = (wg[None, None, :] * wg[None, :, None] * wg[:, None, None]).reshape(-1)
+        J = (0.5 / 48) ** 3
+        terms = (wijk[None, :] * a[:, :, 3] * J).reshape(-1)
+        dep = s.deposited_charge()[0]
+        assert abs(dep - math.fsum(terms)) <= 1e-12 * np.abs(terms).sum()
+        # CalcPotentialEnergy of the frozen field, the same way
+        e2 = (w.E ** 2).sum(-1)
+        ref = 0.5 * stage.EPS0 * math.fsum((wijk[None, :] * J * e2).reshape(-1))
+        assert abs(s.field_energy() - ref) <= 1e-12 * ref
