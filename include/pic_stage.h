@@ -9,6 +9,8 @@
  *   InterpolateFieldToParticle()    src/particles/pic/interpolation/pic_interpolation.f90:248-322
  *   push loop (inline in stepper)   timedisc_TimeStepPoissonByBorisLeapfrog.f90:128-198
  *   PerformTracking()               src/particles/tracking/particle_tracking.f90:32-70
+ *                                   (TriaTracking, particle_triatracking.f90:63-484, and
+ *                                    RefMapping, particle_reftracking.f90:30-414)
  *
  * In the reference these are Fortran module procedures that communicate
  * through module arrays (PartState, PEM, PDM, U_N, PS_N, FieldAtParticle).
@@ -61,7 +63,7 @@ enum { PIC_PREC_FP64 = 0, PIC_PREC_MIXED = 1 };
  * (src/mesh/mesh_readin.f90; index constants src/piclas.h:136-176). */
 typedef struct {
   int32_t nElems, nSides, nNodes;
-  int32_t NGeo;                   /* geometry degree (1: linear hexahedra)        */
+  int32_t NGeo;                   /* geometry degree: 1 linear, 2 curved (RefMapping) */
   int32_t N;                      /* DG polynomial degree of E and PartSource     */
   const int32_t *ElemInfo;        /* [nElems][6]                                  */
   const int32_t *SideInfo;        /* [nSides][5]                                  */
@@ -76,7 +78,7 @@ typedef struct {
 
 /* Run-time parameters the reference reads from parameter.ini. */
 typedef struct {
-  int32_t tracking;               /* TrackingMethod (PIC_TRACK_TRIA)              */
+  int32_t tracking;               /* TrackingMethod: PIC_TRACK_TRIA or _REFMAPPING */
   int32_t deposition;             /* PIC-Deposition-Type                          */
   double  sf_radius;              /* PIC-shapefunction-radius                     */
   int32_t sf_alpha;               /* PIC-shapefunction-alpha                      */
@@ -135,6 +137,9 @@ int  pic_time_step(pic_handle *h, double dt, int32_t nsteps);
 int  pic_get_particles(pic_handle *h, double *PartState, int32_t *elem, uint8_t *inside);
 int  pic_get_last_part_pos(pic_handle *h, double *LastPartPos);
 int  pic_get_field_at_particle(pic_handle *h, double *FieldAtParticle);
+/* PartPosRef(1:3,:) of RefMapping (particle_vars.f90:52): reference coordinates
+ * of every particle in its element, caller order. */
+int  pic_get_part_pos_ref(pic_handle *h, double *PartPosRef);
 int  pic_get_part_source(pic_handle *h, double *PS);
 /* Device pointer to PS[nElems][(N+1)^3][4], valid until the next deposition. */
 int  pic_part_source_device(pic_handle *h, const double **PS_dev);

@@ -70,6 +70,8 @@ def lib():
         L.om_get_concave.argtypes = [ctypes.c_void_p, _u8p]
         L.om_get_elem_bary.argtypes = [ctypes.c_void_p, _f64p, _f64p]
         L.om_get_xieta_basis.argtypes = [ctypes.c_void_p, _f64p, _f64p]
+        L.om_get_part_pos_ref.argtypes = [ctypes.c_void_p, _f64p]
+        L.om_get_refmapping_tables.argtypes = [ctypes.c_void_p, _u8p, _f64p]
         L.om_position_in_ref_elem.argtypes = [ctypes.c_void_p, ctypes.c_int32, _f64p, _i32p, _f64p, _u8p]
         L.om_eval_field_at_ref_pos.argtypes = [ctypes.c_void_p, ctypes.c_int32, _f64p, _i32p, _f64p]
         L.om_legendre_gauss_nodes.argtypes = [ctypes.c_int32, _f64p, _f64p]
@@ -233,6 +235,19 @@ class Oracle:
         s = np.zeros((self.nElems, 6))
         lib().om_get_xieta_basis(self.h, _p(b, _f64p), _p(s, _f64p))
         return b, s
+
+    def part_pos_ref(self):
+        n = lib().om_get_nparticles(self.h)
+        a = np.zeros((n, 3))
+        lib().om_get_part_pos_ref(self.h, _p(a, _f64p))
+        return a
+
+    def refmapping_tables(self):
+        """(ElemCurved, ElemEpsOneCell) per element."""
+        c = np.zeros(self.nElems, np.uint8)
+        e = np.zeros(self.nElems)
+        lib().om_get_refmapping_tables(self.h, _p(c, _u8p), _p(e, _f64p))
+        return c.astype(bool), e
 
     def position_in_ref_elem(self, x, elem):
         x = np.ascontiguousarray(x, np.float64)

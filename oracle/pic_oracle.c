@@ -168,6 +168,13 @@ struct om_state {
   double *bary, *radius;   /* [e][3], [e]: BuildElementRadiusTria */
   double *xiBasis, *slen;  /* [e][6][3], [e][6] */
   double *xGPe, *sJ;       /* [e][nDof][3], [e][nDof] */
+  /* reference mapping (RefMapping tracking, NGeo > 1) */
+  uint8_t *curved;         /* [e] ElemCurved                                  */
+  double *dXCL1;           /* [e][8][9] dXCL_NGeo at the 8 corner CL points   */
+  double *epsOneCell;      /* [e] ElemEpsOneCell                              */
+  uint8_t *openElem;       /* [e] element has an open particle boundary side  */
+  double gmin[3], gmax[3]; /* GEO%x/y/zminglob, maxglob                       */
+  int dirPV[3];            /* GEO%DirPeriodicVectors (0-based axis)           */
   int32_t *perOff, *perNodes; /* twins of each unique node (CSR) */
   double *NodeVolume;      /* [nUnique] */
   /* 1D bases */
@@ -186,7 +193,7 @@ struct om_state {
   double *E, *PS, *NodeSource;
   /* particles */
   int n;
-  double *PartState, *LastPartPos, *Field;
+  double *PartState, *LastPartPos, *Field, *PartPosRef;
   int32_t *elem, *lastElem, *species; /* 0-based elem; -1 unknown */
   uint8_t *inside, *isNew;
   int64_t lost;
@@ -457,6 +464,151 @@ static int build_mesh(om_state *s, const om_mesh_desc *m) {
   }
   (void)n1;
 
+  /* ElemCurved (particle_mesh_tools.f90:976-996): the trilinear map of the 8
+   * corner CL points differs from XCL_NGeo by more than 1e-14 somewhere;
+   * dXCL_NGeo at the corners feeds the trilinear Newton of the others. */
+  s->curved = xcalloc(ne, 1);
+  s->dXCL1 = xcalloc((size_t)ne * 72, sizeof(double));
+  {
+    double xi1[2] = {-1.0, 1.0}, V1[MAXN * 2];
+    vandermonde(1, xi1, ng, s->xiCL, V1);
+    double X1[8 * 3], Xn[MAXN * MAXN * MAXN * 3];
+    for (int e = 0; e < ne; e++) {
+      const double *X = &s->XCL[(size_t)e * nGP * 3];
+      for (int t = 0; t < 8; t++) {
+        int i = (t & 1) * ng, j = ((t >> 1) & 1) * ng, k = ((t >> 2) & 1) * ng;
+        int p = (k * g1 + j) * g1 + i;
+        for (int c = 0; c < 3; c++) X1[t * 3 + c] = X[p * 3 + c];
+        for (int q = 0; q < 9; q++) s->dXCL1[(size_t)e * 72 + t * 9 + q] = s->dXCL[((size_t)e * nGP + p) * 9 + q];
+      }
+      if (ng == 1) continue;
+      change_basis_3d(1, ng, V1, X1, Xn);
+      for (int p = 0; p < nGP && !s->curved[e]; p++)
+        for (int c = 0; c < 3; c++)
+          if (fabs(Xn[p * 3 + c] - X[p * 3 + c]) > 1e-14) { s->curved[e] = 1; break; }
+    }
+  }
+
+  /* elements with an open particle boundary side */
+  s->openElem = xcalloc(ne, 1);
+  for (int sd = 0; sd < s->nSides; sd++) {
+    int bc = SIDEINFO(sd, SIDE_BCID);
+    if (bc > 0 && s->partBC[bc - 1] == 1) s->openElem[s->sideElem[sd]] = 1;
+  }
+  /* GEO%x/y/zminglob and maxglob over the mesh nodes; DirPeriodicVectors */
+  for (int c = 0; c < 3; c++) { s->gmin[c] = 1e300; s->gmax[c] = -1e300; }
+  for (int i = 0; i < s->nNodes; i++)
+    for (int c = 0; c < 3; c++) {
+      if (s->NodeCoords[(size_t)i * 3 + c] < s->gmin[c]) s->gmin[c] = s->NodeCoords[(size_t)i * 3 + c];
+      if (s->NodeCoords[(size_t)i * 3 + c] > s->gmax[c]) s->gmax[c] = s->NodeCoords[(size_t)i * 3 + c];
+    }
+  for (int v = 0; v < s->nPV; v++) {
+    s->dirPV[v] = -1;
+    for (int c = 0; c < 3; c++)
+      if (s->PV[v][c] != 0.0) {
+        if (s->dirPV[v] >= 0) { set_err("periodic vector %d is not aligned with an axis", v + 1); return -1; }
+        s->dirPV[v] = c;
+      }
+  }
+
+  if (s->par.tracking == OM_TRACK_REFMAPPING) {
+    /* ElemBaryNGeo = X(0,0,0) (BuildElementOriginShared, particle_mesh_build.f90:1385-1475) */
+    double L0[MAXN];
+    om_lagrange_polys(0.0, ng, s->xiCL, s->wBaryCL, L0);
+    for (int e = 0; e < ne; e++) {
+      const double *X = &s->XCL[(size_t)e * nGP * 3];
+      double xp[3] = {0, 0, 0};
+      for (int k = 0; k < g1; k++) for (int j = 0; j < g1; j++) {
+        double buf = L0[j] * L0[k];
+        for (int i = 0; i < g1; i++)
+          for (int c = 0; c < 3; c++) xp[c] = xp[c] + X[((k * g1 + j) * g1 + i) * 3 + c] * L0[i] * buf;
+      }
+      for (int c = 0; c < 3; c++) s->bary[e * 3 + c] = xp[c];
+    }
+    /* XiEtaZetaBasis from that origin and ElemRadiusNGeo = the largest distance
+     * of a Bezier control point of the six faces (BuildElementBasisAndRadius,
+     * particle_mesh_build.f90:1546-1685).  Face control points: the face's
+     * CL points mapped to the Bernstein basis. */
+    const double Xi[6][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {-1, 0, 0}, {0, -1, 0}, {0, 0, -1}};
+    double Vb[MAXN * MAXN], sVb[MAXN * MAXN];
+    for (int a = 0; a < g1; a++)
+      for (int i = 0; i < g1; i++) {
+        double binom = 1.0;
+        for (int q = 1; q <= i; q++) binom = binom * (double)(ng - q + 1) / (double)q;
+        Vb[a * g1 + i] = binom * pow(0.5 * (1.0 + s->xiCL[a]), i) * pow(0.5 * (1.0 - s->xiCL[a]), ng - i);
+      }
+    { /* Gauss-Jordan inverse of the small Bernstein Vandermonde matrix */
+      double A[MAXN][2 * MAXN];
+      for (int r = 0; r < g1; r++)
+        for (int c = 0; c < 2 * g1; c++) A[r][c] = c < g1 ? Vb[r * g1 + c] : (c - g1 == r ? 1.0 : 0.0);
+      for (int c = 0; c < g1; c++) {
+        int piv = c;
+        for (int r = c + 1; r < g1; r++) if (fabs(A[r][c]) > fabs(A[piv][c])) piv = r;
+        for (int q = 0; q < 2 * g1; q++) { double t = A[c][q]; A[c][q] = A[piv][q]; A[piv][q] = t; }
+        double d = A[c][c];
+        for (int q = 0; q < 2 * g1; q++) A[c][q] /= d;
+        for (int r = 0; r < g1; r++)
+          if (r != c) { double f = A[r][c]; for (int q = 0; q < 2 * g1; q++) A[r][q] -= f * A[c][q]; }
+      }
+      for (int r = 0; r < g1; r++) for (int c = 0; c < g1; c++) sVb[r * g1 + c] = A[r][g1 + c];
+    }
+    for (int e = 0; e < ne; e++) {
+      const double *X = &s->XCL[(size_t)e * nGP * 3];
+      for (int d = 0; d < 6; d++) {
+        double L1[MAXN], L2[MAXN], L3[MAXN];
+        om_lagrange_polys(Xi[d][0], ng, s->xiCL, s->wBaryCL, L1);
+        om_lagrange_polys(Xi[d][1], ng, s->xiCL, s->wBaryCL, L2);
+        om_lagrange_polys(Xi[d][2], ng, s->xiCL, s->wBaryCL, L3);
+        double xp[3] = {0, 0, 0};
+        for (int k = 0; k < g1; k++) for (int j = 0; j < g1; j++) for (int i = 0; i < g1; i++)
+          for (int c = 0; c < 3; c++) xp[c] = xp[c] + X[((k * g1 + j) * g1 + i) * 3 + c] * L1[i] * L2[j] * L3[k];
+        double *b = &s->xiBasis[(e * 6 + d) * 3];
+        for (int c = 0; c < 3; c++) b[c] = xp[c] - s->bary[e * 3 + c];
+        s->slen[e * 6 + d] = 1.0 / (b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+      }
+      double r = 0.0;
+      for (int f = 0; f < 6; f++) {
+        int ax = f % 3, fixed = f < 3 ? ng : 0; /* faces xi=+-1, eta=+-1, zeta=+-1 */
+        double P[MAXN][MAXN][3], Bz[MAXN][MAXN][3], T[MAXN][MAXN][3];
+        for (int a = 0; a < g1; a++) for (int bq = 0; bq < g1; bq++) {
+          int ijk[3];
+          ijk[ax] = fixed;
+          ijk[(ax + 1) % 3] = a;
+          ijk[(ax + 2) % 3] = bq;
+          for (int c = 0; c < 3; c++) P[a][bq][c] = X[((ijk[2] * g1 + ijk[1]) * g1 + ijk[0]) * 3 + c];
+        }
+        for (int a = 0; a < g1; a++) for (int bq = 0; bq < g1; bq++) for (int c = 0; c < 3; c++) {
+          double acc = 0.0;
+          for (int q = 0; q < g1; q++) acc += sVb[a * g1 + q] * P[q][bq][c];
+          T[a][bq][c] = acc;
+        }
+        for (int a = 0; a < g1; a++) for (int bq = 0; bq < g1; bq++) for (int c = 0; c < 3; c++) {
+          double acc = 0.0;
+          for (int q = 0; q < g1; q++) acc += sVb[bq * g1 + q] * T[a][q][c];
+          Bz[a][bq][c] = acc;
+        }
+        for (int a = 0; a < g1; a++) for (int bq = 0; bq < g1; bq++) {
+          double d0 = Bz[a][bq][0] - s->bary[e * 3], d1 = Bz[a][bq][1] - s->bary[e * 3 + 1], d2 = Bz[a][bq][2] - s->bary[e * 3 + 2];
+          double v = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+          if (v > r) r = v;
+        }
+      }
+      s->radius[e] = r;
+    }
+    /* ElemEpsOneCell = 1 + sqrt(3 scaleJ RefMappingEps), scaleJ = max sJ / min sJ
+     * (BuildEpsOneCell, particle_mesh_build.f90:447-642) */
+    s->epsOneCell = xcalloc(ne, sizeof(double));
+    for (int e = 0; e < ne; e++) {
+      double mx = -1e300, mn = 1e300;
+      for (int p = 0; p < nDof; p++) {
+        double v = s->sJ[(size_t)e * nDof + p];
+        if (v > mx) mx = v;
+        if (v < mn) mn = v;
+      }
+      s->epsOneCell[e] = 1.0 + sqrt(3.0 * (mx / mn) * s->par.refmapping_eps);
+    }
+  }
+
   /* Periodic node twins (InitializePeriodicNodes, pic_depo.f90:1218-): every
    * unique node that coincides with this node under a combination of the
    * periodic vectors with coefficients in {-1,0,1}.  The reference finds the
@@ -581,11 +733,18 @@ static int build_mesh(om_state *s, const om_mesh_desc *m) {
     for (int e = 0; e < ne; e++) {
       for (int c = 0; c < 3; c++) { emin[e * 3 + c] = 1e300; emax[e * 3 + c] = -1e300; }
       int fn = ELEMINFO(e, ELEM_FIRSTNODEIND), ln = ELEMINFO(e, ELEM_LASTNODEIND);
-      for (int nn = fn; nn < ln; nn++) for (int c = 0; c < 3; c++) {
-        double v = s->NodeCoords[(size_t)nn * 3 + c];
-        if (v < emin[e * 3 + c]) emin[e * 3 + c] = v;
-        if (v > emax[e * 3 + c]) emax[e * 3 + c] = v;
-      }
+      if (s->par.tracking == OM_TRACK_REFMAPPING) {
+        /* the bounding sphere (ElemBaryNGeo, ElemRadiusNGeo) holds the curved element */
+        for (int c = 0; c < 3; c++) {
+          emin[e * 3 + c] = s->bary[e * 3 + c] - s->radius[e];
+          emax[e * 3 + c] = s->bary[e * 3 + c] + s->radius[e];
+        }
+      } else
+        for (int nn = fn; nn < ln; nn++) for (int c = 0; c < 3; c++) {
+          double v = s->NodeCoords[(size_t)nn * 3 + c];
+          if (v < emin[e * 3 + c]) emin[e * 3 + c] = v;
+          if (v > emax[e * 3 + c]) emax[e * 3 + c] = v;
+        }
       for (int c = 0; c < 3; c++) {
         if (emin[e * 3 + c] < s->bmin[c]) s->bmin[c] = emin[e * 3 + c];
         if (emax[e * 3 + c] > s->bmax[c]) s->bmax[c] = emax[e * 3 + c];
@@ -649,15 +808,14 @@ static void get_inv(const double M[3][3], double sdet, double I[3][3]) {
 /* RefElemNewton, eval_xyz.f90:298-445.  mode 1 = ForceMode (aborts without
  * isSuccessful), mode 2 = default.  Returns isSuccessful (1) or 0; sets
  * *aborted when the reference would abort. */
-static int ref_elem_newton(const om_state *s, double xi[3], const double x_in[3], int e,
-                           int mode, int want_success, int *aborted) {
-  const int ng = s->NGeo, g1 = ng + 1;
-  const double *X = &s->XCL[(size_t)e * s->nGP * 3];
-  const double *dX = &s->dXCL[(size_t)e * s->nGP * 9];
+static int ref_elem_newton_basis(const om_state *s, double xi[3], const double x_in[3], int ng,
+                                 const double *xiCL, const double *wBaryCL, const double *X, const double *dX,
+                                 int mode, int want_success, int *aborted) {
+  const int g1 = ng + 1;
   double Lag[3][MAXN], F[3], xi_old[3];
   int ok = 1;
   *aborted = 0;
-  for (int d = 0; d < 3; d++) om_lagrange_polys(xi[d], ng, s->xiCL, s->wBaryCL, Lag[d]);
+  for (int d = 0; d < 3; d++) om_lagrange_polys(xi[d], ng, xiCL, wBaryCL, Lag[d]);
   for (int c = 0; c < 3; c++) F[c] = -x_in[c];
   for (int k = 0; k < g1; k++) for (int j = 0; j < g1; j++) {
     double buff = Lag[1][j] * Lag[2][k];
@@ -695,7 +853,7 @@ static int ref_elem_newton(const om_state *s, double xi[3], const double x_in[3]
     int iArmijo = 1;
     while (NormF > NormF_old * (1. - 0.0001 * lambda) && iArmijo <= 8) {
       for (int d = 0; d < 3; d++) xi[d] = xi_old[d] - lambda * dxi[d];
-      for (int d = 0; d < 3; d++) om_lagrange_polys(xi[d], ng, s->xiCL, s->wBaryCL, Lag[d]);
+      for (int d = 0; d < 3; d++) om_lagrange_polys(xi[d], ng, xiCL, wBaryCL, Lag[d]);
       for (int c = 0; c < 3; c++) F[c] = -x_in[c];
       for (int k = 0; k < g1; k++) for (int j = 0; j < g1; j++) {
         double buff = Lag[1][j] * Lag[2][k];
@@ -715,6 +873,25 @@ static int ref_elem_newton(const om_state *s, double xi[3], const double x_in[3]
     }
   }
   return ok;
+}
+
+/* GetPositionInRefElem's choice of map (eval_xyz.f90:86-117): the full NGeo
+ * map for curved elements and NGeo = 1 meshes, else the trilinear map of the
+ * 8 corner CL points (XCL_NGeo1, dXCL_NGeo1). */
+static int ref_elem_newton(const om_state *s, double xi[3], const double x_in[3], int e,
+                           int mode, int want_success, int *aborted) {
+  if (s->NGeo == 1 || (s->curved && s->curved[e]))
+    return ref_elem_newton_basis(s, xi, x_in, s->NGeo, s->xiCL, s->wBaryCL, &s->XCL[(size_t)e * s->nGP * 3],
+                                 &s->dXCL[(size_t)e * s->nGP * 9], mode, want_success, aborted);
+  static const double xi1[2] = {-1.0, 1.0}, wb1[2] = {-0.5, 0.5};
+  const int g = s->NGeo, g1 = g + 1;
+  double X1[8][3];
+  for (int t = 0; t < 8; t++) {
+    int i = (t & 1) * g, j = ((t >> 1) & 1) * g, k = ((t >> 2) & 1) * g;
+    for (int c = 0; c < 3; c++) X1[t][c] = s->XCL[((size_t)e * s->nGP + (k * g1 + j) * g1 + i) * 3 + c];
+  }
+  return ref_elem_newton_basis(s, xi, x_in, 1, xi1, wb1, &X1[0][0], &s->dXCL1[(size_t)e * 72], mode, want_success,
+                               aborted);
 }
 
 /* GetRefNewtonStartValue, eval_xyz.f90:498-612 */
@@ -833,6 +1010,12 @@ static void em_field_dw(const om_state *s, int e, const double x[3], double U[3]
 
 /* ================================================================ API basics */
 om_state *om_create(const om_mesh_desc *mesh, const om_params *par) {
+  if (par->tracking != OM_TRACK_TRIA && par->tracking != OM_TRACK_REFMAPPING) { set_err("tracking method"); return NULL; }
+  if (par->tracking == OM_TRACK_REFMAPPING && par->deposition == OM_DEPO_CVWM) {
+    /* pic_depo_method.f90:127-129 */
+    set_err("cell_volweight_mean needs TriaTracking");
+    return NULL;
+  }
   om_state *s = xcalloc(1, sizeof(om_state));
   s->par = *par;
   s->nthreads = 1;
@@ -851,7 +1034,17 @@ om_state *om_create(const om_mesh_desc *mesh, const om_params *par) {
     s->w_sf = 1. / (2. * BetaFac * a + 2 * BetaFac) * (a + 1.) / (PI_ * (s->r_sf * s->r_sf * s->r_sf));
     /* InitPeriodicSFCaseMatrix, pic_depo.f90:828-910 (dim_sf = 3) */
     if (s->nPV <= 0) s->nCases = 0;
-    else {
+    else if (s->nPV < 3) {
+      /* the reference's 3D table (below) adds PeriodicVectors(:,2:3), which
+       * do not exist with fewer vectors; use the 3^nPV combinations of the
+       * vectors that do, in the same +1 / 0 / -1 order */
+      s->nCases = s->nPV == 1 ? 3 : 9;
+      memset(s->caseMat, 0, sizeof s->caseMat);
+      for (int c = 0; c < s->nCases; c++) {
+        int cc = c;
+        for (int v = s->nPV - 1; v >= 0; v--) { s->caseMat[c][v] = 1 - cc % 3; cc /= 3; }
+      }
+    } else {
       s->nCases = 27;
       memset(s->caseMat, 0, sizeof s->caseMat);
       for (int c = 0; c < 9; c++) s->caseMat[c][0] = 1;
@@ -875,7 +1068,8 @@ void om_destroy(om_state *s) {
                   s->sideLocal, s->sideNbSide, s->ElemSideNodeID, s->Concave, s->ElemNodeID, s->XCL, s->dXCL,
                   s->bary, s->radius, s->xiBasis, s->slen, s->xGPe, s->sJ, s->perOff, s->perNodes,
                   s->NodeVolume, s->binOff, s->binElems, s->E, s->PS, s->NodeSource, s->PartState,
-                  s->LastPartPos, s->Field, s->elem, s->lastElem, s->species, s->inside, s->isNew};
+                  s->LastPartPos, s->Field, s->elem, s->lastElem, s->species, s->inside, s->isNew,
+                  s->curved, s->dXCL1, s->epsOneCell, s->openElem, s->PartPosRef};
   for (size_t i = 0; i < sizeof ptrs / sizeof ptrs[0]; i++) free(ptrs[i]);
   free(s);
 }
@@ -890,8 +1084,9 @@ int om_set_field(om_state *s, const double *E) {
 int om_set_particles(om_state *s, int32_t n, const double *PartState, const int32_t *elem,
                      const int32_t *species, const uint8_t *is_new) {
   free(s->PartState); free(s->LastPartPos); free(s->Field); free(s->elem); free(s->lastElem);
-  free(s->species); free(s->inside); free(s->isNew);
+  free(s->species); free(s->inside); free(s->isNew); free(s->PartPosRef);
   s->n = n;
+  s->PartPosRef = xcalloc((size_t)n * 3, sizeof(double));
   s->PartState = xcalloc((size_t)n * 6, sizeof(double));
   s->LastPartPos = xcalloc((size_t)n * 3, sizeof(double));
   s->Field = xcalloc((size_t)n * 6, sizeof(double));
@@ -908,6 +1103,11 @@ int om_set_particles(om_state *s, int32_t n, const double *PartState, const int3
     s->inside[i] = 1;
     s->isNew[i] = is_new ? is_new[i] : 0;
     if (s->elem[i] >= s->nElems) { set_err("bad element id"); return -1; }
+    if (s->par.tracking == OM_TRACK_REFMAPPING && s->elem[i] >= 0) {
+      /* LocateParticleInElement leaves PartPosRef of the owner (particle_localization.f90:47-74) */
+      int ab;
+      position_in_ref_elem(s, &s->PartState[(size_t)i * 6], &s->PartPosRef[(size_t)i * 3], s->elem[i], 0, 0, &ab);
+    }
   }
   s->lost = 0;
   return 0;
@@ -936,6 +1136,15 @@ int32_t om_get_n_unique_nodes(const om_state *s) { return s->nUnique; }
 int om_get_elem_bary(const om_state *s, double *b, double *r) {
   if (b) memcpy(b, s->bary, (size_t)s->nElems * 3 * sizeof(double));
   if (r) memcpy(r, s->radius, (size_t)s->nElems * sizeof(double));
+  return 0;
+}
+int om_get_part_pos_ref(const om_state *s, double *ref) {
+  memcpy(ref, s->PartPosRef, (size_t)s->n * 3 * sizeof(double));
+  return 0;
+}
+int om_get_refmapping_tables(const om_state *s, uint8_t *curved, double *eps) {
+  if (curved) memcpy(curved, s->curved, (size_t)s->nElems);
+  if (eps) for (int e = 0; e < s->nElems; e++) eps[e] = s->epsOneCell ? s->epsOneCell[e] : 0.0;
   return 0;
 }
 int om_get_xieta_basis(const om_state *s, double *b, double *sl) {
@@ -1202,12 +1411,130 @@ static int tria_track_one(om_state *s, int i) {
   return 0;
 }
 
+static double max_abs3(const double v[3]) { return fmax(fabs(v[0]), fmax(fabs(v[1]), fabs(v[2]))); }
+
+/* PeriodicMovement with FastPeriodic = F (particle_reftracking.f90:697-983):
+ * a particle beyond the global bounds along the axis of a periodic vector is
+ * moved back by that vector, LastPartPos with it.  Returns 1 if it moved. */
+static int periodic_movement(const om_state *s, double *x, double *last) {
+  int moved = 0;
+  for (int dir = 0; dir < 3; dir++) {
+    int v = -1;
+    for (int q = 0; q < s->nPV; q++)
+      if (s->dirPV[q] == dir) { v = q; break; }
+    if (v < 0) continue;
+    if (x[dir] > s->gmax[dir]) {
+      const double sg = s->PV[v][dir] > 0 ? -1.0 : 1.0;
+      for (int c = 0; c < 3; c++) { x[c] = x[c] + sg * s->PV[v][c]; last[c] = last[c] + sg * s->PV[v][c]; }
+      moved = 1;
+    }
+    if (x[dir] < s->gmin[dir]) {
+      const double sg = s->PV[v][dir] > 0 ? 1.0 : -1.0;
+      for (int c = 0; c < 3; c++) { x[c] = x[c] + sg * s->PV[v][c]; last[c] = last[c] + sg * s->PV[v][c]; }
+      moved = 1;
+    }
+  }
+  return moved;
+}
+
+/* FIBGM cell of a point as ParticleRefTracking computes it (:180-186) */
+static size_t bgm_cell(const om_state *s, const double x[3]) {
+  int b3[3];
+  for (int c = 0; c < 3; c++) {
+    int v = (int)floor((x[c] - s->bmin[c]) / s->bdelta[c]);
+    if (v < 0) v = 0;
+    v = v + 1;
+    if (v > s->nb[c]) v = s->nb[c];
+    b3[c] = v - 1;
+  }
+  return ((size_t)b3[2] * s->nb[1] + b3[1]) * s->nb[0] + b3[0];
+}
+
+/* ParticleRefTracking, particle_reftracking.f90:30-414, for CartesianPeriodic = T
+ * (periodic boundaries by PeriodicMovement) and open walls.  The reference
+ * sends particles of elements with boundary sides through ParticleBCTracking,
+ * which intersects the trajectory with the (curved) boundary faces; here a
+ * particle that no element holds after the move and whose last element has an
+ * open side has left through that wall and is removed.  Both decide the same
+ * unless a trajectory leaves and re-enters the domain within one step. */
+static int ref_track_one(om_state *s, int i) {
+  double *x = &s->PartState[(size_t)i * 6], *last = &s->LastPartPos[(size_t)i * 3];
+  double *ref = &s->PartPosRef[(size_t)i * 3];
+  const int LastElemID = s->lastElem[i];
+  int ab;
+  if (s->nPV > 0)
+    for (int g = 0; g < 4; g++)
+      if (!periodic_movement(s, x, last)) break;
+  position_in_ref_elem(s, x, ref, LastElemID, 0, 0, &ab);
+  if (max_abs3(ref) < 1.0) { s->elem[i] = LastElemID; return 0; }
+  const int oldElemID = LastElemID;
+  const size_t b = bgm_cell(s, x);
+  const int n = s->binOff[b + 1] - s->binOff[b];
+  double dist[512];
+  int list[512];
+  if (n > 512) { set_err("too many elements in a background cell"); return -1; }
+  if (n > 1) {
+    for (int q = 0; q < n; q++) {
+      int e = s->binElems[s->binOff[b] + q];
+      list[q] = e;
+      if (e == oldElemID) dist[q] = -HUGE_VAL;
+      else {
+        double d0 = x[0] - s->bary[e * 3], d1 = x[1] - s->bary[e * 3 + 1], d2 = x[2] - s->bary[e * 3 + 2];
+        dist[q] = d0 * d0 + d1 * d1 + d2 * d2;
+        if (dist[q] > s->radius[e] * s->radius[e]) dist[q] = -HUGE_VAL;
+      }
+    }
+    for (int a = 1; a < n; a++) { /* InsertionSort (utils), stable */
+      double dv = dist[a]; int lv = list[a]; int j = a - 1;
+      while (j >= 0 && dist[j] > dv) { dist[j + 1] = dist[j]; list[j + 1] = list[j]; j--; }
+      dist[j + 1] = dv; list[j + 1] = lv;
+    }
+  } else if (n == 1) {
+    dist[0] = 0.0;
+    list[0] = s->binElems[s->binOff[b]];
+  }
+  double oldXi[3] = {ref[0], ref[1], ref[2]}, newXi[3] = {HUGE_VAL, HUGE_VAL, HUGE_VAL};
+  int newElemID = -1;
+  for (int q = 0; q < n; q++) {
+    if (dist[q] == -HUGE_VAL) continue;
+    const int e = list[q];
+    position_in_ref_elem(s, x, ref, e, 0, 0, &ab);
+    if (max_abs3(ref) < 1.0) { s->elem[i] = e; return 0; }
+    if (max_abs3(ref) < max_abs3(newXi)) {
+      for (int c = 0; c < 3; c++) newXi[c] = ref[c];
+      newElemID = e;
+    }
+  }
+  if (s->openElem[LastElemID]) { /* left the domain through an open wall: RemoveParticle */
+    s->inside[i] = 0;
+    return 0;
+  }
+  int TestElem;
+  if (max_abs3(oldXi) < max_abs3(newXi)) {
+    for (int c = 0; c < 3; c++) ref[c] = oldXi[c];
+    TestElem = oldElemID;
+  } else {
+    for (int c = 0; c < 3; c++) ref[c] = newXi[c];
+    TestElem = newElemID;
+  }
+  s->elem[i] = TestElem;
+  if (TestElem < 0 || max_abs3(ref) > s->epsOneCell[TestElem]) {
+    set_err("RefMapping: particle %d not inside of element %d", i + 1, TestElem + 1);
+    return -1;
+  }
+  return 0;
+}
+
+static int track_one(om_state *s, int i) {
+  return s->par.tracking == OM_TRACK_REFMAPPING ? ref_track_one(s, i) : tria_track_one(s, i);
+}
+
 int om_perform_tracking(om_state *s) {
   int err = 0;
-  /* ParticleTriaTracking, particle_triatracking.f90:63-134 */
+  /* ParticleTriaTracking (particle_triatracking.f90:63-134) / ParticleRefTracking */
   for (int i = 0; i < s->n; i++)
     if (s->inside[i] && s->lastElem[i] >= 0)
-      if (tria_track_one(s, i) != 0) err = -1;
+      if (track_one(s, i) != 0) err = -1;
   return err;
 }
 
@@ -1243,8 +1570,15 @@ static int single_point_to_element(const om_state *s, const double x[3]) {
     }
     for (int q = 0; q < n; q++) {
       if (dist[q] == -1.0) continue;
-      double det[6][2];
-      if (inside_quad_3d(s, x, list[q], det)) { found = list[q]; break; }
+      if (s->par.tracking == OM_TRACK_REFMAPPING) {
+        double xi[3];
+        int ab;
+        position_in_ref_elem(s, x, xi, list[q], 0, 0, &ab);
+        if (max_abs3(xi) <= s->epsOneCell[list[q]]) { found = list[q]; break; }
+      } else {
+        double det[6][2];
+        if (inside_quad_3d(s, x, list[q], det)) { found = list[q]; break; }
+      }
     }
   }
   free(dist); free(list);
@@ -1257,6 +1591,10 @@ int om_locate_particles(om_state *s) {
     if (s->elem[i] < 0) {
       s->elem[i] = single_point_to_element(s, &s->PartState[(size_t)i * 6]);
       if (s->elem[i] < 0) { s->inside[i] = 0; missing++; }
+      else if (s->par.tracking == OM_TRACK_REFMAPPING) {
+        int ab;
+        position_in_ref_elem(s, &s->PartState[(size_t)i * 6], &s->PartPosRef[(size_t)i * 3], s->elem[i], 0, 0, &ab);
+      }
     }
   return missing;
 }
@@ -1270,8 +1608,11 @@ static void interpolate_one(om_state *s, int i) {
   for (int c = 0; c < 6; c++) F[c] = s->par.external_field[c];
   /* GetInterpolatedFieldPartPos, pic_interpolation_tools.f90:198-254 */
   double xi[3];
-  int ab;
-  int ok = position_in_ref_elem(s, x, xi, e, 0, 1, &ab);
+  int ab, ok = 1;
+  if (s->par.tracking == OM_TRACK_REFMAPPING) /* PartPosRef from the tracking (:242-244) */
+    for (int c = 0; c < 3; c++) xi[c] = s->PartPosRef[(size_t)i * 3 + c];
+  else
+    ok = position_in_ref_elem(s, x, xi, e, 0, 1, &ab);
   double U[6] = {0, 0, 0, 0, 0, 0};
   if (!ok && s->par.deposition == OM_DEPO_CVWM) em_field_dw(s, e, x, U);
   else eval_field_at_ref_pos(s, xi, e, U); /* GetEMField, :399-455 */
@@ -1498,7 +1839,6 @@ static void sf_source_one(const om_state *s, int i, double *PS, uint8_t *done) {
 
 static int deposit_sf(om_state *s) {
   const size_t nps = (size_t)s->nElems * s->nDof * 4;
-  if (s->nPV > 0 && s->nPV != 3) { set_err("3D shape function with %d periodic vectors unsupported", s->nPV); return -1; }
   if (s->nthreads <= 1) {
     uint8_t *done = xcalloc(s->nElems, 1);
     for (int i = 0; i < s->n; i++) {
@@ -1555,7 +1895,7 @@ int om_time_step(om_state *s, double dt, int32_t nsteps) {
 #pragma omp parallel for num_threads(s->nthreads) schedule(static) reduction(| : err)
     for (int i = 0; i < s->n; i++)
       if (s->inside[i] && s->lastElem[i] >= 0)
-        if (tria_track_one(s, i) != 0) err |= 1;
+        if (track_one(s, i) != 0) err |= 1;
     if (err) return -1;
   }
   return 0;

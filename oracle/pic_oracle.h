@@ -3,7 +3,8 @@
  *
  * TEST INFRASTRUCTURE ONLY.  This is a plain-C, FP64, operation-for-operation
  * restatement of the reference's particle stage (field interpolation, the
- * Boris-Leapfrog push, TriaTracking, cell_volweight_mean and shape_function
+ * Boris-Leapfrog push, TriaTracking and RefMapping tracking on linear and
+ * curved hexahedra, cell_volweight_mean and shape_function
  * deposition, and the mesh tables those routines read).  It is the checker
  * for the HIP implementation in piclas_amd/ and the CPU baseline timed
  * beside it.  Only tests/, __graft_entry__.smoke() and bench.py's
@@ -96,6 +97,8 @@ int  om_get_node_volume(const om_state *s, double *nv);      /* [nUniqueNodes]  
 int32_t om_get_n_unique_nodes(const om_state *s);
 int  om_get_elem_bary(const om_state *s, double *bary, double *radius);
 int  om_get_xieta_basis(const om_state *s, double *basis, double *slen); /* [nElems][6][3], [nElems][6] */
+int  om_get_part_pos_ref(const om_state *s, double *PartPosRef);         /* [nPart][3] (RefMapping) */
+int  om_get_refmapping_tables(const om_state *s, uint8_t *ElemCurved, double *ElemEpsOneCell); /* [nElems] */
 
 /* Reference-coordinate mapping of single points (GetPositionInRefElem). */
 int  om_position_in_ref_elem(om_state *s, int32_t n, const double *x, const int32_t *elem,

@@ -31,11 +31,24 @@ struct DevMesh {
   const double *slen;       // [e][6]
   const double *xGPe;       // [e][nDof][3]
   const double *E;          // [e][nDof][3]
-  double xiCL[2], wBaryCL[2];
+  double xiCL[2], wBaryCL[2];   // linear CL basis (corner map)
   double xGP[8], wBaryGP[8];
   double PV[3][3];
   double ext[6];
   double charge[8], mass[8], mpf[8];
+  // reference-mapping tracking (curved NGeo = 2 or linear elements)
+  int NGeo, nGP, tracking, nPV;
+  const double *XCL;        // [e][nGP][3] XCL_NGeo
+  const double *dXCLg;      // [e][nGP][3][3] dXCL_NGeo
+  const uint8_t *curved;    // [e] ElemCurved
+  const double *epsOneCell; // [e]
+  const uint8_t *openElem;  // [e] element with an open boundary side
+  const int32_t *binOff, *binElems;
+  double bmin[3], bdelta[3];
+  int nb[3];
+  double gmin[3], gmax[3];
+  int dirPV[3];
+  double xiCLg[3], wBaryCLg[3];  // CL basis of NGeo
 };
 
 struct DevParts {
@@ -47,6 +60,7 @@ struct DevParts {
   double *lx, *ly, *lz;   // LastPartPos
   int32_t *lastElem;      // PEM%LastGlobalElemID
   double *F;              // FieldAtParticle [n][6]
+  double *rx, *ry, *rz;   // PartPosRef (RefMapping)
 };
 
 constexpr double kEpsMach = 2.220446049250313e-16;

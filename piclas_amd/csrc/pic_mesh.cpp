@@ -124,7 +124,9 @@ struct UnionFind {
 std::string build_host_mesh(const pic_mesh_desc &m, const pic_params &par, HostMesh &hm) {
   const int ne = m.nElems;
   if (ne <= 0 || m.nSides != 6 * ne) return "mesh: expected 6 sides per element (no mortars)";
-  if (m.NGeo < 1 || m.NGeo > kMaxNGeo) return "mesh: NGeo > 1 (curved elements) is not supported by this build";
+  if (m.NGeo < 1 || m.NGeo > kMaxNGeo) return "mesh: NGeo must be 1 or 2";
+  const bool refmap = par.tracking == PIC_TRACK_REFMAPPING;
+  if (!refmap && m.NGeo != 1) return "mesh: TriaTracking needs linear elements (NGeo = 1); use RefMapping for NGeo = 2";
   if (m.N < 1 || m.N > kMaxN) return "mesh: N out of range 1..7";
   hm.nElems = ne; hm.nSides = m.nSides; hm.nNodes = m.nNodes; hm.NGeo = m.NGeo; hm.N = m.N;
   hm.nDof = (m.N + 1) * (m.N + 1) * (m.N + 1);
@@ -323,6 +325,186 @@ std::string build_host_mesh(const pic_mesh_desc &m, const pic_params &par, HostM
     }
   }
 
+  // ---- reference-mapping tables ---------------------------------------------
+  // ElemCurved: the trilinear map of the 8 corner CL points differs from
+  // XCL_NGeo by more than 1e-14 (particle_mesh_tools.f90:976-996); dXCL_NGeo
+  // at the corners drives the trilinear Newton of the other elements.
+  hm.curved.assign(ne, 0);
+  hm.dXCL1.assign(size_t(ne) * 72, 0.0);
+  {
+    double V1[8 * 2];
+    const double x1[2] = {-1.0, 1.0};
+    double w1[2];
+    bary_weights(1, x1, w1);
+    for (int io = 0; io <= m.NGeo; ++io) lagrange(hm.xiCL[io], 1, x1, w1, &V1[io * 2]);
+    std::vector<double> X1(24), Xn(size_t(nGP) * 3);
+    for (int e = 0; e < ne; ++e) {
+      const double *X = &hm.XCL[size_t(e) * nGP * 3];
+      for (int t = 0; t < 8; ++t) {
+        const int p = corner_node(t);
+        for (int c = 0; c < 3; ++c) X1[t * 3 + c] = X[p * 3 + c];
+        for (int q = 0; q < 9; ++q) hm.dXCL1[size_t(e) * 72 + t * 9 + q] = hm.dXCL[(size_t(e) * nGP + p) * 9 + q];
+      }
+      if (m.NGeo == 1) continue;
+      change_basis(1, m.NGeo, V1, X1.data(), Xn.data());
+      for (int p = 0; p < nGP && !hm.curved[e]; ++p)
+        for (int c = 0; c < 3; ++c)
+          if (std::fabs(Xn[p * 3 + c] - X[p * 3 + c]) > 1e-14) { hm.curved[e] = 1; break; }
+    }
+  }
+  hm.openElem.assign(ne, 0);
+  for (int e = 0; e < ne; ++e)
+    for (int l = 0; l < 6; ++l)
+      if (hm.sideBC[e * 6 + l] == PIC_BC_OPEN) hm.openElem[e] = 1;
+  for (int c = 0; c < 3; ++c) { hm.gmin[c] = 1e300; hm.gmax[c] = -1e300; }
+  for (int i = 0; i < m.nNodes; ++i)
+    for (int c = 0; c < 3; ++c) {
+      hm.gmin[c] = std::min(hm.gmin[c], m.NodeCoords[size_t(i) * 3 + c]);
+      hm.gmax[c] = std::max(hm.gmax[c], m.NodeCoords[size_t(i) * 3 + c]);
+    }
+  for (int v = 0; v < hm.nPV; ++v) {
+    hm.dirPV[v] = -1;
+    for (int c = 0; c < 3; ++c)
+      if (hm.PV[v][c] != 0.0) {
+        if (hm.dirPV[v] >= 0) return "mesh: periodic vectors must be aligned with an axis";
+        hm.dirPV[v] = c;
+      }
+  }
+  if (refmap) {
+    // ElemBaryNGeo = X(0,0,0) (BuildElementOriginShared, particle_mesh_build.f90:1385-1475)
+    double L0[8];
+    lagrange(0.0, m.NGeo, hm.xiCL, hm.wBaryCL, L0);
+    for (int e = 0; e < ne; ++e) {
+      const double *X = &hm.XCL[size_t(e) * nGP * 3];
+      double xp[3] = {0, 0, 0};
+      for (int k = 0; k < g1; ++k) for (int j = 0; j < g1; ++j) {
+        const double buf = L0[j] * L0[k];
+        for (int i = 0; i < g1; ++i)
+          for (int c = 0; c < 3; ++c) xp[c] = xp[c] + X[((k * g1 + j) * g1 + i) * 3 + c] * L0[i] * buf;
+      }
+      for (int c = 0; c < 3; ++c) hm.bary[e * 3 + c] = xp[c];
+    }
+    // XiEtaZetaBasis from that origin, ElemRadiusNGeo = largest distance of a
+    // Bezier control point of the six faces (BuildElementBasisAndRadius,
+    // particle_mesh_build.f90:1546-1685)
+    const double Xi[6][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {-1, 0, 0}, {0, -1, 0}, {0, 0, -1}};
+    double sVb[9];
+    {
+      double A[3][6];
+      for (int a = 0; a < g1; ++a)
+        for (int i = 0; i < g1; ++i) {
+          double binom = 1.0;
+          for (int q = 1; q <= i; ++q) binom = binom * double(m.NGeo - q + 1) / double(q);
+          A[a][i] = binom * std::pow(0.5 * (1.0 + hm.xiCL[a]), i) * std::pow(0.5 * (1.0 - hm.xiCL[a]), m.NGeo - i);
+          A[a][g1 + i] = a == i ? 1.0 : 0.0;
+        }
+      for (int c = 0; c < g1; ++c) {  // Gauss-Jordan with partial pivoting
+        int piv = c;
+        for (int r = c + 1; r < g1; ++r) if (std::fabs(A[r][c]) > std::fabs(A[piv][c])) piv = r;
+        for (int q = 0; q < 2 * g1; ++q) std::swap(A[c][q], A[piv][q]);
+        const double d = A[c][c];
+        for (int q = 0; q < 2 * g1; ++q) A[c][q] /= d;
+        for (int r = 0; r < g1; ++r)
+          if (r != c) { const double f = A[r][c]; for (int q = 0; q < 2 * g1; ++q) A[r][q] -= f * A[c][q]; }
+      }
+      for (int r = 0; r < g1; ++r) for (int c = 0; c < g1; ++c) sVb[r * g1 + c] = A[r][g1 + c];
+    }
+    for (int e = 0; e < ne; ++e) {
+      const double *X = &hm.XCL[size_t(e) * nGP * 3];
+      double L[3][8];
+      for (int d = 0; d < 6; ++d) {
+        for (int q = 0; q < 3; ++q) lagrange(Xi[d][q], m.NGeo, hm.xiCL, hm.wBaryCL, L[q]);
+        double xp[3] = {0, 0, 0};
+        for (int k = 0; k < g1; ++k) for (int j = 0; j < g1; ++j) for (int i = 0; i < g1; ++i)
+          for (int c = 0; c < 3; ++c) xp[c] = xp[c] + X[((k * g1 + j) * g1 + i) * 3 + c] * L[0][i] * L[1][j] * L[2][k];
+        double *b = &hm.xiBasis[(e * 6 + d) * 3];
+        for (int c = 0; c < 3; ++c) b[c] = xp[c] - hm.bary[e * 3 + c];
+        hm.slen[e * 6 + d] = 1.0 / (b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+      }
+      double r = 0.0;
+      for (int f = 0; f < 6; ++f) {
+        const int ax = f % 3, fixed = f < 3 ? m.NGeo : 0;
+        double P[3][3][3], T[3][3][3];
+        for (int a = 0; a < g1; ++a) for (int bq = 0; bq < g1; ++bq) {
+          int ijk[3];
+          ijk[ax] = fixed; ijk[(ax + 1) % 3] = a; ijk[(ax + 2) % 3] = bq;
+          for (int c = 0; c < 3; ++c) P[a][bq][c] = X[((ijk[2] * g1 + ijk[1]) * g1 + ijk[0]) * 3 + c];
+        }
+        for (int a = 0; a < g1; ++a) for (int bq = 0; bq < g1; ++bq) for (int c = 0; c < 3; ++c) {
+          double acc = 0.0;
+          for (int q = 0; q < g1; ++q) acc += sVb[a * g1 + q] * P[q][bq][c];
+          T[a][bq][c] = acc;
+        }
+        for (int a = 0; a < g1; ++a) for (int bq = 0; bq < g1; ++bq) {
+          double B[3];
+          for (int c = 0; c < 3; ++c) {
+            double acc = 0.0;
+            for (int q = 0; q < g1; ++q) acc += sVb[bq * g1 + q] * T[a][q][c];
+            B[c] = acc;
+          }
+          const double d0 = B[0] - hm.bary[e * 3], d1 = B[1] - hm.bary[e * 3 + 1], d2 = B[2] - hm.bary[e * 3 + 2];
+          r = std::max(r, std::sqrt(d0 * d0 + d1 * d1 + d2 * d2));
+        }
+      }
+      hm.radius[e] = r;
+    }
+    // ElemEpsOneCell = 1 + sqrt(3 scaleJ RefMappingEps), scaleJ = max sJ / min sJ (BuildEpsOneCell)
+    hm.epsOneCell.assign(ne, 0.0);
+    for (int e = 0; e < ne; ++e) {
+      double mx = -1e300, mn = 1e300;
+      for (int p = 0; p < nDof; ++p) {
+        mx = std::max(mx, hm.sJ[size_t(e) * nDof + p]);
+        mn = std::min(mn, hm.sJ[size_t(e) * nDof + p]);
+      }
+      hm.epsOneCell[e] = 1.0 + std::sqrt(3.0 * (mx / mn) * par.refmapping_eps);
+    }
+    // background cells over the bounding spheres; cell size = the largest extent
+    double ext = 0.0;
+    for (int c = 0; c < 3; ++c) { hm.bmin[c] = 1e300; }
+    double bmax[3] = {-1e300, -1e300, -1e300};
+    for (int e = 0; e < ne; ++e)
+      for (int c = 0; c < 3; ++c) {
+        hm.bmin[c] = std::min(hm.bmin[c], hm.bary[e * 3 + c] - hm.radius[e]);
+        bmax[c] = std::max(bmax[c], hm.bary[e * 3 + c] + hm.radius[e]);
+        ext = std::max(ext, 2.0 * hm.radius[e]);
+      }
+    for (int c = 0; c < 3; ++c) {
+      const double L = bmax[c] - hm.bmin[c];
+      hm.nb[c] = std::max(1, int(std::ceil(L / ext)));
+      hm.bdelta[c] = L / hm.nb[c];
+      if (!(hm.bdelta[c] > 0)) hm.bdelta[c] = 1.0;
+    }
+    const size_t nbins = size_t(hm.nb[0]) * hm.nb[1] * hm.nb[2];
+    hm.binOff.assign(nbins + 1, 0);
+    for (int pass = 0; pass < 2; ++pass) {
+      std::vector<int32_t> fill(pass ? nbins : 0, 0);
+      for (int e = 0; e < ne; ++e) {
+        int lo3[3], hi3[3];
+        for (int c = 0; c < 3; ++c) {
+          lo3[c] = std::max(0, int(std::floor((hm.bary[e * 3 + c] - hm.radius[e] - hm.bmin[c]) / hm.bdelta[c])));
+          hi3[c] = std::min(hm.nb[c] - 1, int(std::floor((hm.bary[e * 3 + c] + hm.radius[e] - hm.bmin[c]) / hm.bdelta[c])));
+        }
+        for (int z = lo3[2]; z <= hi3[2]; ++z) for (int y = lo3[1]; y <= hi3[1]; ++y) for (int x = lo3[0]; x <= hi3[0]; ++x) {
+          const size_t b = (size_t(z) * hm.nb[1] + y) * hm.nb[0] + x;
+          if (!pass) hm.binOff[b + 1]++;
+          else hm.binElems[hm.binOff[b] + fill[b]++] = e;
+        }
+      }
+      if (!pass) {
+        for (size_t q = 0; q < nbins; ++q) hm.binOff[q + 1] += hm.binOff[q];
+        hm.binElems.resize(hm.binOff[nbins]);
+      }
+    }
+    for (int e = 0; e < ne; ++e)  // the domain box holds the curved elements
+      for (int c = 0; c < 3; ++c) {
+        hm.lo[c] = std::min(hm.lo[c], hm.bary[e * 3 + c] - hm.radius[e]);
+        hm.hi[c] = std::max(hm.hi[c], hm.bary[e * 3 + c] + hm.radius[e]);
+      }
+    for (size_t q = 0; q < nbins; ++q)
+      if (hm.binOff[q + 1] - hm.binOff[q] > kMaxBinElems)
+        return "RefMapping: a background cell holds more than " + std::to_string(kMaxBinElems) + " elements";
+  }
+
   // ---- cell_volweight_mean node classes ------------------------------------
   int maxid = 0;
   for (int i = 0; i < m.nNodes; ++i) maxid = std::max(maxid, m.GlobalNodeIDs[i]);
@@ -419,25 +601,34 @@ std::string build_host_mesh(const pic_mesh_desc &m, const pic_params &par, HostM
   if (par.deposition == PIC_DEPO_SF) {
     const double R = par.sf_radius;
     if (!(R > 0)) return "shape_function: radius must be positive";
-    if (hm.nPV != 0 && hm.nPV != 3) return "shape_function: 3D deposition needs 0 or 3 periodic vectors";
-    // shift table: the 27 periodic cases (pic_depo.f90:858-888), or identity only
-    int nsh = hm.nPV == 3 ? 27 : 1;
+    // shift table: the periodic cases (pic_depo.f90:858-888): every combination
+    // of -1/0/+1 times each periodic vector (27 for three vectors, 3^nPV for
+    // fewer, where the reference's fixed 3D table would use absent vectors)
+    int nsh = 1;
+    for (int v = 0; v < hm.nPV; ++v) nsh *= 3;
+    hm.nShift = nsh;
     for (int s = 0; s < 27; ++s) for (int c = 0; c < 3; ++c) hm.shift[s][c] = 0.0;
-    if (nsh == 27)
-      for (int s = 0; s < 27; ++s) {
-        int c0 = s / 9 - 1, c1 = (s / 3) % 3 - 1, c2 = s % 3 - 1;
-        for (int c = 0; c < 3; ++c) hm.shift[s][c] = c0 * hm.PV[0][c] + c1 * hm.PV[1][c] + c2 * hm.PV[2][c];
-      }
+    for (int s = 0; s < nsh && hm.nPV > 0; ++s) {
+      int cc = s, co[3] = {0, 0, 0};
+      for (int v = hm.nPV - 1; v >= 0; --v) { co[v] = cc % 3 - 1; cc /= 3; }
+      for (int c = 0; c < 3; ++c) hm.shift[s][c] = co[0] * hm.PV[0][c] + co[1] * hm.PV[1][c] + co[2] * hm.PV[2][c];
+    }
     // element bounding boxes (sources: all nodes; targets: Gauss points)
     std::vector<double> sbb(size_t(ne) * 6), tbb(size_t(ne) * 6);
     double ext = 0;
     for (int e = 0; e < ne; ++e) {
       const double *nc = &m.NodeCoords[size_t(EI(e, 4)) * 3];
       for (int c = 0; c < 3; ++c) { sbb[e * 6 + c] = 1e300; sbb[e * 6 + 3 + c] = -1e300; tbb[e * 6 + c] = 1e300; tbb[e * 6 + 3 + c] = -1e300; }
-      for (int q = 0; q < nGP; ++q)
+      if (m.NGeo == 1)
+        for (int q = 0; q < nGP; ++q)
+          for (int c = 0; c < 3; ++c) {
+            sbb[e * 6 + c] = std::min(sbb[e * 6 + c], nc[q * 3 + c]);
+            sbb[e * 6 + 3 + c] = std::max(sbb[e * 6 + 3 + c], nc[q * 3 + c]);
+          }
+      else  // a curved element lies inside its bounding sphere
         for (int c = 0; c < 3; ++c) {
-          sbb[e * 6 + c] = std::min(sbb[e * 6 + c], nc[q * 3 + c]);
-          sbb[e * 6 + 3 + c] = std::max(sbb[e * 6 + 3 + c], nc[q * 3 + c]);
+          sbb[e * 6 + c] = hm.bary[e * 3 + c] - hm.radius[e];
+          sbb[e * 6 + 3 + c] = hm.bary[e * 3 + c] + hm.radius[e];
         }
       for (int q = 0; q < nDof; ++q)
         for (int c = 0; c < 3; ++c) {

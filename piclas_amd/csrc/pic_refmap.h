@@ -1,0 +1,291 @@
+// Device routines of reference-mapping tracking (RefMapping) on linear and
+// curved (NGeo = 2) hexahedra.  Every routine restates one routine of the
+// reference (paths relative to its src/) with the same operation order.
+#pragma once
+
+#include "pic_device.h"
+
+namespace pic {
+
+// RefElemNewton on a map of G points per direction, mode 2 without
+// isSuccessful (interpolation/eval_xyz.f90:298-445): Newton with Armijo
+// back-tracking until |dxi|^2 <= RefMappingEps; leaves the loop when a
+// component of xi exceeds 1.5.
+template <int G>
+__device__ inline void newton_map(const double *X, const double *dX, const double *xiCL, const double *wb,
+                                  double refEps, const double xin[3], double xi[3]) {
+  double L[3][G], F[3], xo[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) lagrange<G>(xi[d], xiCL, wb, L[d]);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) F[c] = -xin[c];
+#pragma unroll
+  for (int k = 0; k < G; ++k)
+#pragma unroll
+    for (int j = 0; j < G; ++j) {
+      const double buff = L[1][j] * L[2][k];
+#pragma unroll
+      for (int i = 0; i < G; ++i)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) F[c] = F[c] + X[((k * G + j) * G + i) * 3 + c] * L[0][i] * buff;
+    }
+  double deltaXi2 = (fabs(F[0]) < kEpsMach && fabs(F[1]) < kEpsMach && fabs(F[2]) < kEpsMach) ? 0.0 : 1.0;
+  double NormF = F[0] * F[0] + F[1] * F[1] + F[2] * F[2];
+  double NormF_old;
+  int it = 0;
+  while (deltaXi2 > refEps && it < 100) {
+    ++it;
+    asm volatile("" ::: "memory");
+    double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+#pragma unroll 1
+    for (int k = 0; k < G; ++k)
+#pragma unroll
+      for (int j = 0; j < G; ++j) {
+        const double buff = L[1][j] * L[2][k];
+#pragma unroll
+        for (int i = 0; i < G; ++i) {
+          const double b2 = L[0][i] * buff;
+          const double *dp = dX + ((k * G + j) * G + i) * 9;
+#pragma unroll
+          for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int b = 0; b < 3; ++b) J[a][b] = J[a][b] + dp[a * 3 + b] * b2;
+        }
+      }
+    double sdet = get_det(J);
+    if (sdet > 0.) sdet = 1. / sdet;
+    double S[3][3];
+    S[0][0] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]) * sdet;
+    S[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * sdet;
+    S[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * sdet;
+    S[1][0] = (J[1][2] * J[2][0] - J[1][0] * J[2][2]) * sdet;
+    S[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * sdet;
+    S[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * sdet;
+    S[2][0] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) * sdet;
+    S[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * sdet;
+    S[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * sdet;
+    double dxi[3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) dxi[r] = S[r][0] * F[0] + S[r][1] * F[1] + S[r][2] * F[2];
+    deltaXi2 = dxi[0] * dxi[0] + dxi[1] * dxi[1] + dxi[2] * dxi[2];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) xo[d] = xi[d];
+    NormF_old = NormF;
+    NormF = NormF * 2.;
+    double lambda = 1.0;
+    int iArmijo = 1;
+    while (NormF > NormF_old * (1. - 0.0001 * lambda) && iArmijo <= 8) {
+      asm volatile("" ::: "memory");
+#pragma unroll
+      for (int d = 0; d < 3; ++d) xi[d] = xo[d] - lambda * dxi[d];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) lagrange<G>(xi[d], xiCL, wb, L[d]);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) F[c] = -xin[c];
+#pragma unroll 1
+      for (int k = 0; k < G; ++k)
+#pragma unroll
+        for (int j = 0; j < G; ++j) {
+          const double buff = L[1][j] * L[2][k];
+#pragma unroll
+          for (int i = 0; i < G; ++i) {
+            const double b2 = L[0][i] * buff;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) F[c] = F[c] + X[((k * G + j) * G + i) * 3 + c] * b2;
+          }
+        }
+      lambda = 0.2 * lambda;
+      iArmijo = iArmijo + 1;
+      NormF = F[0] * F[0] + F[1] * F[1] + F[2] * F[2];
+    }
+    if (fabs(xi[0]) > 1.5 || fabs(xi[1]) > 1.5 || fabs(xi[2]) > 1.5) break;
+  }
+}
+
+// GetRefNewtonStartValue guess 3: the closest CL point of XCL_NGeo
+// (interpolation/eval_xyz.f90:580-598); guesses 1, 2 and 4 as ref_newton_start
+template <int N, int G>
+__device__ inline void rm_start(const DevMesh &M, int e, const double x[3], double xi[3]) {
+  if (M.refGuess != 3) {
+    ref_newton_start<N>(M, e, x, xi);
+    return;
+  }
+  const double *X = M.XCL + (size_t)e * G * G * G * 3;
+  const double d0[3] = {x[0] - X[0], x[1] - X[1], x[2] - X[2]};
+  double win = sqrt(d0[0] * d0[0] + d0[1] * d0[1] + d0[2] * d0[2]);
+  xi[0] = xi[1] = xi[2] = M.xiCLg[0];
+#pragma unroll 1
+  for (int i = 0; i < G; ++i)
+#pragma unroll 1
+    for (int j = 0; j < G; ++j)
+#pragma unroll 1
+      for (int k = 0; k < G; ++k) {
+        const int r = (k * G + j) * G + i;
+        const double dX = fabs(x[0] - X[r * 3]);
+        if (dX > win) continue;
+        const double dY = fabs(x[1] - X[r * 3 + 1]);
+        if (dY > win) continue;
+        const double dZ = fabs(x[2] - X[r * 3 + 2]);
+        if (dZ > win) continue;
+        const double dist = sqrt(dX * dX + dY * dY + dZ * dZ);
+        if (dist < win) { win = dist; xi[0] = M.xiCLg[i]; xi[1] = M.xiCLg[j]; xi[2] = M.xiCLg[k]; }
+      }
+}
+
+// GetPositionInRefElem (interpolation/eval_xyz.f90:35-123): start value, then
+// Newton on the full NGeo map for curved elements and NGeo = 1 meshes, else on
+// the trilinear map of the 8 corner CL points.
+template <int N, int G>
+__device__ inline void rm_position(const DevMesh &M, int e, const double x[3], double xi[3]) {
+  rm_start<N, G>(M, e, x, xi);
+  if (G == 2 || M.curved[e])
+    newton_map<G>(M.XCL + (size_t)e * G * G * G * 3, M.dXCLg + (size_t)e * G * G * G * 9, M.xiCLg, M.wBaryCLg,
+                  M.refEps, x, xi);
+  else
+    newton_map<2>(M.corner + (size_t)e * 24, M.dXCL + (size_t)e * 72, M.xiCL, M.wBaryCL, M.refEps, x, xi);
+}
+
+__device__ inline double max_abs3(const double v[3]) { return fmax(fabs(v[0]), fmax(fabs(v[1]), fabs(v[2]))); }
+
+// PeriodicMovement with FastPeriodic = F (particles/tracking/particle_reftracking.f90:697-983)
+__device__ inline bool periodic_movement(const DevMesh &M, double x[3], double last[3]) {
+  bool moved = false;
+#pragma unroll 1
+  for (int dir = 0; dir < 3; ++dir) {
+    int v = -1;
+    for (int q = 0; q < M.nPV; ++q)
+      if (M.dirPV[q] == dir) { v = q; break; }
+    if (v < 0) continue;
+    if (x[dir] > M.gmax[dir]) {
+      const double sg = M.PV[v][dir] > 0 ? -1.0 : 1.0;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) { x[c] = x[c] + sg * M.PV[v][c]; last[c] = last[c] + sg * M.PV[v][c]; }
+      moved = true;
+    }
+    if (x[dir] < M.gmin[dir]) {
+      const double sg = M.PV[v][dir] > 0 ? 1.0 : -1.0;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) { x[c] = x[c] + sg * M.PV[v][c]; last[c] = last[c] + sg * M.PV[v][c]; }
+      moved = true;
+    }
+  }
+  return moved;
+}
+
+// Candidate order of ParticleRefTracking / SinglePointToElement: a stable
+// ascending sort of the background cell's elements by |x - bary|^2, without
+// the excluded ones.  Instead of sorting, each call returns the next entry
+// after (dPrev, qPrev) in that order (or -1).
+__device__ inline int next_candidate(const DevMesh &M, int b0, int n, const double x[3], int exclude, bool single,
+                                     double &dPrev, int &qPrev) {
+  double best = 0.0;
+  int bq = -1;
+  for (int q = 0; q < n; ++q) {
+    const int e = M.binElems[b0 + q];
+    double d2 = 0.0;
+    if (!single) {
+      if (e == exclude) continue;
+      const double *b = M.bary + (size_t)e * 3;
+      const double v0 = x[0] - b[0], v1 = x[1] - b[1], v2 = x[2] - b[2];
+      d2 = v0 * v0 + v1 * v1 + v2 * v2;
+      if (d2 > M.radius[e] * M.radius[e]) continue;
+    }
+    if (d2 < dPrev || (d2 == dPrev && q <= qPrev)) continue;
+    if (bq < 0 || d2 < best) { best = d2; bq = q; }
+  }
+  if (bq >= 0) { dPrev = best; qPrev = bq; }
+  return bq;
+}
+
+enum RmStatus { kRmInside = 0, kRmRemoved = 1, kRmNotFound = 2 };
+
+// ParticleRefTracking (particles/tracking/particle_reftracking.f90:30-414) for
+// CartesianPeriodic = T and open walls.  The reference sends particles of
+// boundary elements through ParticleBCTracking (trajectory against the
+// boundary faces); here a particle that no element holds after the move and
+// whose last element has an open side has left through that wall.  Both
+// decide alike unless a trajectory leaves and re-enters within one step.
+// Returns the new owner, or -1 with status kRmRemoved / kRmNotFound (where
+// the reference aborts).  xi: PartPosRef in and out.
+template <int N, int G>
+__device__ inline int rm_track(const DevMesh &M, int e0, double x[3], double last[3], double xi[3], int &status) {
+  status = kRmInside;
+  if (M.nPV > 0)
+    for (int g = 0; g < 4; ++g)
+      if (!periodic_movement(M, x, last)) break;
+  rm_position<N, G>(M, e0, x, xi);
+  if (max_abs3(xi) < 1.0) return e0;
+  // FIBGM cell (:180-186)
+  int b3[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    int v = int(floor((x[c] - M.bmin[c]) / M.bdelta[c]));
+    v = v < 0 ? 0 : v;
+    v = v + 1;
+    v = v > M.nb[c] ? M.nb[c] : v;
+    b3[c] = v - 1;
+  }
+  const int b = (b3[2] * M.nb[1] + b3[1]) * M.nb[0] + b3[0];
+  const int b0 = M.binOff[b], n = M.binOff[b + 1] - b0;
+  const double oldXi[3] = {xi[0], xi[1], xi[2]};
+  double newXi[3] = {HUGE_VAL, HUGE_VAL, HUGE_VAL};
+  int newElem = -1;
+  double dPrev = -HUGE_VAL;
+  int qPrev = -1;
+  int q;
+  while ((q = next_candidate(M, b0, n, x, e0, n == 1, dPrev, qPrev)) >= 0) {
+    const int e = M.binElems[b0 + q];
+    double t[3];
+    rm_position<N, G>(M, e, x, t);
+    if (max_abs3(t) < 1.0) {
+      xi[0] = t[0]; xi[1] = t[1]; xi[2] = t[2];
+      return e;
+    }
+    if (max_abs3(t) < max_abs3(newXi)) {
+      newXi[0] = t[0]; newXi[1] = t[1]; newXi[2] = t[2];
+      newElem = e;
+    }
+  }
+  if (M.openElem[e0]) {
+    status = kRmRemoved;
+    return -1;
+  }
+  int test;
+  if (max_abs3(oldXi) < max_abs3(newXi)) {
+    xi[0] = oldXi[0]; xi[1] = oldXi[1]; xi[2] = oldXi[2];
+    test = e0;
+  } else {
+    xi[0] = newXi[0]; xi[1] = newXi[1]; xi[2] = newXi[2];
+    test = newElem;
+  }
+  if (test < 0 || max_abs3(xi) > M.epsOneCell[test]) {
+    status = kRmNotFound;
+    return -1;
+  }
+  return test;
+}
+
+// SinglePointToElement for RefMapping (particles/tracking/particle_localization.f90:81-203)
+template <int N, int G>
+__device__ inline int rm_locate(const DevMesh &M, const double x[3], double xi[3]) {
+  int b3[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    int v = int(ceil((x[c] - M.bmin[c]) / M.bdelta[c]));
+    v = v > M.nb[c] ? M.nb[c] : v;
+    v = v < 1 ? 1 : v;
+    b3[c] = v - 1;
+  }
+  const int b = (b3[2] * M.nb[1] + b3[1]) * M.nb[0] + b3[0];
+  const int b0 = M.binOff[b], n = M.binOff[b + 1] - b0;
+  double dPrev = -HUGE_VAL;
+  int qPrev = -1, q;
+  while ((q = next_candidate(M, b0, n, x, -1, false, dPrev, qPrev)) >= 0) {
+    const int e = M.binElems[b0 + q];
+    rm_position<N, G>(M, e, x, xi);
+    if (max_abs3(xi) <= M.epsOneCell[e]) return e;
+  }
+  return -1;
+}
+
+}  // namespace pic

@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "pic_device.h"
+#include "pic_refmap.h"
 #include "stage_internal.h"
 
 #pragma clang fp contract(off)
@@ -230,6 +231,120 @@ __global__ __launch_bounds__(kBlock) void k_track_movers(DevMesh M, DevParts P, 
   }
 }
 
+// ---- RefMapping (ParticleRefTracking) kernels -------------------------------
+// G = NGeo + 1 geometry points per direction.  The field is evaluated at the
+// stored PartPosRef (GetInterpolatedFieldPartPos, pic_interpolation_tools.f90:242-244).
+template <int N>
+__device__ inline void field_at_ref(const DevMesh &M, int e, const double xi[3], double F[6]) {
+#pragma unroll
+  for (int c = 0; c < 6; ++c) F[c] = M.ext[c];
+  double U[3];
+  eval_field<N>(M, e, xi, U);
+  F[0] = F[0] + U[0];
+  F[1] = F[1] + U[1];
+  F[2] = F[2] + U[2];
+}
+
+__device__ inline void rm_removed(DevParts &P, int64_t i, const double x[3], const double v[3], int status,
+                                  unsigned long long *lostCount, uint8_t *insideById, double *stateById) {
+  const int id = P.id[i];
+  insideById[id] = 0;
+  double *st = stateById + (size_t)id * 6;
+  st[0] = x[0]; st[1] = x[1]; st[2] = x[2]; st[3] = v[0]; st[4] = v[1]; st[5] = v[2];
+  if (status == kRmNotFound) atomicAdd(lostCount, 1ull);
+}
+
+// fused step: interpolate at PartPosRef + push + ParticleRefTracking
+template <int N, int G>
+__global__ __launch_bounds__(kBlock) void k_rm_step(DevMesh M, DevParts P, int64_t n, double dt, int32_t *newElem,
+                                                    unsigned long long *lostCount, uint8_t *insideById,
+                                                    double *stateById) {
+  const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const int e0 = P.elem[i];
+  if (e0 < 0) { newElem[i] = -1; return; }
+  double x[3] = {P.x[i], P.y[i], P.z[i]};
+  double v[3] = {P.vx[i], P.vy[i], P.vz[i]};
+  double last[3] = {x[0], x[1], x[2]};
+  double xi[3] = {P.rx[i], P.ry[i], P.rz[i]};
+  const int sp = P.spec[i];
+  double F[6] = {0, 0, 0, 0, 0, 0};
+  if (fabs(M.charge[sp]) > 0.0) field_at_ref<N>(M, e0, xi, F);
+  boris_push(M.charge[sp], M.mass[sp], P.isNew[i] != 0, F, dt, x, v);
+  int status;
+  const int e = rm_track<N, G>(M, e0, x, last, xi, status);
+  P.x[i] = x[0]; P.y[i] = x[1]; P.z[i] = x[2];
+  P.vx[i] = v[0]; P.vy[i] = v[1]; P.vz[i] = v[2];
+  P.rx[i] = xi[0]; P.ry[i] = xi[1]; P.rz[i] = xi[2];
+  P.isNew[i] = 0;
+  newElem[i] = e;
+  if (e < 0) rm_removed(P, i, x, v, status, lostCount, insideById, stateById);
+}
+
+// staged InterpolateFieldToParticle with RefMapping
+template <int N>
+__global__ __launch_bounds__(kBlock) void k_rm_interp(DevMesh M, DevParts P, int64_t n) {
+  const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const int e = P.elem[i];
+  double F[6] = {0, 0, 0, 0, 0, 0};
+  const int sp = P.spec[i];
+  if (e >= 0 && fabs(M.charge[sp]) > 0.0) {
+    const double xi[3] = {P.rx[i], P.ry[i], P.rz[i]};
+    field_at_ref<N>(M, e, xi, F);
+  }
+#pragma unroll
+  for (int c = 0; c < 6; ++c) P.F[i * 6 + c] = F[c];
+}
+
+// staged PerformTracking with RefMapping
+template <int N, int G>
+__global__ __launch_bounds__(kBlock) void k_rm_track(DevMesh M, DevParts P, int64_t n, int32_t *newElem,
+                                                     unsigned long long *lostCount, uint8_t *insideById,
+                                                     double *stateById) {
+  const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const int e0 = P.lastElem[i];
+  if (e0 < 0) { newElem[i] = -1; return; }
+  double x[3] = {P.x[i], P.y[i], P.z[i]};
+  double last[3] = {P.lx[i], P.ly[i], P.lz[i]};
+  double xi[3] = {P.rx[i], P.ry[i], P.rz[i]};
+  int status;
+  const int e = rm_track<N, G>(M, e0, x, last, xi, status);
+  P.x[i] = x[0]; P.y[i] = x[1]; P.z[i] = x[2];
+  P.lx[i] = last[0]; P.ly[i] = last[1]; P.lz[i] = last[2];
+  P.rx[i] = xi[0]; P.ry[i] = xi[1]; P.rz[i] = xi[2];
+  newElem[i] = e;
+  P.elem[i] = e;
+  if (e < 0) {
+    const double v[3] = {P.vx[i], P.vy[i], P.vz[i]};
+    rm_removed(P, i, x, v, status, lostCount, insideById, stateById);
+  }
+}
+
+// PartPosRef of every particle in its element (LocateParticleInElement's
+// GetPositionInRefElem, particle_localization.f90:47-74)
+template <int N, int G>
+__global__ __launch_bounds__(kBlock) void k_rm_init(DevMesh M, DevParts P, int64_t n) {
+  const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const double x[3] = {P.x[i], P.y[i], P.z[i]};
+  double xi[3];
+  rm_position<N, G>(M, P.elem[i], x, xi);
+  P.rx[i] = xi[0]; P.ry[i] = xi[1]; P.rz[i] = xi[2];
+}
+
+// SinglePointToElement for particles handed over without an element
+template <int N, int G>
+__global__ __launch_bounds__(kBlock) void k_rm_locate(DevMesh M, int64_t n, const double *xyz, int32_t *elem) {
+  const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (i >= n) return;
+  if (elem[i] >= 0) return;
+  const double x[3] = {xyz[i * 3], xyz[i * 3 + 1], xyz[i * 3 + 2]};
+  double xi[3];
+  elem[i] = rm_locate<N, G>(M, x, xi);
+}
+
 // ---- sort by element --------------------------------------------------------
 // One wave per old element segment: stayers get their rank among the
 // stayers of the segment (order preserved); leavers take an arrival slot of
@@ -328,6 +443,7 @@ __global__ __launch_bounds__(kBlock) void k_gather_state(const int32_t *total, c
   d.vx[j] = s.vx[i]; d.vy[j] = s.vy[i]; d.vz[j] = s.vz[i];
   d.id[j] = s.id[i]; d.spec[j] = s.spec[i]; d.isNew[j] = s.isNew[i];
   d.elem[j] = newElem[i];
+  if (s.rx) { d.rx[j] = s.rx[i]; d.ry[j] = s.ry[i]; d.rz[j] = s.rz[i]; }
 }
 
 // ---- cell_volweight_mean ----------------------------------------------------
@@ -757,6 +873,24 @@ int alloc_parts(pic_handle *h, DevParts &P, int64_t cap) {
     default: break;                                                                                 \
   }
 
+#define LAUNCH_NG_N(KERNEL, N, G, GRID, ...)                                                                  \
+  switch (N) {                                                                                                 \
+    case 1: hipLaunchKernelGGL((KERNEL<1, G>), GRID, dim3(kBlock), 0, h->stream, __VA_ARGS__); break;           \
+    case 2: hipLaunchKernelGGL((KERNEL<2, G>), GRID, dim3(kBlock), 0, h->stream, __VA_ARGS__); break;           \
+    case 3: hipLaunchKernelGGL((KERNEL<3, G>), GRID, dim3(kBlock), 0, h->stream, __VA_ARGS__); break;           \
+    case 4: hipLaunchKernelGGL((KERNEL<4, G>), GRID, dim3(kBlock), 0, h->stream, __VA_ARGS__); break;           \
+    case 5: hipLaunchKernelGGL((KERNEL<5, G>), GRID, dim3(kBlock), 0, h->stream, __VA_ARGS__); break;           \
+    case 6: hipLaunchKernelGGL((KERNEL<6, G>), GRID, dim3(kBlock), 0, h->stream, __VA_ARGS__); break;           \
+    case 7: hipLaunchKernelGGL((KERNEL<7, G>), GRID, dim3(kBlock), 0, h->stream, __VA_ARGS__); break;           \
+    default: break;                                                                                            \
+  }
+// kernels templated on the DG degree and the geometry points per direction
+#define LAUNCH_NG(KERNEL, N, NGEO, GRID, ...)                   \
+  do {                                                          \
+    if ((NGEO) == 1) { LAUNCH_NG_N(KERNEL, N, 2, GRID, __VA_ARGS__) } \
+    else { LAUNCH_NG_N(KERNEL, N, 3, GRID, __VA_ARGS__) }       \
+  } while (0)
+
 float elapsed(hipEvent_t a, hipEvent_t b) {
   float ms = 0.f;
   (void)hipEventElapsedTime(&ms, a, b);
@@ -827,6 +961,7 @@ int run_sort(pic_handle *h, bool withStaged) {
   std::swap(h->P.vx, h->Q.vx); std::swap(h->P.vy, h->Q.vy); std::swap(h->P.vz, h->Q.vz);
   std::swap(h->P.elem, h->Q.elem); std::swap(h->P.id, h->Q.id);
   std::swap(h->P.spec, h->Q.spec); std::swap(h->P.isNew, h->Q.isNew);
+  std::swap(h->P.rx, h->Q.rx); std::swap(h->P.ry, h->Q.ry); std::swap(h->P.rz, h->Q.rz);
   std::swap(h->start, h->startNew);
   int32_t total_h = 0;
   HIPCHK(hipMemcpyAsync(&total_h, h->start + ne, sizeof(int32_t), hipMemcpyDeviceToHost, h->stream));
@@ -850,11 +985,15 @@ const char *pic_last_error(void) { return g_err.c_str(); }
 int pic_create(const pic_mesh_desc *mesh, const pic_params *par, pic_handle **out) {
   if (!mesh || !par || !out) return fail(PIC_EINVAL, "pic_create: null argument");
   *out = nullptr;
-  if (par->tracking != PIC_TRACK_TRIA) return fail(PIC_EINVAL, "pic_create: only TriaTracking is implemented");
+  if (par->tracking != PIC_TRACK_TRIA && par->tracking != PIC_TRACK_REFMAPPING)
+    return fail(PIC_EINVAL, "pic_create: tracking must be TriaTracking or RefMapping");
+  if (par->tracking == PIC_TRACK_REFMAPPING && par->deposition == PIC_DEPO_CVWM)  // pic_depo_method.f90:127-129
+    return fail(PIC_EINVAL, "pic_create: cell_volweight_mean needs TriaTracking");
   if (par->deposition < PIC_DEPO_NONE || par->deposition > PIC_DEPO_SF) return fail(PIC_EINVAL, "pic_create: deposition type");
   if (par->nSpecies < 1 || par->nSpecies > kMaxSpecies) return fail(PIC_EINVAL, "pic_create: nSpecies must be 1..8");
-  if (par->refmapping_guess != 1 && par->refmapping_guess != 2 && par->refmapping_guess != 4)
-    return fail(PIC_EINVAL, "pic_create: RefMappingGuess must be 1, 2 or 4 on linear meshes");
+  if (par->refmapping_guess < 1 || par->refmapping_guess > 4 ||
+      (par->tracking == PIC_TRACK_TRIA && par->refmapping_guess == 3))
+    return fail(PIC_EINVAL, "pic_create: RefMappingGuess must be 1..4 (1, 2 or 4 with TriaTracking)");
   if (par->deposition == PIC_DEPO_SF && par->sf_alpha < 2) return fail(PIC_EINVAL, "pic_create: PIC-shapefunction-alpha must be >= 2");
   if (par->max_particles < 1 || par->max_particles > (int64_t(1) << 31) - 1)
     return fail(PIC_EINVAL, "pic_create: max_particles must be in 1..2^31-1");
@@ -880,7 +1019,7 @@ int pic_create(const pic_mesh_desc *mesh, const pic_params *par, pic_handle **ou
   rc |= upload(h, const_cast<uint8_t **>(&M.sideBC), hm.sideBC);
   rc |= upload(h, const_cast<int8_t **>(&M.sidePV), hm.sidePV);
   rc |= upload(h, const_cast<uint8_t **>(&M.concave), hm.concave);
-  rc |= upload(h, const_cast<double **>(&M.dXCL), hm.dXCL);
+  rc |= upload(h, const_cast<double **>(&M.dXCL), hm.dXCL1);
   rc |= upload(h, const_cast<double **>(&M.bary), hm.bary);
   rc |= upload(h, const_cast<double **>(&M.radius), hm.radius);
   rc |= upload(h, const_cast<double **>(&M.xiBasis), hm.xiBasis);
@@ -888,7 +1027,24 @@ int pic_create(const pic_mesh_desc *mesh, const pic_params *par, pic_handle **ou
   rc |= upload(h, const_cast<double **>(&M.xGPe), hm.xGPe);
   rc |= dalloc(h, &h->E, size_t(hm.nElems) * hm.nDof * 3);
   M.E = h->E;
-  for (int i = 0; i < 2; ++i) { M.xiCL[i] = hm.xiCL[i]; M.wBaryCL[i] = hm.wBaryCL[i]; }
+  M.xiCL[0] = -1.0; M.xiCL[1] = 1.0;      // linear CL basis of the corner map
+  M.wBaryCL[0] = -0.5; M.wBaryCL[1] = 0.5;
+  M.NGeo = hm.NGeo; M.nGP = hm.nGP; M.tracking = par->tracking; M.nPV = hm.nPV;
+  for (int i = 0; i <= hm.NGeo; ++i) { M.xiCLg[i] = hm.xiCL[i]; M.wBaryCLg[i] = hm.wBaryCL[i]; }
+  for (int c = 0; c < 3; ++c) {
+    M.gmin[c] = hm.gmin[c]; M.gmax[c] = hm.gmax[c]; M.dirPV[c] = hm.dirPV[c];
+    M.bmin[c] = hm.bmin[c]; M.bdelta[c] = hm.bdelta[c]; M.nb[c] = hm.nb[c];
+  }
+  if (par->tracking == PIC_TRACK_REFMAPPING) {
+    if (hm.NGeo == 1) M.XCL = M.corner;
+    else rc |= upload(h, const_cast<double **>(&M.XCL), hm.XCL);
+    rc |= upload(h, const_cast<double **>(&M.dXCLg), hm.dXCL);
+    rc |= upload(h, const_cast<uint8_t **>(&M.curved), hm.curved);
+    rc |= upload(h, const_cast<double **>(&M.epsOneCell), hm.epsOneCell);
+    rc |= upload(h, const_cast<uint8_t **>(&M.openElem), hm.openElem);
+    rc |= upload(h, const_cast<int32_t **>(&M.binOff), hm.binOff);
+    rc |= upload(h, const_cast<int32_t **>(&M.binElems), hm.binElems);
+  }
   for (int i = 0; i <= hm.N; ++i) { M.xGP[i] = hm.xGP[i]; M.wBaryGP[i] = hm.wBaryGP[i]; }
   for (int v = 0; v < 3; ++v)
     for (int c = 0; c < 3; ++c) M.PV[v][c] = hm.PV[v][c];
@@ -932,6 +1088,10 @@ int pic_create(const pic_mesh_desc *mesh, const pic_params *par, pic_handle **ou
   rc |= dalloc(h, &h->Q.lx, cap); rc |= dalloc(h, &h->Q.ly, cap); rc |= dalloc(h, &h->Q.lz, cap);
   rc |= dalloc(h, &h->P.lastElem, cap); rc |= dalloc(h, &h->Q.lastElem, cap);
   rc |= dalloc(h, &h->P.F, cap * 6); rc |= dalloc(h, &h->Q.F, cap * 6);
+  if (par->tracking == PIC_TRACK_REFMAPPING) {
+    rc |= dalloc(h, &h->P.rx, cap); rc |= dalloc(h, &h->P.ry, cap); rc |= dalloc(h, &h->P.rz, cap);
+    rc |= dalloc(h, &h->Q.rx, cap); rc |= dalloc(h, &h->Q.ry, cap); rc |= dalloc(h, &h->Q.rz, cap);
+  }
   rc |= dalloc(h, &h->start, hm.nElems + 1);
   rc |= dalloc(h, &h->startNew, hm.nElems + 1);
   rc |= dalloc(h, &h->newElem, cap);
@@ -1025,15 +1185,35 @@ int pic_set_particles(pic_handle *h, int64_t n, const double *PartState, const i
   const int ne = h->hm.nElems;
   std::vector<int32_t> el(n), sp(n, 0);
   std::vector<uint8_t> ins(n, 1);
+  const bool refmap = h->par.tracking == PIC_TRACK_REFMAPPING;
+  std::vector<int64_t> pending;
   for (int64_t i = 0; i < n; ++i) {
     int e = elem ? elem[i] : 0;
     if (e < 0 || e > ne) return fail(PIC_EINVAL, "pic_set_particles: element id out of range");
-    if (e == 0) e = locate_host(h->hm, &PartState[i * 6]) + 1;
+    if (e == 0) {
+      if (refmap) pending.push_back(i);
+      else e = locate_host(h->hm, &PartState[i * 6]) + 1;
+    }
     el[i] = e - 1;
-    if (el[i] < 0) ins[i] = 0;
     sp[i] = species ? species[i] - 1 : 0;
     if (sp[i] < 0 || sp[i] >= h->par.nSpecies) return fail(PIC_EINVAL, "pic_set_particles: species out of range");
   }
+  if (!pending.empty()) {
+    // SinglePointToElement on the device for the particles without an element
+    const int64_t np = int64_t(pending.size());
+    std::vector<double> xyz(np * 3);
+    std::vector<int32_t> pe(np, -1);
+    for (int64_t k = 0; k < np; ++k)
+      for (int c = 0; c < 3; ++c) xyz[k * 3 + c] = PartState[pending[k] * 6 + c];
+    HIPCHK(hipMemcpy(h->mv.last, xyz.data(), np * 3 * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->newElem, pe.data(), np * sizeof(int32_t), hipMemcpyHostToDevice));
+    LAUNCH_NG(k_rm_locate, h->hm.N, h->hm.NGeo, dim3(grid_for(np)), h->M, np, h->mv.last, h->newElem);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpy(pe.data(), h->newElem, np * sizeof(int32_t), hipMemcpyDeviceToHost));
+    for (int64_t k = 0; k < np; ++k) el[pending[k]] = pe[k];
+  }
+  for (int64_t i = 0; i < n; ++i)
+    if (el[i] < 0) ins[i] = 0;
   // counting sort by element, stable in caller order
   std::vector<int64_t> cnt(ne + 1, 0);
   for (int64_t i = 0; i < n; ++i)
@@ -1074,6 +1254,11 @@ int pic_set_particles(pic_handle *h, int64_t n, const double *PartState, const i
   h->n = nin;
   h->nTotal = n;
   h->stagedValid = false;
+  if (refmap && nin > 0) {
+    LAUNCH_NG(k_rm_init, h->hm.N, h->hm.NGeo, dim3(grid_for(nin)), h->M, h->P, nin);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipDeviceSynchronize());
+  }
   return PIC_OK;
 }
 
@@ -1095,7 +1280,10 @@ int pic_interpolate_field_to_particle(pic_handle *h) {
   if (!h) return fail(PIC_EINVAL, "pic_interpolate_field_to_particle: null handle");
   if (int rc = require_particles(h)) return rc;
   if (set_device(h)) return PIC_EHIP;
-  if (h->n > 0) { LAUNCH_N(k_interp, h->hm.N, dim3(grid_for(h->n)), h->M, h->P, h->n); }
+  if (h->n > 0) {
+    if (h->par.tracking == PIC_TRACK_REFMAPPING) { LAUNCH_N(k_rm_interp, h->hm.N, dim3(grid_for(h->n)), h->M, h->P, h->n); }
+    else { LAUNCH_N(k_interp, h->hm.N, dim3(grid_for(h->n)), h->M, h->P, h->n); }
+  }
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(h->stream));
   h->stagedValid = true;
@@ -1119,8 +1307,13 @@ int pic_perform_tracking(pic_handle *h) {
   if (!h->stagedValid) return fail(PIC_ESTATE, "pic_perform_tracking: call pic_push_boris_leapfrog first");
   if (set_device(h)) return PIC_EHIP;
   if (h->n > 0) {
-    hipLaunchKernelGGL(k_track, dim3(grid_for(h->n)), dim3(kBlock), 0, h->stream, h->M, h->P, h->n, h->newElem,
-                       h->lostCount, h->insideById, h->stateById);
+    if (h->par.tracking == PIC_TRACK_REFMAPPING) {
+      LAUNCH_NG(k_rm_track, h->hm.N, h->hm.NGeo, dim3(grid_for(h->n)), h->M, h->P, h->n, h->newElem, h->lostCount,
+                h->insideById, h->stateById);
+    } else {
+      hipLaunchKernelGGL(k_track, dim3(grid_for(h->n)), dim3(kBlock), 0, h->stream, h->M, h->P, h->n, h->newElem,
+                         h->lostCount, h->insideById, h->stateById);
+    }
     HIPCHK(hipGetLastError());
     if (int rc = run_sort(h, true)) return rc;
   }
@@ -1137,7 +1330,10 @@ int pic_time_step(pic_handle *h, double dt, int32_t nsteps) {
     if (int rc = run_deposition(h, true)) return rc;
     // LastPartPos <- PartState, interpolate, push, track (one kernel)
     HIPCHK(hipEventRecord(h->ev[3], h->stream));
-    if (h->n > 0) {
+    if (h->n > 0 && h->par.tracking == PIC_TRACK_REFMAPPING) {
+      LAUNCH_NG(k_rm_step, h->hm.N, h->hm.NGeo, dim3(grid_for(h->n)), h->M, h->P, h->n, dt, h->newElem, h->lostCount,
+                h->insideById, h->stateById);
+    } else if (h->n > 0) {
       HIPCHK(hipMemsetAsync(h->mv.count, 0, sizeof(int32_t), h->stream));
       LAUNCH_N(k_push_inside, h->hm.N, dim3(grid_for(h->n)), h->M, h->P, h->n, dt, h->newElem, h->mv);
       hipLaunchKernelGGL(k_track_movers, dim3(1024), dim3(kBlock), 0, h->stream, h->M, h->P, h->newElem, h->mv,
@@ -1209,6 +1405,23 @@ int pic_get_last_part_pos(pic_handle *h, double *L) {
   for (int c = 0; c < 3; ++c) {
     HIPCHK(hipMemcpy(a.data(), src[c], n * sizeof(double), hipMemcpyDeviceToHost));
     for (int64_t j = 0; j < n; ++j) L[int64_t(id[j]) * 3 + c] = a[j];
+  }
+  return PIC_OK;
+}
+
+int pic_get_part_pos_ref(pic_handle *h, double *ref) {
+  if (!h || !ref) return fail(PIC_EINVAL, "pic_get_part_pos_ref: null argument");
+  if (h->par.tracking != PIC_TRACK_REFMAPPING) return fail(PIC_ESTATE, "pic_get_part_pos_ref: RefMapping only");
+  if (set_device(h)) return PIC_EHIP;
+  HIPCHK(hipStreamSynchronize(h->stream));
+  const int64_t n = h->n;
+  std::vector<int32_t> id(n);
+  std::vector<double> a(n);
+  HIPCHK(hipMemcpy(id.data(), h->P.id, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+  double *src[3] = {h->P.rx, h->P.ry, h->P.rz};
+  for (int c = 0; c < 3; ++c) {
+    HIPCHK(hipMemcpy(a.data(), src[c], n * sizeof(double), hipMemcpyDeviceToHost));
+    for (int64_t j = 0; j < n; ++j) ref[int64_t(id[j]) * 3 + c] = a[j];
   }
   return PIC_OK;
 }

@@ -17,8 +17,9 @@
 namespace pic {
 
 constexpr int kMaxN = 7;          // DG degree supported by the kernels
-constexpr int kMaxNGeo = 1;       // geometry degree supported by the tracking kernels
+constexpr int kMaxNGeo = 2;       // geometry degree (TriaTracking: 1; RefMapping: 1 or 2)
 constexpr int kMaxSpecies = 8;
+constexpr int kMaxBinElems = 256;  // elements per RefMapping background cell
 
 // Host-side mesh tables: the reference's particle-mesh arrays, derived from
 // the HOPR arrays exactly as its init routines derive them.
@@ -42,6 +43,18 @@ struct HostMesh {
   std::vector<double> slen;        // [e][6]
   std::vector<double> xGPe;        // [e][nDof][3] Elem_xGP
   std::vector<double> sJ;          // [e][nDof]
+  // RefMapping (particle_mesh_build.f90, particle_mesh_tools.f90:976-996)
+  std::vector<uint8_t> curved;     // [e] ElemCurved
+  std::vector<double> dXCL1;       // [e][8][3][3] dXCL_NGeo at the corner CL points
+  std::vector<double> epsOneCell;  // [e] ElemEpsOneCell
+  std::vector<uint8_t> openElem;   // [e] element has an open particle boundary side
+  double gmin[3] = {0}, gmax[3] = {0};  // GEO%x/y/zminglob, maxglob
+  int dirPV[3] = {-1, -1, -1};     // axis of each periodic vector
+  // background cells for the RefMapping candidate search: elements whose
+  // bounding sphere (bary, radius) overlaps the cell
+  double bmin[3] = {0}, bdelta[3] = {1, 1, 1};
+  int nb[3] = {1, 1, 1};
+  std::vector<int32_t> binOff, binElems;
   // 1D bases
   double xiCL[8] = {0}, wBaryCL[8] = {0};
   double xGP[kMaxN + 1] = {0}, wGP[kMaxN + 1] = {0}, wBaryGP[kMaxN + 1] = {0};
@@ -50,6 +63,7 @@ struct HostMesh {
   std::vector<int32_t> classOff;   // [nClasses+1] CSR over (e*8+corner) incidences
   std::vector<int32_t> classInc;
   std::vector<double> classVolume; // NodeVolume of the class
+  int nShift = 1;                  // shape-function periodic cases (1, 3, 9 or 27)
   // shape function: target element -> (source element, periodic shift) list
   std::vector<int32_t> sfOff;      // [nElems+1]
   std::vector<int32_t> sfSrc;      // source element

@@ -16,7 +16,7 @@ USE ISO_C_BINDING
 IMPLICIT NONE
 PRIVATE
 
-INTEGER(C_INT),PARAMETER,PUBLIC :: PIC_TRACK_TRIA=3, PIC_DEPO_NONE=0, PIC_DEPO_CVWM=1, PIC_DEPO_SF=2
+INTEGER(C_INT),PARAMETER,PUBLIC :: PIC_TRACK_REFMAPPING=1, PIC_TRACK_TRIA=3, PIC_DEPO_NONE=0, PIC_DEPO_CVWM=1, PIC_DEPO_SF=2
 INTEGER(C_INT),PARAMETER,PUBLIC :: PIC_BC_OPEN=1, PIC_BC_REFLECTIVE=2, PIC_BC_PERIODIC=3
 
 TYPE, BIND(C), PUBLIC :: tPicMeshDesc
@@ -108,6 +108,11 @@ INTERFACE
     TYPE(C_PTR),VALUE          :: h
     REAL(C_DOUBLE),INTENT(OUT) :: F(*)
   END FUNCTION pic_get_field_at_particle
+  INTEGER(C_INT) FUNCTION pic_get_part_pos_ref(h, PartPosRef) BIND(C, NAME='pic_get_part_pos_ref')
+    IMPORT :: C_INT, C_PTR, C_DOUBLE
+    TYPE(C_PTR),VALUE          :: h
+    REAL(C_DOUBLE),INTENT(OUT) :: PartPosRef(*)
+  END FUNCTION pic_get_part_pos_ref
   INTEGER(C_INT) FUNCTION pic_deposited_charge(h, PartCharge) BIND(C, NAME='pic_deposited_charge')
     IMPORT :: C_INT, C_PTR, C_DOUBLE
     TYPE(C_PTR),VALUE          :: h
@@ -128,7 +133,8 @@ TYPE(C_PTR),SAVE :: stage = C_NULL_PTR
 
 PUBLIC :: InitPICStage, FinalizePICStage, PICStage_SetField, PICStage_SetParticles, PICStage_GetParticles
 PUBLIC :: Deposition, InterpolateFieldToParticle, PushBorisLeapfrog, PerformTracking, TimeStepPoissonByBorisLeapfrog
-PUBLIC :: PICStage_GetPartSource, PICStage_GetFieldAtParticle, CalcDepositedCharge, CalcPotentialEnergy
+PUBLIC :: PICStage_GetPartSource, PICStage_GetFieldAtParticle, PICStage_GetPartPosRef
+PUBLIC :: CalcDepositedCharge, CalcPotentialEnergy
 
 CONTAINS
 
@@ -208,6 +214,12 @@ SUBROUTINE PICStage_GetFieldAtParticle(FieldAtParticle)
 REAL(C_DOUBLE),INTENT(OUT) :: FieldAtParticle(:,:)
 CALL Check(pic_get_field_at_particle(stage, FieldAtParticle), 'pic_get_field_at_particle')
 END SUBROUTINE PICStage_GetFieldAtParticle
+
+!> PartPosRef(1:3,nPart) of RefMapping (src/particles/particle_vars.f90:52)
+SUBROUTINE PICStage_GetPartPosRef(PartPosRef)
+REAL(C_DOUBLE),INTENT(OUT) :: PartPosRef(:,:)
+CALL Check(pic_get_part_pos_ref(stage, PartPosRef), 'pic_get_part_pos_ref')
+END SUBROUTINE PICStage_GetPartPosRef
 
 !> PartCharge(1:3) of src/particles/pic/analyze/pic_analyze.f90:136-210: deposited charge, absolute and relative error
 SUBROUTINE CalcDepositedCharge(PartCharge)

@@ -45,11 +45,11 @@ READ(10) PartState, GlobalElemID, PartSpecies
 CLOSE(10)
 IsNewPart = .FALSE.
 
-par%tracking = PIC_TRACK_TRIA
+par%tracking = MERGE(PIC_TRACK_REFMAPPING, PIC_TRACK_TRIA, hdr(14).EQ.PIC_TRACK_REFMAPPING)
 par%deposition = hdr(11)
 par%sf_alpha = hdr(12)
 par%sf_radius = rpar(1)
-par%refmapping_guess = 1
+par%refmapping_guess = MAX(hdr(15), 1)
 par%refmapping_eps = rpar(2)
 par%external_field = rpar(3:8)
 par%nSpecies = nSpecies

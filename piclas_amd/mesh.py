@@ -216,6 +216,121 @@ def _side_corner_grid(ijk, ls, n, periodic):
     return out
 
 
+def annulus(shape, r0=0.5, r1=1.0, length=1.0, NGeo=2, wall_bc=PART_BC_OPEN):
+    """Annular cylinder r0 <= r <= r1, 0 <= z <= length of nr*nt*nz curved
+    hexahedra (NGeo >= 1) in HOPR layout.
+
+    Element (i, j, k) spans [r0 + i dr, r0 + (i+1) dr] x [j dt, (j+1) dt] x
+    [k dz, (k+1) dz] in (r, theta, z) and is numbered i + nr*(j + nt*k);
+    xi, eta, zeta run along r, theta, z, so every Jacobian is positive.  Its
+    (NGeo+1)^3 nodes are the images of the equidistant reference points,
+    stored in tensor order (i fastest) as HOPR stores curved elements; nodes
+    shared by two elements are computed once, so their coordinates are equal
+    bit for bit.  The mesh closes on itself in theta (conforming interior
+    faces), z is periodic (vector (0,0,length); lower face alpha=+1, upper
+    face alpha=-1) and the inner / outer cylinder walls carry wall_bc.
+    """
+    nr, nt, nz = shape
+    n = np.array(shape)
+    nE = nr * nt * nz
+    g = NGeo
+    n1 = g + 1
+    # fine node grid: radius, angle, height of every geometry node
+    R = r0 + (r1 - r0) * np.arange(g * nr + 1) / (g * nr)
+    R[-1] = r1
+    T = 2.0 * np.pi * np.arange(g * nt) / (g * nt)
+    Z = length * np.arange(g * nz + 1) / (g * nz)
+    Z[-1] = length
+    cosT, sinT = np.cos(T), np.sin(T)
+    e = np.arange(nE)
+    ii, jj, kk = e % nr, (e // nr) % nt, e // (nr * nt)
+
+    ElemInfo = np.zeros((nE, 6), np.int32)
+    ElemInfo[:, 0] = 208 if g > 1 else 108
+    ElemInfo[:, 1] = 1
+    ElemInfo[:, 2] = 6 * e
+    ElemInfo[:, 3] = 6 * e + 6
+    ElemInfo[:, 4] = n1 ** 3 * e
+    ElemInfo[:, 5] = n1 ** 3 * (e + 1)
+
+    NodeCoords = np.zeros((nE, n1 ** 3, 3))
+    GlobalNodeIDs = np.zeros((nE, n1 ** 3), np.int32)
+    for c in range(n1):
+        for b in range(n1):
+            for a in range(n1):
+                q = (c * n1 + b) * n1 + a
+                I = g * ii + a
+                J = (g * jj + b) % (g * nt)
+                K = g * kk + c
+                NodeCoords[:, q, 0] = R[I] * cosT[J]
+                NodeCoords[:, q, 1] = R[I] * sinT[J]
+                NodeCoords[:, q, 2] = Z[K]
+                GlobalNodeIDs[:, q] = I + (g * nr + 1) * (J + g * nt * K) + 1
+
+    BCType = [(2 if wall_bc != PART_BC_REFLECTIVE else 4, 0, 0, 0)] * 2 + [(1, 0, 0, +1), (1, 0, 0, -1)]
+    PartBC = [wall_bc, wall_bc, PART_BC_PERIODIC, PART_BC_PERIODIC]
+    names = ["BC_inner", "BC_outer", "BC_z+", "BC_z-"]
+    bc_of = {(0, -1): 1, (0, +1): 2, (2, -1): 3, (2, +1): 4}
+    wrap = (False, True, True)
+    SideInfo = _structured_sides(np.stack([ii, jj, kk], 1), n, wrap, bc_of, 24 if g > 1 else 4)
+    pv = np.array([[0.0, 0.0, length]])
+    return HoprMesh(ElemInfo=ElemInfo, SideInfo=SideInfo, NodeCoords=NodeCoords.reshape(-1, 3),
+                    GlobalNodeIDs=GlobalNodeIDs.reshape(-1), BCType=np.array(BCType, np.int32),
+                    PartBoundCond=np.array(PartBC, np.int32), PeriodicVectors=pv, NGeo=g, BCNames=names)
+
+
+def _structured_sides(ijk, n, wrap, bc_of, side_type):
+    """SideInfo [nE*6][5] of a structured hexahedral grid numbered i + n0*(j + n1*k).
+
+    wrap[a]: the grid closes on itself along axis a (through a periodic BC
+    when bc_of has entries for that axis, else through interior faces).
+    Master side of an interior face: the element with the smaller id; of a
+    periodic face: the element on the upper boundary."""
+    nE = ijk.shape[0]
+    nx, ny = n[0], n[1]
+    me = np.arange(nE)
+    SideInfo = np.zeros((nE, 6, 5), np.int32)
+    SideInfo[:, :, 0] = side_type
+    opposite = {0: 5, 5: 0, 1: 3, 3: 1, 2: 4, 4: 2}
+
+    def nb_of(a, sgn):
+        t = ijk.copy()
+        t[:, a] += sgn
+        t[:, a] %= n[a]
+        return t[:, 0] + nx * (t[:, 1] + ny * t[:, 2])
+
+    for ls in range(6):
+        a, sgn = SIDE_DIR[ls]
+        nb = nb_of(a, sgn)
+        on_edge = ((ijk[:, a] == 0) & (sgn < 0)) | ((ijk[:, a] == n[a] - 1) & (sgn > 0))
+        bcid = bc_of.get((a, sgn), 0)
+        on_bc = on_edge & (bcid > 0)
+        SideInfo[:, ls, 4] = np.where(on_bc, bcid, 0)
+        connected = ~on_edge | bool(wrap[a])
+        SideInfo[:, ls, 2] = np.where(connected, nb + 1, 0)
+        master = np.where(on_bc, sgn > 0, me < nb)
+        master = np.where(connected, master, True)
+        SideInfo[:, ls, 1] = np.where(master, 1, -1)
+        own = _side_corner_grid(ijk, ls, n, wrap)
+        other = _side_corner_grid(np.stack(_unravel(nb, nx, ny), 1), opposite[ls], n, wrap)
+        first = other[:, 0, :]
+        pos = np.full(nE, -1)
+        for q in range(4):
+            pos = np.where((pos < 0) & np.all(own[:, q, :] == first, axis=1), q, pos)
+        assert np.all(pos[connected] >= 0)
+        SideInfo[:, ls, 3] = np.where(connected, 10 * (opposite[ls] + 1) + pos + 1, 0)
+    is_master = SideInfo[:, :, 1] > 0
+    ids = np.zeros((nE, 6), np.int64)
+    ids[is_master] = np.arange(1, is_master.sum() + 1)
+    for ls in range(6):
+        a, sgn = SIDE_DIR[ls]
+        nb = nb_of(a, sgn)
+        slave = ~is_master[:, ls]
+        ids[slave, ls] = -ids[nb[slave], opposite[ls]]
+    SideInfo[:, :, 1] = ids
+    return SideInfo.reshape(nE * 6, 5)
+
+
 def from_hopr_arrays(ElemInfo, SideInfo, NodeCoords, GlobalNodeIDs, BCType, part_bc_by_name, BCNames,
                      PeriodicVectors, NGeo=1):
     """Mesh from arrays read out of a HOPR file; particle BCs are given per BC name."""

@@ -73,6 +73,7 @@ EXPORTS = {
     "pic_get_particles": (ctypes.c_int, [_H, _f64p, _i32p, _u8p]),
     "pic_get_last_part_pos": (ctypes.c_int, [_H, _f64p]),
     "pic_get_field_at_particle": (ctypes.c_int, [_H, _f64p]),
+    "pic_get_part_pos_ref": (ctypes.c_int, [_H, _f64p]),
     "pic_get_part_source": (ctypes.c_int, [_H, _f64p]),
     "pic_part_source_device": (ctypes.c_int, [_H, ctypes.POINTER(ctypes.c_void_p)]),
     "pic_deposited_charge": (ctypes.c_int, [_H, _f64p]),
@@ -215,6 +216,12 @@ class PICStage:
     def field_at_particle(self):
         a = np.zeros((self.nPart, 6))
         _chk(load_library().pic_get_field_at_particle(self.h, _p(a, _f64p)), "pic_get_field_at_particle")
+        return a
+
+    def part_pos_ref(self):
+        """PartPosRef (RefMapping): reference coordinates in the owner element."""
+        a = np.zeros((self.nPart, 3))
+        _chk(load_library().pic_get_part_pos_ref(self.h, _p(a, _f64p)), "pic_get_part_pos_ref")
         return a
 
     def part_source(self):

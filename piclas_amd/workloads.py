@@ -9,6 +9,13 @@ N=3, 200 electrons/cell uniform, Maxwellian 2 eV, frozen smooth E (8 random
 Fourier modes, |E| <= 1e3 V/m) at the Gauss points, uniform B = 0.01 T as
 external field, shape_function R = 1.5h, alpha = 2, dt with omega_c dt = 0.2.
 
+Uneven occupancy on curved elements: annular cylinder r in [0.5, 1] m,
+length 1 m, of curved (NGeo = 2) hexahedra, N = 3, RefMapping tracking;
+electrons (5 eV) and Ar+ ions (39.95 amu, 0.05 eV) with radial density
+exp(-(r-0.5)/0.1), so elements hold from a few to several hundred particles;
+frozen smooth E as above, B = 0.01 T along z, shape_function deposition, open
+walls, periodic in z.
+
 Every array is a pure function of (configuration, seed); the generator is
 numpy's PCG64.
 """
@@ -17,7 +24,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import mesh as M
-from .stage import ELECTRON_MASS, ELEMENTARY_CHARGE, EPS0, DEPO_CVWM, DEPO_SF
+from .stage import ELECTRON_MASS, ELEMENTARY_CHARGE, EPS0, DEPO_CVWM, DEPO_SF, TRACK_TRIA, TRACK_REFMAPPING
 
 KB_EV = ELEMENTARY_CHARGE   # 1 eV in J
 
@@ -40,6 +47,8 @@ class Workload:
     charge: tuple
     mass: tuple
     mpf: tuple
+    tracking: int = TRACK_TRIA
+    refmapping_guess: int = 1
 
 
 def gauss_nodes(N):
@@ -125,6 +134,55 @@ def thermal_cube(nx=48, ppc=200, seed=20261204, deposition=DEPO_SF, steps=100):
                     (-ELEMENTARY_CHARGE,), (ELECTRON_MASS,), (mpf,))
 
 
+def curved_dof_positions(mesh, N):
+    """Physical Gauss-point positions [nElems][(N+1)^3][3] of a mesh whose
+    element nodes are equidistant in reference space (tensor order)."""
+    g = mesh.NGeo
+    nodes = mesh.NodeCoords.reshape(mesh.nElems, g + 1, g + 1, g + 1, 3)   # [e][k][j][i][c]
+    xe = np.linspace(-1.0, 1.0, g + 1)
+    xg = gauss_nodes(N)
+    L = np.ones((N + 1, g + 1))
+    for a in range(g + 1):
+        for b in range(g + 1):
+            if a != b:
+                L[:, a] *= (xg - xe[b]) / (xe[a] - xe[b])
+    x = np.einsum("ri,sj,tk,ekjic->etsrc", L, L, L, nodes)      # [e][t=k][s=j][r=i][c]
+    return x.reshape(mesh.nElems, (N + 1) ** 3, 3)
+
+
+AMU = 1.66053906660e-27
+
+
+def annulus_plasma(shape=(24, 96, 24), n_total=30_000_000, seed=20261205, sf_radius=0.03, steps=100, dt=2e-10,
+                   NGeo=2, deposition=DEPO_SF):
+    mesh = M.annulus(shape, 0.5, 1.0, 1.0, NGeo=NGeo)
+    N = 3
+    rng = np.random.Generator(np.random.PCG64(seed))
+    # r with density exp(-(r-0.5)/0.1) in the annulus (pdf ~ r exp(..)), by a tabulated inverse CDF
+    rr = np.linspace(0.5, 1.0, 200001)
+    pdf = rr * np.exp(-(rr - 0.5) / 0.1)
+    cdf = np.concatenate([[0.0], np.cumsum(0.5 * (pdf[1:] + pdf[:-1]) * np.diff(rr))])
+    cdf /= cdf[-1]
+    r = np.interp(rng.uniform(0.0, 1.0, n_total), cdf, rr)
+    th = rng.uniform(0.0, 2.0 * np.pi, n_total)
+    z = rng.uniform(0.0, 1.0, n_total)
+    pos = np.stack([r * np.cos(th), r * np.sin(th), z], 1)
+    ne = n_total // 2
+    species = np.ones(n_total, np.int32)
+    species[ne:] = 2
+    m_ar = 39.95 * AMU
+    vel = np.concatenate([maxwellian(rng, ne, 5.0, ELECTRON_MASS), maxwellian(rng, n_total - ne, 0.05, m_ar)])
+    vol = np.pi * (1.0 - 0.25) * 1.0
+    mpf = 1e12 * vol / n_total
+    xg = curved_dof_positions(mesh, N)
+    E = fourier_field(xg.reshape(-1, 3), (2.0, 2.0, 1.0), 8, 1e3, rng).reshape(xg.shape)
+    return Workload("annulus%dx%dx%d_NGeo%d_N3_refmap" % (shape + (NGeo,)), mesh, N, np.hstack([pos, vel]),
+                    np.zeros(n_total, np.int32), species, E, (0, 0, 0, 0, 0, 0.01), deposition, sf_radius, 2, dt,
+                    steps, (-ELEMENTARY_CHARGE, ELEMENTARY_CHARGE), (ELECTRON_MASS, m_ar), (mpf, mpf),
+                    tracking=TRACK_REFMAPPING, refmapping_guess=2 if NGeo > 1 else 1)
+
+
 def stage_kwargs(w: Workload):
     return dict(deposition=w.deposition, sf_radius=w.sf_radius, sf_alpha=w.sf_alpha,
-                external_field=w.external_field, charge=w.charge, mass=w.mass, mpf=w.mpf)
+                external_field=w.external_field, charge=w.charge, mass=w.mass, mpf=w.mpf,
+                tracking=w.tracking, refmapping_guess=w.refmapping_guess)

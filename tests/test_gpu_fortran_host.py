@@ -18,8 +18,9 @@ def write_case(path, w, nsteps, mode, deposition):
     m = w.mesh
     n = len(w.PartState)
     hdr = np.zeros(16, np.int32)
-    hdr[:13] = [m.nElems, m.SideInfo.shape[0], m.NodeCoords.shape[0], m.NGeo, w.N, m.BCType.shape[0],
-                m.PeriodicVectors.shape[0], n, nsteps, mode, deposition, w.sf_alpha, len(w.charge)]
+    hdr[:15] = [m.nElems, m.SideInfo.shape[0], m.NodeCoords.shape[0], m.NGeo, w.N, m.BCType.shape[0],
+                m.PeriodicVectors.shape[0], n, nsteps, mode, deposition, w.sf_alpha, len(w.charge), w.tracking,
+                w.refmapping_guess]
     with open(path, "wb") as f:
         f.write(hdr.tobytes())
         for a, t in ((m.ElemInfo, np.int32), (m.SideInfo, np.int32), (m.NodeCoords, np.float64),
@@ -77,4 +78,32 @@ def test_fortran_host_matches_oracle(oracle_mod, tmp_path, mode, depo):
     assert abs(part_charge[1] - abs(qp - part_charge[0])) <= 1e-12 * abs(qp)
     if depo == stage.DEPO_CVWM:   # node-volume weighting conserves charge to round-off
         assert part_charge[2] <= 1e-12
+    assert abs(wel - o.field_energy()) <= 1e-12 * o.field_energy()
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_fortran_host_refmapping_annulus(oracle_mod, tmp_path, mode):
+    """The Fortran host on curved (NGeo=2) elements with RefMapping; the library
+    places the particles (no element handed over)."""
+    w = W.annulus_plasma(shape=(3, 12, 3), n_total=2000, dt=2e-9)
+    case, res = str(tmp_path / "case.bin"), str(tmp_path / "res.bin")
+    write_case(case, w, 5, mode, w.deposition)
+    print("case fortran host refmapping mode=%d" % mode, flush=True)
+    out = subprocess.run([DRIVER, case, res], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    ps, el, ins, PS, part_charge, wel = read_result(res, len(w.PartState), w.mesh.nElems, (w.N + 1) ** 3)
+    o = oracle_mod.Oracle(w.mesh, w.N, tracking=w.tracking, refmapping_guess=w.refmapping_guess,
+                          deposition=w.deposition, sf_radius=w.sf_radius, sf_alpha=w.sf_alpha,
+                          external_field=w.external_field, charge=w.charge, mass=w.mass, mpf=w.mpf)
+    o.set_field(w.E)
+    o.set_particles(w.PartState, elem=w.elem, species=w.species)
+    assert o.locate() == 0
+    o.time_step(w.dt, 5)
+    o.deposition()
+    ops, oel, oins = o.particles()
+    assert np.array_equal(ins, oins) and np.array_equal(el[ins], oel[ins])
+    assert np.abs(ps[ins, :3] - ops[ins, :3]).max() <= 1e-12
+    assert np.abs(ps[ins, 3:] - ops[ins, 3:]).max() <= 1e-12 * np.abs(ops[ins, 3:]).max()
+    oPS = o.part_source()
+    assert np.abs(PS - oPS).max() <= 1e-12 * np.abs(oPS).max()
     assert abs(wel - o.field_energy()) <= 1e-12 * o.field_energy()

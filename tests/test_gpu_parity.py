@@ -31,9 +31,12 @@ def run_pair(O, w, steps, deposition=None, staged=False, **over):
     s.set_field(w.E)
     s.set_particles(w.PartState, elem=w.elem, species=w.species)
     o = O.Oracle(w.mesh, w.N, deposition=kw["deposition"], sf_radius=kw["sf_radius"], sf_alpha=kw["sf_alpha"],
-                 external_field=kw["external_field"], charge=kw["charge"], mass=kw["mass"], mpf=kw["mpf"])
+                 external_field=kw["external_field"], charge=kw["charge"], mass=kw["mass"], mpf=kw["mpf"],
+                 tracking=kw["tracking"], refmapping_guess=kw["refmapping_guess"])
     o.set_field(w.E)
     o.set_particles(w.PartState, elem=w.elem, species=w.species)
+    if (np.asarray(w.elem) == 0).any():
+        o.locate()
     if staged:
         for _ in range(steps):
             s.deposition(); o.deposition()
@@ -229,3 +232,35 @@ def test_thermal_cube48_full_size_properties():
         e2 = (w.E ** 2).sum(-1)
         ref = 0.5 * stage.EPS0 * math.fsum((wijk[None, :] * J * e2).reshape(-1))
         assert abs(s.field_energy() - ref) <= 1e-12 * ref
+
+
+@pytest.mark.parametrize("staged", [False, True])
+def test_refmapping_annulus_parity(oracle_mod, staged):
+    """RefMapping on curved (NGeo=2) hexahedra: device localization, 20 steps
+    of interpolation at PartPosRef, push, ParticleRefTracking with periodic z
+    and open walls, and shape-function deposition, against the oracle."""
+    w = W.annulus_plasma(shape=(4, 16, 4), n_total=4000, dt=2e-9)
+    s, o = run_pair(oracle_mod, w, 20, staged=staged)
+    ps, el, ins = s.particles()
+    ops, oel, oins = o.particles()
+    assert np.array_equal(ins, oins), "inside flags differ: %d" % (ins != oins).sum()
+    assert np.array_equal(el[ins], oel[ins]), "owners differ: %d" % (el[ins] != oel[ins]).sum()
+    assert (~ins).sum() > 0          # some particles left through the walls
+    assert s.stats()["n_lost"] == 0 and o.lost() == 0
+    assert np.abs(ps[ins, :3] - ops[ins, :3]).max() <= 1e-12
+    vmax = np.abs(ops[ins, 3:]).max()
+    assert np.abs(ps[ins, 3:] - ops[ins, 3:]).max() <= 1e-12 * vmax
+    assert np.abs(s.part_pos_ref()[ins] - o.part_pos_ref()[ins]).max() <= 1e-12
+    PS, oPS = s.part_source(), o.part_source()
+    assert np.abs(PS - oPS).max() <= 1e-12 * np.abs(oPS).max()
+    s.close()
+
+
+def test_refmapping_linear_box_matches_triatracking(oracle_mod):
+    """On a linear periodic box RefMapping and TriaTracking own every particle
+    alike (away from faces), both against the oracle."""
+    w = W.thermal_cube(nx=5, ppc=30, steps=5)
+    w.tracking, w.refmapping_guess = stage.TRACK_REFMAPPING, 1
+    s, o = run_pair(oracle_mod, w, 5)
+    compare(s, o, 1.0, mesh=w.mesh)
+    s.close()

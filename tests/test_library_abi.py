@@ -48,6 +48,8 @@ def _create(**over):
     (dict(tracking=2), "TriaTracking"),
     (dict(charge=tuple([1.0] * 9), mass=tuple([1.0] * 9), mpf=tuple([1.0] * 9)), None),
     (dict(refmapping_guess=3), "RefMappingGuess"),
+    (dict(refmapping_guess=5, tracking=stage.TRACK_REFMAPPING), "RefMappingGuess"),
+    (dict(tracking=stage.TRACK_REFMAPPING, deposition=stage.DEPO_CVWM), "cell_volweight_mean needs TriaTracking"),
     (dict(deposition=stage.DEPO_SF, sf_radius=0.1, sf_alpha=1), "alpha"),
     (dict(max_particles=0), "max_particles"),
     (dict(precision=stage.PREC_MIXED), "precision"),
@@ -61,6 +63,12 @@ def test_create_rejects_bad_parameters(bad, msg):
     with pytest.raises(stage.StageError) as ei:
         _create(**bad)
     assert msg in str(ei.value)
+
+
+def test_create_rejects_curved_triatracking():
+    m = M.annulus((2, 4, 2))
+    with pytest.raises(stage.StageError, match="NGeo"):
+        stage.PICStage(m, 3, tracking=stage.TRACK_TRIA)
 
 
 def test_create_rejects_reflective_and_mortars():

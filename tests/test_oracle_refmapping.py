@@ -1,0 +1,200 @@
+"""Closed-form checks of the oracle's reference mapping and RefMapping
+tracking on curved (NGeo = 2) hexahedra of an annular cylinder.
+
+The annulus elements have planar faces at constant angle and height, so the
+owner of a point is known in closed form from (theta, z) away from those
+faces; the cylinder faces are quadratic patches through the exact circle
+points, so the radial index is checked only away from them.  The mapping
+itself is pinned by an independent numpy evaluation of the element map
+(Lagrange interpolation of the equidistant nodes) and of its Jacobian.
+"""
+import numpy as np
+import pytest
+
+from piclas_amd import mesh as M
+from piclas_amd import workloads as W
+
+SHAPE = (3, 12, 3)
+
+
+def lagrange_eq(x, g):
+    """Lagrange basis on g+1 equidistant nodes of [-1,1] and its derivative, at points x."""
+    p = np.linspace(-1.0, 1.0, g + 1)
+    L = np.ones((len(x), g + 1))
+    dL = np.zeros((len(x), g + 1))
+    for a in range(g + 1):
+        for b in range(g + 1):
+            if a == b:
+                continue
+            L[:, a] *= (x - p[b]) / (p[a] - p[b])
+        for c in range(g + 1):
+            if c == a:
+                continue
+            term = np.ones(len(x)) / (p[a] - p[c])
+            for b in range(g + 1):
+                if b != a and b != c:
+                    term *= (x - p[b]) / (p[a] - p[b])
+            dL[:, a] += term
+    return L, dL
+
+
+def element_map(mesh, el, xi):
+    """x(xi) and dx/dxi of elements el (0-based) at reference points xi."""
+    g = mesh.NGeo
+    nodes = mesh.NodeCoords.reshape(mesh.nElems, g + 1, g + 1, g + 1, 3)[el]   # [n][k][j][i][c]
+    (L0, D0), (L1, D1), (L2, D2) = (lagrange_eq(xi[:, d], g) for d in range(3))
+    x = np.einsum("ni,nj,nk,nkjic->nc", L0, L1, L2, nodes)
+    J = np.stack([np.einsum("ni,nj,nk,nkjic->nc", D0, L1, L2, nodes),
+                  np.einsum("ni,nj,nk,nkjic->nc", L0, D1, L2, nodes),
+                  np.einsum("ni,nj,nk,nkjic->nc", L0, L1, D2, nodes)], 2)          # J[n][c][d]
+    return x, J
+
+
+def make(O, mesh, **kw):
+    return O.Oracle(mesh, 3, tracking=O.TRACK_REFMAPPING, refmapping_guess=kw.pop("guess", 2), **kw)
+
+
+def polar_owner(mesh_shape, x):
+    nr, nt, nz = mesh_shape
+    r = np.hypot(x[:, 0], x[:, 1])
+    th = np.mod(np.arctan2(x[:, 1], x[:, 0]), 2 * np.pi)
+    i = np.clip(np.floor((r - 0.5) / 0.5 * nr), 0, nr - 1).astype(int)
+    j = np.clip(np.floor(th / (2 * np.pi) * nt), 0, nt - 1).astype(int)
+    k = np.clip(np.floor(x[:, 2] * nz), 0, nz - 1).astype(int)
+    return i, j, k, r, th
+
+
+def test_annulus_mesh_is_conforming():
+    """Every side and its neighbour's side share their corner coordinates (flip included)."""
+    m = M.annulus(SHAPE)
+    ne = m.nElems
+    nodes = m.NodeCoords.reshape(ne, 27, 3)
+    side = m.SideInfo.reshape(ne, 6, 5)
+    cg = [i + 3 * j + 9 * k for (i, j, k) in [(0, 0, 0), (2, 0, 0), (2, 2, 0), (0, 2, 0), (0, 0, 2), (2, 0, 2), (2, 2, 2),
+                                              (0, 2, 2)]]
+    for e in range(ne):
+        for ls in range(6):
+            nb = side[e, ls, 2]
+            if nb == 0:
+                continue
+            mine = {tuple(np.round(nodes[e, cg[c - 1]], 12)) for c in M.SIDE_CORNERS[ls]}
+            ols = side[e, ls, 3] // 10 - 1
+            theirs = {tuple(np.round(nodes[nb - 1, cg[c - 1]], 12)) for c in M.SIDE_CORNERS[ols]}
+            if side[e, ls, 4] == 0:
+                assert mine == theirs
+            else:   # periodic in z: the same face shifted by the periodic vector
+                sh = {(a, b, round(c + (1.0 if ls == 0 else -1.0), 12)) for (a, b, c) in mine}
+                assert sh == theirs
+
+
+def test_curved_flags_and_jacobian(oracle_mod):
+    m = M.annulus(SHAPE)
+    o = make(oracle_mod, m)
+    curved, eps = o.refmapping_tables()
+    assert curved.all()
+    # sJ at the Gauss points against the numpy Jacobian of the element map
+    xg, _ = np.polynomial.legendre.leggauss(4)
+    k, j, i = np.meshgrid(xg, xg, xg, indexing="ij")
+    ref = np.stack([i.ravel(), j.ravel(), k.ravel()], 1)
+    sJ = o.sj()
+    for e in (0, 5, m.nElems - 1):
+        _, J = element_map(m, np.full(64, e), ref)
+        assert np.allclose(1.0 / np.linalg.det(J), sJ[e], rtol=1e-12, atol=0)
+    # ElemEpsOneCell = 1 + sqrt(3 * max(sJ)/min(sJ) * RefMappingEps) > 1
+    assert (eps > 1.0).all() and (eps < 1.1).all()
+
+
+def test_bary_is_center_and_radius_bounds_element(oracle_mod):
+    m = M.annulus(SHAPE)
+    o = make(oracle_mod, m)
+    bary, rad = o.elem_bary()
+    e = np.arange(m.nElems)
+    x0, _ = element_map(m, e, np.zeros((m.nElems, 3)))
+    assert np.allclose(bary, x0, atol=1e-14)
+    # sample points on the faces: all within the radius, and the radius is not loose
+    rng = np.random.default_rng(3)
+    worst = np.zeros(m.nElems)
+    for _ in range(40):
+        xi = rng.uniform(-1, 1, (m.nElems, 3))
+        ax = rng.integers(0, 3, m.nElems)
+        xi[np.arange(m.nElems), ax] = rng.choice([-1.0, 1.0], m.nElems)
+        x, _ = element_map(m, e, xi)
+        worst = np.maximum(worst, np.linalg.norm(x - bary, axis=1))
+    assert (worst <= rad * (1 + 1e-12)).all()
+    assert (rad <= 1.3 * worst).all()
+
+
+def test_newton_inverts_curved_map(oracle_mod):
+    m = M.annulus(SHAPE)
+    rng = np.random.default_rng(7)
+    n = 3000
+    el = rng.integers(0, m.nElems, n)
+    xi = rng.uniform(-0.999, 0.999, (n, 3))
+    x, _ = element_map(m, el, xi)
+    for guess in (1, 2, 3, 4):
+        o = make(oracle_mod, m, guess=guess, refmapping_eps=1e-24)
+        xr, ok = o.position_in_ref_elem(x, el + 1)
+        assert ok.all()
+        assert np.abs(xr - xi).max() < 1e-10, guess
+
+
+def test_locate_and_track_ownership(oracle_mod):
+    """Localization and 30 RefMapping steps: every particle inside is in the
+    element that holds it (max|xi| < 1 there) and, away from element faces, in
+    the closed-form (r, theta, z) cell; z stays in [0, 1] (periodic)."""
+    w = W.annulus_plasma(shape=SHAPE, n_total=3000, dt=2e-9, deposition=0)
+    o = oracle_mod.Oracle(w.mesh, w.N, tracking=w.tracking, refmapping_guess=w.refmapping_guess,
+                          external_field=w.external_field, charge=w.charge, mass=w.mass, mpf=w.mpf)
+    o.set_field(w.E)
+    o.set_particles(w.PartState, elem=w.elem, species=w.species)
+    assert o.locate() == 0
+    nr, nt, nz = SHAPE
+    for steps in (0, 30):
+        if steps:
+            o.time_step(w.dt, steps)
+        ps, el, ins = o.particles()
+        x = ps[ins, :3]
+        e = el[ins] - 1
+        xi, ok = o.position_in_ref_elem(x, e + 1)
+        mx = np.abs(xi).max(1)
+        if steps == 0:   # SinglePointToElement accepts max|xi| <= ElemEpsOneCell
+            assert (mx <= o.refmapping_tables()[1][e]).all()
+        else:            # ParticleRefTracking keeps the element that holds the particle
+            assert mx.max() < 1.0
+        assert np.allclose(o.part_pos_ref()[ins], xi, atol=1e-9)
+        i, j, k, r, th = polar_owner(SHAPE, x)
+        away = (mx < 1.0) & (np.abs(np.mod(th / (2 * np.pi) * nt + 0.5, 1.0) - 0.5) > 1e-9) & \
+               (np.abs(np.mod(x[:, 2] * nz + 0.5, 1.0) - 0.5) > 1e-9) & \
+               (np.abs(np.mod((r - 0.5) / 0.5 * nr + 0.5, 1.0) - 0.5) > 1e-3)
+        ei, ej, ek = e % nr, (e // nr) % nt, e // (nr * nt)
+        assert (ei[away] == i[away]).all() and (ej[away] == j[away]).all() and (ek[away] == k[away]).all()
+        assert (x[:, 2] >= 0).all() and (x[:, 2] <= 1).all()
+    assert o.lost() == 0
+
+
+def test_open_wall_removes_and_periodic_z_wraps(oracle_mod):
+    m = M.annulus(SHAPE)
+    o = oracle_mod.Oracle(m, 3, tracking=oracle_mod.TRACK_REFMAPPING, refmapping_guess=2,
+                          charge=(0.0,), mass=(1.0,), mpf=(1.0,))
+    o.set_field(np.zeros((m.nElems, 64, 3)))
+    # 0: radially outward through r = 1; 1: inward through r = 0.5; 2: +z through z = 1; 3: -z through z = 0
+    ps = np.array([[0.97, 0.01, 0.5, 1.0, 0.0, 0.0],
+                   [0.52, 0.02, 0.5, -1.0, 0.0, 0.0],
+                   [0.0, 0.75, 0.98, 0.0, 0.0, 1.0],
+                   [-0.8, 0.0, 0.01, 0.0, 0.0, -1.0]])
+    o.set_particles(ps, elem=np.zeros(4, np.int32))
+    assert o.locate() == 0
+    o.time_step(0.05, 1)
+    out, el, ins = o.particles()
+    assert list(ins) == [False, False, True, True]
+    assert np.isclose(out[2, 2], 0.03, atol=1e-12) and np.isclose(out[3, 2], 0.96, atol=1e-12)
+    i, j, k, _, _ = polar_owner(SHAPE, out[2:, :3])
+    nr, nt, _ = SHAPE
+    assert list(el[2:] - 1) == list(i + nr * (j + nt * k))
+    assert o.lost() == 0
+
+
+def test_cvwm_with_refmapping_is_rejected(oracle_mod):
+    m = M.annulus(SHAPE)
+    with pytest.raises(RuntimeError, match="TriaTracking"):
+        oracle_mod.Oracle(m, 3, tracking=oracle_mod.TRACK_REFMAPPING, deposition=oracle_mod.DEPO_CVWM)
