@@ -82,6 +82,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-fraction", type=float, default=1.0 / 16)
     ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--workload", choices=["cube", "annulus"], default="cube",
+                    help="cube: 48^3 linear box (default); annulus: 24x96x24 curved NGeo=2, RefMapping, 30M particles")
+    ap.add_argument("--particles", type=int, default=30_000_000, help="annulus workload only")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; multi-GPU runs are not part of it (got --gpus %d)" % args.gpus,
@@ -89,6 +92,8 @@ def main():
         sys.exit(2)
 
     from piclas_amd import stage, workloads as W
+    if args.workload == "annulus":
+        return bench_annulus(args)
     w = W.thermal_cube(nx=args.nx, ppc=args.ppc)
     n = len(w.PartState)
     print("bench: %s, %d particles, %d warmup + %d timed steps" % (w.name, n, args.warmup, args.steps),
@@ -138,6 +143,46 @@ def main():
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(w, args.cpu_fraction, args.cpu_steps, threads)
+    print(json.dumps(out), flush=True)
+
+
+def bench_annulus(args):
+    """Uneven occupancy on curved elements: 24x96x24 NGeo=2 annulus, RefMapping,
+    electrons and Ar+ ions (2 to ~800 per element), shape-function deposition."""
+    from piclas_amd import stage, workloads as W
+    w = W.annulus_plasma(n_total=args.particles)
+    n = len(w.PartState)
+    print("bench: %s, %d particles, %d warmup + %d timed steps" % (w.name, n, args.warmup, args.steps),
+          file=sys.stderr, flush=True)
+    s = stage.PICStage(w.mesh, w.N, max_particles=n, **W.stage_kwargs(w))
+    s.set_field(w.E)
+    t0 = time.perf_counter()
+    s.set_particles(w.PartState, elem=w.elem, species=w.species)
+    s.synchronize()
+    t_locate = time.perf_counter() - t0
+    n_in = s.stats()["n_particles"]
+    if args.warmup:
+        s.time_step(w.dt, args.warmup)
+    s.synchronize()
+    s.reset_stats()
+    t0 = time.perf_counter()
+    s.time_step(w.dt, args.steps)
+    s.synchronize()
+    t = time.perf_counter() - t0
+    st = s.stats()
+    s.close()
+    ms_step = t / args.steps * 1e3
+    out = {"metric": METRIC, "value": n_in * args.steps / t, "unit": "particle-steps/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": w.name, "elements": w.mesh.nElems, "NGeo": w.mesh.NGeo, "N": w.N,
+                      "particles": n, "particles_placed": n_in, "deposition": "shape_function R=0.03 m alpha=2",
+                      "tracking": "refmapping", "parallelism": "1 GPU"},
+           "setup_s_locate_and_sort": round(t_locate, 3),
+           "stage_ms_per_step": {"deposit": st["ms_deposit"] / args.steps,
+                                 "interp_push_track": st["ms_push_track"] / args.steps,
+                                 "sort": st["ms_sort"] / args.steps},
+           "n_removed_or_lost": {"lost": st["n_lost"], "inside_after": st["n_particles"]}}
     print(json.dumps(out), flush=True)
 
 

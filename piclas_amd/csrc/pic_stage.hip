@@ -12,6 +12,7 @@
 #include <atomic>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <numeric>
@@ -710,6 +711,116 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
     }
 }
 
+// Shape-function gather for N = 3 (64 DOFs) with the accumulation on the
+// matrix cores.  PartSource(dof, c) += sum_p S(dof, p)^alpha * f_p (v_p, 1)_c
+// is, per group of 4 staged particles, a 64x4x4 product; v_mfma_f64_4x4x4
+// (4 blocks of 4x4x4) does a 16-DOF quarter of it, and runs beside the VALU
+// that forms S.  Lane layout of that instruction (A: lane 16k+4b+i holds
+// A_b[i][k]; B: lane 16k+4b+j holds B_b[k][j]; D: lane 16i+4b+j holds
+// D_b[i][j]) puts particle k = lane/16 and DOF 16Q + lane%16 on each lane for
+// the operands, and DOF 16Q + 4((lane/4)%4) + lane/16, component lane%4 on
+// the accumulator.
+template <int ALPHA>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE_MX))) void k_sf_gather_mfma(DevMesh M, DevParts P, const int32_t *start, SFArgs A, double *PS) {
+#pragma clang fp contract(fast)
+  constexpr int nd = 64;
+  __shared__ double lds[kBlock / kWave][kWave + 4][8];
+  const int wib = threadIdx.x / kWave;
+  const int lane = lane_id();
+  const int lb = xcd_block(blockIdx.x, gridDim.x);
+  const int t = lb * (kBlock / kWave) + wib;
+  if (t >= M.nElems) return;
+  const double bx = M.bary[t * 3], by = M.bary[t * 3 + 1], bz = M.bary[t * 3 + 2];
+  const double R2i0 = A.r2_sf_inv;
+  // operand-side DOF constants, one DOF per quarter:  S = cd - qq + m.q
+  double mx[4], my[4], mz[4], cd[4], acc[4];
+  double blo[3] = {1e300, 1e300, 1e300}, bhi[3] = {-1e300, -1e300, -1e300};
+#pragma unroll
+  for (int Q = 0; Q < 4; ++Q) {
+    const int d = 16 * Q + (lane & 15);
+    const double *xg = M.xGPe + ((size_t)t * nd + d) * 3;
+    const double dx = xg[0] - bx, dy = xg[1] - by, dz = xg[2] - bz;
+    mx[Q] = 2.0 * R2i0 * dx; my[Q] = 2.0 * R2i0 * dy; mz[Q] = 2.0 * R2i0 * dz;
+    cd[Q] = 1.0 - R2i0 * (dx * dx + dy * dy + dz * dz);
+    blo[0] = fmin(blo[0], dx); blo[1] = fmin(blo[1], dy); blo[2] = fmin(blo[2], dz);
+    bhi[0] = fmax(bhi[0], dx); bhi[1] = fmax(bhi[1], dy); bhi[2] = fmax(bhi[2], dz);
+    acc[Q] = 0.0;
+  }
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1)
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      blo[a] = fmin(blo[a], __shfl_xor(blo[a], off, kWave));
+      bhi[a] = fmax(bhi[a], __shfl_xor(bhi[a], off, kWave));
+    }
+  const double thr = A.r_sf + M.radius[t];
+  const double R2box = A.r2_sf * (1.0 + 1e-12);
+  double (*L)[8] = lds[wib];
+  const int kp = lane >> 4, jc = lane & 3;
+  for (int q = A.off[t]; q < A.off[t + 1]; ++q) {
+    const int s = A.src[q];
+    const int si = A.shift[q];
+    const double shx = A.sh[si][0], shy = A.sh[si][1], shz = A.sh[si][2];
+    const int s0 = start[s], s1 = start[s + 1];
+    for (int base = s0; base < s1; base += kWave) {
+      const int i = base + lane;
+      bool pass = false;
+      double px = 0, py = 0, pz = 0, f = 0, vx = 0, vy = 0, vz = 0;
+      if (i < s1) {
+        const int sp = P.spec[i];
+        const double q0 = M.charge[sp];
+        px = (P.x[i] + shx) - bx; py = (P.y[i] + shy) - by; pz = (P.z[i] + shz) - bz;
+        const double gx = fmax(0.0, fmax(blo[0] - px, px - bhi[0]));
+        const double gy = fmax(0.0, fmax(blo[1] - py, py - bhi[1]));
+        const double gz = fmax(0.0, fmax(blo[2] - pz, pz - bhi[2]));
+        pass = (fabs(q0) > 0.0) && (gx * gx + gy * gy + gz * gz <= R2box) &&
+               (A.linear || !(sqrt(px * px + py * py + pz * pz) > thr));
+        f = q0 * M.mpf[sp] * A.w_sf;
+        vx = P.vx[i]; vy = P.vy[i]; vz = P.vz[i];
+      }
+      const unsigned long long m = __ballot(pass);
+      const int cnt = __popcll(m);
+      if (cnt == 0) continue;
+      if (pass) {
+        const int slot = __popcll(m & ((1ull << lane) - 1ull));
+        L[slot][0] = px; L[slot][1] = py; L[slot][2] = pz;
+        L[slot][3] = R2i0 * (px * px + py * py + pz * pz);
+        L[slot][4] = vx * f; L[slot][5] = vy * f; L[slot][6] = vz * f; L[slot][7] = f;
+      }
+      // pad the last group of 4 with weightless records
+      const int ng = (cnt + 3) >> 2;
+      if (lane >= cnt && lane < 4 * ng) {
+#pragma unroll
+        for (int c = 0; c < 8; ++c) L[lane][c] = 0.0;
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      for (int g = 0; g < ng; ++g) {
+        const double *rec = L[4 * g + kp];
+        const double qx = rec[0], qy = rec[1], qz = rec[2], qq = rec[3];
+        const double bval = rec[4 + jc];
+#pragma unroll
+        for (int Q = 0; Q < 4; ++Q) {
+          const double S = fmax(mx[Q] * qx + (my[Q] * qy + (mz[Q] * qz + (cd[Q] - qq))), 0.0);
+          double S1 = S * S;
+          if (ALPHA == 0) {
+            for (int ex = 3; ex <= A.alpha; ++ex) S1 = S * S1;
+          } else {
+#pragma unroll
+            for (int ex = 3; ex <= ALPHA; ++ex) S1 = S * S1;
+          }
+          acc[Q] = __builtin_amdgcn_mfma_f64_4x4x4f64(S1, bval, acc[Q], 0, 0, 0);
+        }
+      }
+      __builtin_amdgcn_wave_barrier();
+    }
+  }
+  const int dofLo = 4 * ((lane >> 2) & 3) + (lane >> 4);
+#pragma unroll
+  for (int Q = 0; Q < 4; ++Q) PS[((size_t)t * nd + 16 * Q + dofLo) * 4 + jc] = acc[Q];
+}
+
 // ---- analysis integrals ------------------------------------------------------
 // Per element, in the reference's i-fastest DOF order:
 //   mode 0  sum_ijk (w_i w_j w_k * rho) * (1/sJ)       CalcDepositedCharge, pic_analyze.f90:172-179
@@ -813,6 +924,7 @@ struct pic_handle {
   double *sJ = nullptr, *elemPart = nullptr, *sumOut = nullptr;
   unsigned long long *specCnt = nullptr;
   SFArgs sf{};
+  bool sfMatrix = true;   // N = 3 shape function: accumulate on the matrix cores
   Movers mv{};
   bool stagedValid = false;   // LastPartPos/FieldAtParticle refer to the current order
   std::vector<std::pair<int64_t, std::vector<double>>> unplaced;  // particles no element holds at set-up
@@ -918,7 +1030,12 @@ int run_deposition(pic_handle *h, bool timed) {
   } else {
     const int waves = kBlock / kWave;
     const unsigned nblk = xcd_grid((ne + waves - 1) / waves);
-    if (h->sf.alpha == 2) {
+    if (N == 3 && h->sfMatrix) {
+      if (h->sf.alpha == 2)
+        hipLaunchKernelGGL((k_sf_gather_mfma<2>), dim3(nblk), dim3(kBlock), 0, h->stream, h->M, h->P, h->start, h->sf, h->PS);
+      else
+        hipLaunchKernelGGL((k_sf_gather_mfma<0>), dim3(nblk), dim3(kBlock), 0, h->stream, h->M, h->P, h->start, h->sf, h->PS);
+    } else if (h->sf.alpha == 2) {
       LAUNCH_N2(k_sf_gather, N, 2, dim3(nblk), h->M, h->P, h->start, h->sf, h->PS);
     } else {
       LAUNCH_N2(k_sf_gather, N, 0, dim3(nblk), h->M, h->P, h->start, h->sf, h->PS);
@@ -1078,6 +1195,8 @@ int pic_create(const pic_mesh_desc *mesh, const pic_params *par, pic_handle **ou
     h->sf.w_sf = 1. / (2. * BetaFac * a + 2 * BetaFac) * (a + 1.) / (M_PI * (r * r * r));
     h->sf.alpha = par->sf_alpha;
     h->sf.linear = mesh->NGeo == 1;
+    // PIC_SF_KERNEL=valu selects the all-VALU gather (for comparisons)
+    if (const char *k = std::getenv("PIC_SF_KERNEL")) h->sfMatrix = std::strcmp(k, "valu") != 0;
   }
   // particles
   const int64_t cap = par->max_particles;
@@ -1209,6 +1328,8 @@ int pic_set_particles(pic_handle *h, int64_t n, const double *PartState, const i
     HIPCHK(hipMemcpy(h->newElem, pe.data(), np * sizeof(int32_t), hipMemcpyHostToDevice));
     LAUNCH_NG(k_rm_locate, h->hm.N, h->hm.NGeo, dim3(grid_for(np)), h->M, np, h->mv.last, h->newElem);
     HIPCHK(hipGetLastError());
+    // the stage's stream does not synchronise with the null stream's copies
+    HIPCHK(hipStreamSynchronize(h->stream));
     HIPCHK(hipMemcpy(pe.data(), h->newElem, np * sizeof(int32_t), hipMemcpyDeviceToHost));
     for (int64_t k = 0; k < np; ++k) el[pending[k]] = pe[k];
   }
