@@ -51,6 +51,61 @@ def measured_traffic(kernel):
     return None
 
 
+def run_cube(w, args, precision, keep=False):
+    """Warm-up, then the timed steps; with keep, the initial deposition and the
+    final particle state for the mixed-precision error report (outside the
+    timed region)."""
+    from piclas_amd import stage, workloads as W
+    n = len(w.PartState)
+    s = stage.PICStage(w.mesh, w.N, max_particles=n, precision=precision, **W.stage_kwargs(w))
+    s.set_field(w.E)
+    s.set_particles(w.PartState, elem=w.elem, species=w.species)
+    s.synchronize()
+    res = {}
+    if keep:
+        s.deposition()
+        res["PS0"] = s.part_source()
+    if args.warmup:
+        s.time_step(w.dt, args.warmup)
+    s.synchronize()
+    s.reset_stats()
+    t0 = time.perf_counter()
+    s.time_step(w.dt, args.steps)
+    s.synchronize()
+    res["t"] = time.perf_counter() - t0
+    res["stats"] = s.stats()
+    if keep:
+        res["state"] = s.particles()[0]
+    s.close()
+    return res
+
+
+def mixed_report(w, args, r64):
+    """The same workload in mixed precision: throughput, speedup and the
+    error against the FP64 run (deposited source of the initial state, and
+    positions / kinetic energy after warm-up + timed steps)."""
+    import numpy as np
+    from piclas_amd import stage
+    rm = run_cube(w, args, stage.PREC_MIXED, keep=True)
+    n = len(w.PartState)
+    a, b = rm["PS0"], r64["PS0"]
+    ps_m, ps_d = rm["state"], r64["state"]
+    m_e = w.mass[0]
+    ke_m = 0.5 * m_e * float((ps_m[:, 3:] ** 2).sum())
+    ke_d = 0.5 * m_e * float((ps_d[:, 3:] ** 2).sum())
+    stm = rm["stats"]
+    return {"value": n * args.steps / rm["t"], "ms_per_step": rm["t"] / args.steps * 1e3,
+            "speedup_vs_fp64": r64["t"] / rm["t"],
+            "deposit_ms_per_step": stm["ms_deposit"] / args.steps,
+            "rho_rel_err_L2": float(np.linalg.norm(a - b) / np.linalg.norm(b)),
+            "rho_rel_err_Linf": float(np.abs(a - b).max() / np.abs(b).max()),
+            "pos_err_max_over_L": float(np.abs(ps_m[:, :3] - ps_d[:, :3]).max()),
+            "kinetic_energy_rel_diff": abs(ke_m - ke_d) / ke_d,
+            "after_steps": args.warmup + args.steps,
+            "note": "FP32 Lagrange basis, field interpolation and shape weights, FP32 partial sums per source chunk "
+                    "into FP64; FP64 positions, Newton and face tests"}
+
+
 def cpu_baseline(w, fraction, steps, threads):
     """The FP64 CPU oracle (the reference's algorithm restated in C, OpenMP)
     on a bounded sample of the same workload."""
@@ -80,6 +135,7 @@ def main():
     ap.add_argument("--nx", type=int, default=48)
     ap.add_argument("--ppc", type=int, default=200)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision run")
     ap.add_argument("--cpu-fraction", type=float, default=1.0 / 16)
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--workload", choices=["cube", "annulus"], default="cube",
@@ -98,19 +154,8 @@ def main():
     n = len(w.PartState)
     print("bench: %s, %d particles, %d warmup + %d timed steps" % (w.name, n, args.warmup, args.steps),
           file=sys.stderr, flush=True)
-    s = stage.PICStage(w.mesh, w.N, max_particles=n, **W.stage_kwargs(w))
-    s.set_field(w.E)
-    s.set_particles(w.PartState, elem=w.elem, species=w.species)
-    s.synchronize()
-    if args.warmup:
-        s.time_step(w.dt, args.warmup)
-    s.synchronize()
-    s.reset_stats()
-    t0 = time.perf_counter()
-    s.time_step(w.dt, args.steps)
-    s.synchronize()
-    t = time.perf_counter() - t0
-    st = s.stats()
+    r64 = run_cube(w, args, stage.PREC_FP64, keep=not args.no_mixed)
+    t, st = r64["t"], r64["stats"]
     n_now = st["n_particles"]
     value = n * args.steps / t
 
@@ -120,13 +165,17 @@ def main():
     fpp, cand, inrad = sf_flops_per_particle(w.sf_radius / h, math.sqrt(3) / 2, (w.N + 1) ** 3)
     avg_ms = st["ms_deposit_main"] / max(st["n_deposit_launches"], 1)
     achieved = fpp * n_now / (avg_ms * 1e-3) / 1e12
-    kname = "k_sf_gather<%d, %d>" % (w.N, w.sf_alpha)
+    if w.N == 3 and os.environ.get("PIC_SF_KERNEL", "") != "valu":
+        kname = "k_sf_gather_mfma<%d>" % (w.sf_alpha if w.sf_alpha == 2 else 0)
+    else:
+        kname = "k_sf_gather<%d, %d>" % (w.N, w.sf_alpha if w.sf_alpha == 2 else 0)
     traffic = measured_traffic(kname) if (args.nx, args.ppc) == (48, 200) else None
     roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
             "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": traffic,
             "kernel": kname, "avg_launch_ms": round(avg_ms, 4),
             "algorithmic_flops_per_particle": round(fpp, 1),
-            "note": "FP64 vector ALU bound (no MFMA); peak = MI355X FP64 vector peak"}
+            "note": "FP64 issue bound: S on the VALU, its products summed on FP64 MFMA (4x4x4); "
+                    "peak = MI355X FP64 vector peak (the FP64 matrix peak is the same 78.6 TFLOP/s)"}
     ms_step = t / args.steps * 1e3
     out = {"metric": METRIC, "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
@@ -139,7 +188,8 @@ def main():
                                  "interp_push_track": st["ms_push_track"] / args.steps,
                                  "sort": st["ms_sort"] / args.steps},
            "roofline": roof}
-    s.close()
+    if not args.no_mixed:
+        out["mixed"] = mixed_report(w, args, r64)
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(w, args.cpu_fraction, args.cpu_steps, threads)

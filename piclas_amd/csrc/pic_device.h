@@ -49,6 +49,9 @@ struct DevMesh {
   double gmin[3], gmax[3];
   int dirPV[3];
   double xiCLg[3], wBaryCLg[3];  // CL basis of NGeo
+  // mixed precision: E and the Gauss basis in FP32
+  const float *E32;         // [e][nDof][3]
+  float xGP32[8], wBaryGP32[8];
 };
 
 struct DevParts {
@@ -316,10 +319,58 @@ __device__ inline void em_field_dw(const DevMesh &M, int e, const double x[3], d
       }
 }
 
+// Mixed precision: LagrangeInterpolationPolys and EvaluateFieldAtRefPos in
+// FP32 (nodes equal to the FP32 machine epsilon count as hit)
+template <int NP>
+__device__ inline void lagrange_f32(float x, const float *xg, const float *wb, float *L) {
+  bool eq = false;
+#pragma unroll
+  for (int i = 0; i < NP; ++i) {
+    L[i] = 0.f;
+    const float d = fabsf(x - xg[i]);
+    if ((x == 0.f || xg[i] == 0.f) ? d <= 2.f * 1.1920929e-7f
+                                   : (d <= 1.1920929e-7f * fabsf(x) && d <= 1.1920929e-7f * fabsf(xg[i]))) {
+      L[i] = 1.f;
+      eq = true;
+    }
+  }
+  if (eq) return;
+  float sum = 0.f;
+#pragma unroll
+  for (int i = 0; i < NP; ++i) { L[i] = wb[i] / (x - xg[i]); sum = sum + L[i]; }
+#pragma unroll
+  for (int i = 0; i < NP; ++i) L[i] = L[i] / sum;
+}
+
+template <int N>
+__device__ inline void eval_field_f32(const DevMesh &M, int e, const double xi[3], double U[3]) {
+  constexpr int n1 = N + 1;
+  float L[3][n1];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) lagrange_f32<n1>(float(xi[d]), M.xGP32, M.wBaryGP32, L[d]);
+  const float *Ue = M.E32 + (size_t)e * n1 * n1 * n1 * 3;
+  float u0 = 0.f, u1 = 0.f, u2 = 0.f;
+#pragma unroll 1
+  for (int k = 0; k < n1; ++k)
+#pragma unroll 1
+    for (int j = 0; j < n1; ++j) {
+      const float Lez = L[1][j] * L[2][k];
+#pragma unroll
+      for (int i = 0; i < n1; ++i) {
+        const float *u = Ue + ((k * n1 + j) * n1 + i) * 3;
+        const float w = L[0][i] * Lez;
+        u0 = u0 + u[0] * w;
+        u1 = u1 + u[1] * w;
+        u2 = u2 + u[2] * w;
+      }
+    }
+  U[0] = u0; U[1] = u1; U[2] = u2;
+}
+
 // InterpolateFieldToSingleParticle for the 508 stepper
 // (particles/pic/interpolation/pic_interpolation.f90:325-369):
 // external field + E(xi) with xi from GetPositionInRefElem.
-template <int N>
+template <int N, bool MIXED = false>
 __device__ inline void field_at_particle(const DevMesh &M, int e, const double x[3], double F[6]) {
 #pragma unroll
   for (int c = 0; c < 6; ++c) F[c] = M.ext[c];
@@ -328,6 +379,7 @@ __device__ inline void field_at_particle(const DevMesh &M, int e, const double x
   const bool ok = ref_elem_newton(M, e, x, xi, true);
   double U[3];
   if (!ok && M.deposition == 1) em_field_dw<N>(M, e, x, U);
+  else if (MIXED) eval_field_f32<N>(M, e, xi, U);
   else eval_field<N>(M, e, xi, U);
   F[0] = F[0] + U[0];
   F[1] = F[1] + U[1];

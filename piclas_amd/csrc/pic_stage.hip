@@ -60,7 +60,7 @@ __device__ inline int lane_id() { return __lane_id(); }
 // ------------------------------------------------------------------ kernels
 
 // Interpolation only (InterpolateFieldToParticle): F[i] = ext + E(x_i)
-template <int N>
+template <int N, bool MIXED = false>
 __global__ __launch_bounds__(kBlock) void k_interp(DevMesh M, DevParts P, int64_t n) {
   const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
   if (i >= n) return;
@@ -69,7 +69,7 @@ __global__ __launch_bounds__(kBlock) void k_interp(DevMesh M, DevParts P, int64_
   const int sp = P.spec[i];
   if (e >= 0 && fabs(M.charge[sp]) > 0.0) {
     const double x[3] = {P.x[i], P.y[i], P.z[i]};
-    field_at_particle<N>(M, e, x, F);
+    field_at_particle<N, MIXED>(M, e, x, F);
   }
 #pragma unroll
   for (int c = 0; c < 6; ++c) P.F[i * 6 + c] = F[c];
@@ -166,7 +166,7 @@ struct Movers {
   double *last;     // [cap][3] LastPartPos
 };
 
-template <int N>
+template <int N, bool MIXED = false>
 __global__ __launch_bounds__(kBlock) void k_push_inside(DevMesh M, DevParts P, int64_t n, double dt, int32_t *newElem,
                                                         Movers mv) {
   const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
@@ -182,7 +182,7 @@ __global__ __launch_bounds__(kBlock) void k_push_inside(DevMesh M, DevParts P, i
       last[0] = x[0]; last[1] = x[1]; last[2] = x[2];
       const int sp = P.spec[i];
       double F[6] = {0, 0, 0, 0, 0, 0};
-      if (fabs(M.charge[sp]) > 0.0) field_at_particle<N>(M, e0, x, F);
+      if (fabs(M.charge[sp]) > 0.0) field_at_particle<N, MIXED>(M, e0, x, F);
       boris_push(M.charge[sp], M.mass[sp], P.isNew[i] != 0, F, dt, x, v);
       P.x[i] = x[0]; P.y[i] = x[1]; P.z[i] = x[2];
       P.vx[i] = v[0]; P.vy[i] = v[1]; P.vz[i] = v[2];
@@ -235,12 +235,13 @@ __global__ __launch_bounds__(kBlock) void k_track_movers(DevMesh M, DevParts P, 
 // ---- RefMapping (ParticleRefTracking) kernels -------------------------------
 // G = NGeo + 1 geometry points per direction.  The field is evaluated at the
 // stored PartPosRef (GetInterpolatedFieldPartPos, pic_interpolation_tools.f90:242-244).
-template <int N>
+template <int N, bool MIXED = false>
 __device__ inline void field_at_ref(const DevMesh &M, int e, const double xi[3], double F[6]) {
 #pragma unroll
   for (int c = 0; c < 6; ++c) F[c] = M.ext[c];
   double U[3];
-  eval_field<N>(M, e, xi, U);
+  if (MIXED) eval_field_f32<N>(M, e, xi, U);
+  else eval_field<N>(M, e, xi, U);
   F[0] = F[0] + U[0];
   F[1] = F[1] + U[1];
   F[2] = F[2] + U[2];
@@ -256,7 +257,7 @@ __device__ inline void rm_removed(DevParts &P, int64_t i, const double x[3], con
 }
 
 // fused step: interpolate at PartPosRef + push + ParticleRefTracking
-template <int N, int G>
+template <int N, int G, bool MIXED = false>
 __global__ __launch_bounds__(kBlock) void k_rm_step(DevMesh M, DevParts P, int64_t n, double dt, int32_t *newElem,
                                                     unsigned long long *lostCount, uint8_t *insideById,
                                                     double *stateById) {
@@ -270,7 +271,7 @@ __global__ __launch_bounds__(kBlock) void k_rm_step(DevMesh M, DevParts P, int64
   double xi[3] = {P.rx[i], P.ry[i], P.rz[i]};
   const int sp = P.spec[i];
   double F[6] = {0, 0, 0, 0, 0, 0};
-  if (fabs(M.charge[sp]) > 0.0) field_at_ref<N>(M, e0, xi, F);
+  if (fabs(M.charge[sp]) > 0.0) field_at_ref<N, MIXED>(M, e0, xi, F);
   boris_push(M.charge[sp], M.mass[sp], P.isNew[i] != 0, F, dt, x, v);
   int status;
   const int e = rm_track<N, G>(M, e0, x, last, xi, status);
@@ -283,7 +284,7 @@ __global__ __launch_bounds__(kBlock) void k_rm_step(DevMesh M, DevParts P, int64
 }
 
 // staged InterpolateFieldToParticle with RefMapping
-template <int N>
+template <int N, bool MIXED = false>
 __global__ __launch_bounds__(kBlock) void k_rm_interp(DevMesh M, DevParts P, int64_t n) {
   const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
   if (i >= n) return;
@@ -292,7 +293,7 @@ __global__ __launch_bounds__(kBlock) void k_rm_interp(DevMesh M, DevParts P, int
   const int sp = P.spec[i];
   if (e >= 0 && fabs(M.charge[sp]) > 0.0) {
     const double xi[3] = {P.rx[i], P.ry[i], P.rz[i]};
-    field_at_ref<N>(M, e, xi, F);
+    field_at_ref<N, MIXED>(M, e, xi, F);
   }
 #pragma unroll
   for (int c = 0; c < 6; ++c) P.F[i * 6 + c] = F[c];
@@ -720,6 +721,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
 // D_b[i][j]) puts particle k = lane/16 and DOF 16Q + lane%16 on each lane for
 // the operands, and DOF 16Q + 4((lane/4)%4) + lane/16, component lane%4 on
 // the accumulator.
+#ifndef SF_GROUP_UNROLL
+#define SF_GROUP_UNROLL 1
+#endif
 template <int ALPHA>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE_MX))) void k_sf_gather_mfma(DevMesh M, DevParts P, const int32_t *start, SFArgs A, double *PS) {
 #pragma clang fp contract(fast)
@@ -796,6 +800,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE_MX))
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#if SF_GROUP_UNROLL > 1
+#pragma unroll SF_GROUP_UNROLL
+#endif
       for (int g = 0; g < ng; ++g) {
         const double *rec = L[4 * g + kp];
         const double qx = rec[0], qy = rec[1], qz = rec[2], qq = rec[3];
@@ -819,6 +826,102 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE_MX))
   const int dofLo = 4 * ((lane >> 2) & 3) + (lane >> 4);
 #pragma unroll
   for (int Q = 0; Q < 4; ++Q) PS[((size_t)t * nd + 16 * Q + dofLo) * 4 + jc] = acc[Q];
+}
+
+// Mixed-precision shape-function gather for N = 3: S and the products in
+// FP32, summed in FP32 on the matrix cores (v_mfma_f32_4x4x1, 16 blocks of
+// 4x4: lane 4b+i gives A_b[i] = S^alpha of DOF lane, lane 4b+j gives
+// B_b[j] = f (v,1)_j of the particle, lane 4b+j holds D_b[i][j] in register
+// i), then added into FP64 at the end of every source chunk of <= 64
+// particles.  Positions relative to the target barycentre are formed in FP64.
+typedef float v4f __attribute__((ext_vector_type(4)));
+template <int ALPHA>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) void k_sf_gather_mixed(DevMesh M, DevParts P, const int32_t *start, SFArgs A, double *PS) {
+#pragma clang fp contract(fast)
+  constexpr int nd = 64;
+  __shared__ float lds[kBlock / kWave][kWave][8];
+  const int wib = threadIdx.x / kWave;
+  const int lane = lane_id();
+  const int lb = xcd_block(blockIdx.x, gridDim.x);
+  const int t = lb * (kBlock / kWave) + wib;
+  if (t >= M.nElems) return;
+  const double bx = M.bary[t * 3], by = M.bary[t * 3 + 1], bz = M.bary[t * 3 + 2];
+  const double R2i0 = A.r2_sf_inv;
+  float mx, my, mz, cd;
+  double blo[3], bhi[3];
+  {
+    const double *xg = M.xGPe + ((size_t)t * nd + lane) * 3;
+    const double dx = xg[0] - bx, dy = xg[1] - by, dz = xg[2] - bz;
+    mx = float(2.0 * R2i0 * dx); my = float(2.0 * R2i0 * dy); mz = float(2.0 * R2i0 * dz);
+    cd = float(1.0 - R2i0 * (dx * dx + dy * dy + dz * dz));
+    blo[0] = bhi[0] = dx; blo[1] = bhi[1] = dy; blo[2] = bhi[2] = dz;
+  }
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1)
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      blo[a] = fmin(blo[a], __shfl_xor(blo[a], off, kWave));
+      bhi[a] = fmax(bhi[a], __shfl_xor(bhi[a], off, kWave));
+    }
+  const double thr = A.r_sf + M.radius[t];
+  const double R2box = A.r2_sf * (1.0 + 1e-12);
+  float (*L)[8] = lds[wib];
+  const int jc = lane & 3;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int q = A.off[t]; q < A.off[t + 1]; ++q) {
+    const int s = A.src[q];
+    const int si = A.shift[q];
+    const double shx = A.sh[si][0], shy = A.sh[si][1], shz = A.sh[si][2];
+    const int s0 = start[s], s1 = start[s + 1];
+    for (int base = s0; base < s1; base += kWave) {
+      const int i = base + lane;
+      bool pass = false;
+      double px = 0, py = 0, pz = 0, f = 0, vx = 0, vy = 0, vz = 0;
+      if (i < s1) {
+        const int sp = P.spec[i];
+        const double q0 = M.charge[sp];
+        px = (P.x[i] + shx) - bx; py = (P.y[i] + shy) - by; pz = (P.z[i] + shz) - bz;
+        const double gx = fmax(0.0, fmax(blo[0] - px, px - bhi[0]));
+        const double gy = fmax(0.0, fmax(blo[1] - py, py - bhi[1]));
+        const double gz = fmax(0.0, fmax(blo[2] - pz, pz - bhi[2]));
+        pass = (fabs(q0) > 0.0) && (gx * gx + gy * gy + gz * gz <= R2box) &&
+               (A.linear || !(sqrt(px * px + py * py + pz * pz) > thr));
+        f = q0 * M.mpf[sp] * A.w_sf;
+        vx = P.vx[i]; vy = P.vy[i]; vz = P.vz[i];
+      }
+      const unsigned long long m = __ballot(pass);
+      const int cnt = __popcll(m);
+      if (cnt == 0) continue;
+      if (pass) {
+        const int slot = __popcll(m & ((1ull << lane) - 1ull));
+        L[slot][0] = float(px); L[slot][1] = float(py); L[slot][2] = float(pz);
+        L[slot][3] = float(R2i0 * (px * px + py * py + pz * pz));
+        L[slot][4] = float(vx * f); L[slot][5] = float(vy * f); L[slot][6] = float(vz * f); L[slot][7] = float(f);
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      v4f c = {0.f, 0.f, 0.f, 0.f};
+      for (int j = 0; j < cnt; ++j) {
+        const float qx = L[j][0], qy = L[j][1], qz = L[j][2], qq = L[j][3];
+        const float bval = L[j][4 + jc];
+        const float S = fmaxf(mx * qx + (my * qy + (mz * qz + (cd - qq))), 0.f);
+        float S1 = S * S;
+        if (ALPHA == 0) {
+          for (int ex = 3; ex <= A.alpha; ++ex) S1 = S * S1;
+        } else {
+#pragma unroll
+          for (int ex = 3; ex <= ALPHA; ++ex) S1 = S * S1;
+        }
+        c = __builtin_amdgcn_mfma_f32_4x4x1f32(S1, bval, c, 0, 0, 0);
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc[r] += double(c[r]);
+      __builtin_amdgcn_wave_barrier();
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < 4; ++r) PS[((size_t)t * nd + 4 * (lane >> 2) + r) * 4 + jc] = acc[r];
 }
 
 // ---- analysis integrals ------------------------------------------------------
@@ -881,6 +984,10 @@ __global__ __launch_bounds__(kBlock) void k_species_count(int64_t n, const uint8
   if (threadIdx.x < kMaxSpecies && c[threadIdx.x]) atomicAdd(&cnt[threadIdx.x], (unsigned long long)c[threadIdx.x]);
 }
 
+__global__ void k_to_f32(const double *a, float *b, int64_t n) {
+  const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (i < n) b[i] = float(a[i]);
+}
 __global__ void k_fill_u8(uint8_t *p, int64_t n, uint8_t v) {
   const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
   if (i < n) p[i] = v;
@@ -919,6 +1026,7 @@ struct pic_handle {
   double *stateById = nullptr;
   unsigned long long *lostCount = nullptr;
   double *E = nullptr, *PS = nullptr, *elemCorner = nullptr, *classSrc = nullptr;
+  float *E32 = nullptr;   // mixed precision copy of E
   int32_t *elemClass = nullptr, *classOff = nullptr, *classInc = nullptr;
   double *classVol = nullptr;
   double *sJ = nullptr, *elemPart = nullptr, *sumOut = nullptr;
@@ -1030,7 +1138,12 @@ int run_deposition(pic_handle *h, bool timed) {
   } else {
     const int waves = kBlock / kWave;
     const unsigned nblk = xcd_grid((ne + waves - 1) / waves);
-    if (N == 3 && h->sfMatrix) {
+    if (N == 3 && h->par.precision == PIC_PREC_MIXED) {
+      if (h->sf.alpha == 2)
+        hipLaunchKernelGGL((k_sf_gather_mixed<2>), dim3(nblk), dim3(kBlock), 0, h->stream, h->M, h->P, h->start, h->sf, h->PS);
+      else
+        hipLaunchKernelGGL((k_sf_gather_mixed<0>), dim3(nblk), dim3(kBlock), 0, h->stream, h->M, h->P, h->start, h->sf, h->PS);
+    } else if (N == 3 && h->sfMatrix) {
       if (h->sf.alpha == 2)
         hipLaunchKernelGGL((k_sf_gather_mfma<2>), dim3(nblk), dim3(kBlock), 0, h->stream, h->M, h->P, h->start, h->sf, h->PS);
       else
@@ -1114,7 +1227,10 @@ int pic_create(const pic_mesh_desc *mesh, const pic_params *par, pic_handle **ou
   if (par->deposition == PIC_DEPO_SF && par->sf_alpha < 2) return fail(PIC_EINVAL, "pic_create: PIC-shapefunction-alpha must be >= 2");
   if (par->max_particles < 1 || par->max_particles > (int64_t(1) << 31) - 1)
     return fail(PIC_EINVAL, "pic_create: max_particles must be in 1..2^31-1");
-  if (par->precision != PIC_PREC_FP64) return fail(PIC_EINVAL, "pic_create: only FP64 precision is implemented");
+  if (par->precision != PIC_PREC_FP64 && par->precision != PIC_PREC_MIXED)
+    return fail(PIC_EINVAL, "pic_create: precision must be PIC_PREC_FP64 or PIC_PREC_MIXED");
+  if (par->precision == PIC_PREC_MIXED && !(par->deposition == PIC_DEPO_SF && mesh->N == 3))
+    return fail(PIC_EINVAL, "pic_create: mixed precision is implemented for shape_function deposition with N = 3");
   pic_handle *h = new pic_handle();
   h->meshDesc = *mesh;
   h->par = *par;
@@ -1168,6 +1284,11 @@ int pic_create(const pic_mesh_desc *mesh, const pic_params *par, pic_handle **ou
   for (int c = 0; c < 6; ++c) M.ext[c] = par->external_field[c];
   for (int s = 0; s < kMaxSpecies; ++s) { M.charge[s] = par->charge[s]; M.mass[s] = par->mass[s]; M.mpf[s] = par->mpf[s]; }
   rc |= dalloc(h, &h->PS, size_t(hm.nElems) * hm.nDof * 4);
+  if (par->precision == PIC_PREC_MIXED) {
+    rc |= dalloc(h, &h->E32, size_t(hm.nElems) * hm.nDof * 3);
+    M.E32 = h->E32;
+    for (int i = 0; i <= hm.N; ++i) { M.xGP32[i] = float(hm.xGP[i]); M.wBaryGP32[i] = float(hm.wBaryGP[i]); }
+  }
   rc |= upload(h, &h->sJ, hm.sJ);
   rc |= dalloc(h, &h->elemPart, hm.nElems);
   rc |= dalloc(h, &h->sumOut, 1);
@@ -1245,10 +1366,22 @@ int pic_destroy(pic_handle *h) {
   return PIC_OK;
 }
 
+namespace {
+// mixed precision: the FP32 copy of E read by the interpolation
+int field_to_f32(pic_handle *h) {
+  if (!h->E32) return 0;
+  const int64_t n = int64_t(h->hm.nElems) * h->hm.nDof * 3;
+  hipLaunchKernelGGL(k_to_f32, dim3(grid_for(n)), dim3(kBlock), 0, h->stream, h->E, h->E32, n);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+}  // namespace
+
 int pic_set_field(pic_handle *h, const double *E) {
   if (!h || !E) return fail(PIC_EINVAL, "pic_set_field: null argument");
   if (set_device(h)) return PIC_EHIP;
   HIPCHK(hipMemcpyAsync(h->E, E, size_t(h->hm.nElems) * h->hm.nDof * 3 * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  if (int rc = field_to_f32(h)) return rc;
   HIPCHK(hipStreamSynchronize(h->stream));
   return PIC_OK;
 }
@@ -1258,6 +1391,7 @@ int pic_set_field_device(pic_handle *h, const double *E_dev) {
   if (set_device(h)) return PIC_EHIP;
   HIPCHK(hipMemcpyAsync(h->E, E_dev, size_t(h->hm.nElems) * h->hm.nDof * 3 * sizeof(double), hipMemcpyDeviceToDevice,
                         h->stream));
+  if (int rc = field_to_f32(h)) return rc;
   HIPCHK(hipStreamSynchronize(h->stream));
   return PIC_OK;
 }
@@ -1402,8 +1536,14 @@ int pic_interpolate_field_to_particle(pic_handle *h) {
   if (int rc = require_particles(h)) return rc;
   if (set_device(h)) return PIC_EHIP;
   if (h->n > 0) {
-    if (h->par.tracking == PIC_TRACK_REFMAPPING) { LAUNCH_N(k_rm_interp, h->hm.N, dim3(grid_for(h->n)), h->M, h->P, h->n); }
-    else { LAUNCH_N(k_interp, h->hm.N, dim3(grid_for(h->n)), h->M, h->P, h->n); }
+    const bool mixed = h->par.precision == PIC_PREC_MIXED;   // N = 3
+    if (h->par.tracking == PIC_TRACK_REFMAPPING) {
+      if (mixed) hipLaunchKernelGGL((k_rm_interp<3, true>), dim3(grid_for(h->n)), dim3(kBlock), 0, h->stream, h->M, h->P, h->n);
+      else { LAUNCH_N(k_rm_interp, h->hm.N, dim3(grid_for(h->n)), h->M, h->P, h->n); }
+    } else {
+      if (mixed) hipLaunchKernelGGL((k_interp<3, true>), dim3(grid_for(h->n)), dim3(kBlock), 0, h->stream, h->M, h->P, h->n);
+      else { LAUNCH_N(k_interp, h->hm.N, dim3(grid_for(h->n)), h->M, h->P, h->n); }
+    }
   }
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(h->stream));
@@ -1451,12 +1591,23 @@ int pic_time_step(pic_handle *h, double dt, int32_t nsteps) {
     if (int rc = run_deposition(h, true)) return rc;
     // LastPartPos <- PartState, interpolate, push, track (one kernel)
     HIPCHK(hipEventRecord(h->ev[3], h->stream));
+    const bool mixed = h->par.precision == PIC_PREC_MIXED;   // N = 3
     if (h->n > 0 && h->par.tracking == PIC_TRACK_REFMAPPING) {
-      LAUNCH_NG(k_rm_step, h->hm.N, h->hm.NGeo, dim3(grid_for(h->n)), h->M, h->P, h->n, dt, h->newElem, h->lostCount,
-                h->insideById, h->stateById);
+      if (mixed && h->hm.NGeo == 1)
+        hipLaunchKernelGGL((k_rm_step<3, 2, true>), dim3(grid_for(h->n)), dim3(kBlock), 0, h->stream, h->M, h->P, h->n, dt,
+                           h->newElem, h->lostCount, h->insideById, h->stateById);
+      else if (mixed)
+        hipLaunchKernelGGL((k_rm_step<3, 3, true>), dim3(grid_for(h->n)), dim3(kBlock), 0, h->stream, h->M, h->P, h->n, dt,
+                           h->newElem, h->lostCount, h->insideById, h->stateById);
+      else
+        LAUNCH_NG(k_rm_step, h->hm.N, h->hm.NGeo, dim3(grid_for(h->n)), h->M, h->P, h->n, dt, h->newElem, h->lostCount,
+                  h->insideById, h->stateById);
     } else if (h->n > 0) {
       HIPCHK(hipMemsetAsync(h->mv.count, 0, sizeof(int32_t), h->stream));
-      LAUNCH_N(k_push_inside, h->hm.N, dim3(grid_for(h->n)), h->M, h->P, h->n, dt, h->newElem, h->mv);
+      if (mixed)
+        hipLaunchKernelGGL((k_push_inside<3, true>), dim3(grid_for(h->n)), dim3(kBlock), 0, h->stream, h->M, h->P, h->n, dt,
+                           h->newElem, h->mv);
+      else { LAUNCH_N(k_push_inside, h->hm.N, dim3(grid_for(h->n)), h->M, h->P, h->n, dt, h->newElem, h->mv); }
       hipLaunchKernelGGL(k_track_movers, dim3(1024), dim3(kBlock), 0, h->stream, h->M, h->P, h->newElem, h->mv,
                          h->lostCount, h->insideById, h->stateById);
     }

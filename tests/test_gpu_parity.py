@@ -264,3 +264,46 @@ def test_refmapping_linear_box_matches_triatracking(oracle_mod):
     s, o = run_pair(oracle_mod, w, 5)
     compare(s, o, 1.0, mesh=w.mesh)
     s.close()
+
+
+def test_mixed_precision_error_bounds(oracle_mod):
+    """Mixed precision (FP32 Lagrange basis, field interpolation and shape
+    weights with FP32 partial sums, FP64 positions, Newton and face tests)
+    against the FP64 oracle: deposited source within 1e-6 relative (L2 and
+    max), positions within 1e-9 of the domain after 10 steps."""
+    w = W.thermal_cube(nx=8, ppc=40, steps=10)
+    kw = W.stage_kwargs(w)
+    with stage.PICStage(w.mesh, w.N, max_particles=len(w.PartState), precision=stage.PREC_MIXED, **kw) as s:
+        s.set_field(w.E)
+        s.set_particles(w.PartState, elem=w.elem, species=w.species)
+        s.deposition()
+        PS0 = s.part_source()
+        s.time_step(w.dt, 10)
+        s.deposition()
+        ps, el, ins = s.particles()
+        PS = s.part_source()
+    o = oracle_mod.Oracle(w.mesh, w.N, deposition=kw["deposition"], sf_radius=kw["sf_radius"], sf_alpha=kw["sf_alpha"],
+                          external_field=kw["external_field"], charge=kw["charge"], mass=kw["mass"], mpf=kw["mpf"])
+    o.set_field(w.E)
+    o.set_particles(w.PartState, elem=w.elem, species=w.species)
+    o.deposition()
+    oPS0 = o.part_source()
+    o.time_step(w.dt, 10)
+    o.deposition()
+    ops, oel, oins = o.particles()
+    oPS = o.part_source()
+    for a, b in ((PS0, oPS0), (PS, oPS)):
+        l2 = np.linalg.norm(a - b) / np.linalg.norm(b)
+        linf = np.abs(a - b).max() / np.abs(b).max()
+        print("mixed vs FP64: rho L2 %.2e  Linf %.2e" % (l2, linf))
+        assert l2 <= 1e-6 and linf <= 1e-6
+    assert ins.all() and oins.all()
+    dx = np.abs(ps[:, :3] - ops[:, :3]).max()
+    print("mixed vs FP64: max |dx| %.2e" % dx)
+    assert dx <= 1e-9
+    # owners agree except for particles within 1e-9 of a face
+    diff = el != oel
+    if diff.any():
+        h = 1.0 / 8
+        r = np.mod(ops[diff, :3], h) / h
+        assert (np.minimum(r, 1 - r).min(1) <= 1e-9).all()

@@ -52,7 +52,8 @@ def _create(**over):
     (dict(tracking=stage.TRACK_REFMAPPING, deposition=stage.DEPO_CVWM), "cell_volweight_mean needs TriaTracking"),
     (dict(deposition=stage.DEPO_SF, sf_radius=0.1, sf_alpha=1), "alpha"),
     (dict(max_particles=0), "max_particles"),
-    (dict(precision=stage.PREC_MIXED), "precision"),
+    (dict(precision=stage.PREC_MIXED), "mixed precision is implemented for shape_function"),
+    (dict(precision=2), "precision must be"),
     (dict(N=9), "N out of range"),
 ])
 def test_create_rejects_bad_parameters(bad, msg):
