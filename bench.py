@@ -136,6 +136,7 @@ def main():
     ap.add_argument("--ppc", type=int, default=200)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision run")
+    ap.add_argument("--autotune", action="store_true", help="pick graph replay / shape-function kernel by measurement")
     ap.add_argument("--cpu-fraction", type=float, default=1.0 / 16)
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--workload", choices=["cube", "annulus"], default="cube",
@@ -204,7 +205,13 @@ def bench_annulus(args):
     n = len(w.PartState)
     print("bench: %s, %d particles, %d warmup + %d timed steps" % (w.name, n, args.warmup, args.steps),
           file=sys.stderr, flush=True)
+    tuning = {}
+    if args.autotune:
+        from piclas_amd.autotune import autotune
+        tuning, table = autotune(w, steps=3, warmup=1, verbose=True)
     s = stage.PICStage(w.mesh, w.N, max_particles=n, **W.stage_kwargs(w))
+    if tuning:
+        s.set_tuning(**tuning)
     s.set_field(w.E)
     t0 = time.perf_counter()
     s.set_particles(w.PartState, elem=w.elem, species=w.species)
@@ -228,7 +235,7 @@ def bench_annulus(args):
            "config": {"workload": w.name, "elements": w.mesh.nElems, "NGeo": w.mesh.NGeo, "N": w.N,
                       "particles": n, "particles_placed": n_in, "deposition": "shape_function R=0.03 m alpha=2",
                       "tracking": "refmapping", "parallelism": "1 GPU"},
-           "setup_s_locate_and_sort": round(t_locate, 3),
+           "setup_s_locate_and_sort": round(t_locate, 3), "tuning": tuning,
            "stage_ms_per_step": {"deposit": st["ms_deposit"] / args.steps,
                                  "interp_push_track": st["ms_push_track"] / args.steps,
                                  "sort": st["ms_sort"] / args.steps},

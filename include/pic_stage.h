@@ -151,6 +151,18 @@ int  pic_deposited_charge(pic_handle *h, double *PartCharge);
 /* CalcPotentialEnergy (src/analyze/analyzefield_tools.f90:99-228, HDG with PP_nVar=1):
  * WEl = eps0/2 sum_e sum_ijk w_i w_j w_k / sJ |E|^2 of the field last set. */
 int  pic_field_energy(pic_handle *h, double *WEl);
+/* Execution tuning (no counterpart in the reference; results do not depend on
+ * it beyond round-off in the order of the shape-function sums):
+ *   use_graph  pic_time_step replays a captured two-step HIP graph (no host
+ *              synchronisation between steps; per-stage timers then report the
+ *              replays as push/track time)
+ *   sf_kernel  0 default, 1 shape-function sums on the vector ALUs, 2 on the
+ *              matrix cores (N = 3) */
+typedef struct {
+  int32_t use_graph;
+  int32_t sf_kernel;
+} pic_tuning;
+int  pic_set_tuning(pic_handle *h, const pic_tuning *t);
 int  pic_get_stats(pic_handle *h, pic_stats *st);
 int  pic_reset_stats(pic_handle *h);
 int  pic_synchronize(pic_handle *h);

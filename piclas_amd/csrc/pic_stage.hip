@@ -168,8 +168,9 @@ struct Movers {
 
 template <int N, bool MIXED = false>
 __global__ __launch_bounds__(kBlock) void k_push_inside(DevMesh M, DevParts P, int64_t n, double dt, int32_t *newElem,
-                                                        Movers mv) {
+                                                        Movers mv, const int32_t *nDev) {
   const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (nDev) n = *nDev;   // particle count on the device (graph replay)
   bool mover = false;
   double last[3] = {0, 0, 0};
   if (i < n) {
@@ -260,8 +261,9 @@ __device__ inline void rm_removed(DevParts &P, int64_t i, const double x[3], con
 template <int N, int G, bool MIXED = false>
 __global__ __launch_bounds__(kBlock) void k_rm_step(DevMesh M, DevParts P, int64_t n, double dt, int32_t *newElem,
                                                     unsigned long long *lostCount, uint8_t *insideById,
-                                                    double *stateById) {
+                                                    double *stateById, const int32_t *nDev) {
   const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (nDev) n = *nDev;
   if (i >= n) return;
   const int e0 = P.elem[i];
   if (e0 < 0) { newElem[i] = -1; return; }
@@ -397,8 +399,9 @@ __global__ __launch_bounds__(1024) void k_scan(int nElems, const int32_t *stayCn
 
 __global__ __launch_bounds__(kBlock) void k_dest(int64_t n, const int32_t *oldElemOf, const int32_t *newElem,
                                                  const int32_t *rank, const int32_t *newStart, const int32_t *stayCnt,
-                                                 int32_t *perm) {
+                                                 int32_t *perm, const int32_t *nDev) {
   const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (nDev) n = *nDev;
   if (i >= n) return;
   const int ne = newElem[i];
   if (ne < 0) return;
@@ -992,6 +995,10 @@ __global__ void k_fill_u8(uint8_t *p, int64_t n, uint8_t v) {
   const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
   if (i < n) p[i] = v;
 }
+__global__ void k_fill_f64(double *p, int64_t n) {
+  const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (i < n) p[i] = 0.0;
+}
 __global__ void k_fill_i32(int32_t *p, int64_t n, int32_t v) {
   const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
   if (i < n) p[i] = v;
@@ -1033,6 +1040,12 @@ struct pic_handle {
   unsigned long long *specCnt = nullptr;
   SFArgs sf{};
   bool sfMatrix = true;   // N = 3 shape function: accumulate on the matrix cores
+  bool useGraph = false;  // replay captured two-step graphs in pic_time_step
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graphExec = nullptr;
+  double graphDt = 0;
+  int64_t graphN = 0;
+  const double *graphP = nullptr;
   Movers mv{};
   bool stagedValid = false;   // LastPartPos/FieldAtParticle refer to the current order
   std::vector<std::pair<int64_t, std::vector<double>>> unplaced;  // particles no element holds at set-up
@@ -1122,13 +1135,15 @@ int set_device(pic_handle *h) {
   return 0;
 }
 
+void drop_graph(pic_handle *h);
+
 // Deposition() on the current sorted particles
 int run_deposition(pic_handle *h, bool timed) {
   const int ne = h->hm.nElems, N = h->hm.N;
   const size_t nps = size_t(ne) * h->hm.nDof * 4;
   if (timed) HIPCHK(hipEventRecord(h->ev[0], h->stream));
   if (h->par.deposition == PIC_DEPO_NONE) {
-    HIPCHK(hipMemsetAsync(h->PS, 0, nps * sizeof(double), h->stream));
+    hipLaunchKernelGGL(k_fill_f64, dim3(grid_for(int64_t(nps))), dim3(kBlock), 0, h->stream, h->PS, int64_t(nps));
   } else if (h->par.deposition == PIC_DEPO_CVWM) {
     LAUNCH_N(k_cvwm_elem, N, dim3(grid_for(int64_t(ne) * kWave)), h->M, h->P, h->start, h->elemCorner);
     if (timed) HIPCHK(hipEventRecord(h->ev[1], h->stream));
@@ -1161,16 +1176,20 @@ int run_deposition(pic_handle *h, bool timed) {
 }
 
 // element re-sort after tracking; newElem[i] holds the owner (-1 = gone)
-int run_sort(pic_handle *h, bool withStaged) {
+// Enqueues the re-sort without waiting on the host: grids cover nGrid
+// particles (>= the count), kernels bound themselves by the count that the
+// old segment table holds on the device.  Pointer swaps happen on the host.
+int enqueue_sort(pic_handle *h, int64_t nGrid, bool withStaged) {
   const int ne = h->hm.nElems;
-  const int64_t n = h->n;
-  HIPCHK(hipMemsetAsync(h->arrCnt, 0, size_t(ne) * sizeof(int32_t), h->stream));
+  const int64_t n = nGrid;
+  const int32_t *nOld = h->start + ne;
+  hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(ne)), dim3(kBlock), 0, h->stream, h->arrCnt, int64_t(ne), 0);
   hipLaunchKernelGGL(k_elem_of, dim3(grid_for(int64_t(ne) * kWave)), dim3(kBlock), 0, h->stream, ne, h->start, h->elemOf);
   hipLaunchKernelGGL(k_seg_rank, dim3(grid_for(int64_t(ne) * kWave)), dim3(kBlock), 0, h->stream, ne, h->start,
                      h->newElem, h->stayCnt, h->arrCnt, h->rank);
   hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, h->stream, ne, h->stayCnt, h->arrCnt, h->startNew);
   hipLaunchKernelGGL(k_dest, dim3(grid_for(n)), dim3(kBlock), 0, h->stream, n, h->elemOf, h->newElem, h->rank,
-                     h->startNew, h->stayCnt, h->perm);
+                     h->startNew, h->stayCnt, h->perm, nOld);
   hipLaunchKernelGGL(k_fix_arrivals, dim3(grid_for(ne)), dim3(kBlock), 0, h->stream, ne, h->startNew, h->stayCnt,
                      h->arrCnt, h->P.id, h->perm);
   const int32_t *total = h->startNew + ne;
@@ -1193,11 +1212,21 @@ int run_sort(pic_handle *h, bool withStaged) {
   std::swap(h->P.spec, h->Q.spec); std::swap(h->P.isNew, h->Q.isNew);
   std::swap(h->P.rx, h->Q.rx); std::swap(h->P.ry, h->Q.ry); std::swap(h->P.rz, h->Q.rz);
   std::swap(h->start, h->startNew);
+  return 0;
+}
+
+// particle count after the last enqueued sort (synchronises the stream)
+int read_count(pic_handle *h) {
   int32_t total_h = 0;
-  HIPCHK(hipMemcpyAsync(&total_h, h->start + ne, sizeof(int32_t), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipMemcpyAsync(&total_h, h->start + h->hm.nElems, sizeof(int32_t), hipMemcpyDeviceToHost, h->stream));
   HIPCHK(hipStreamSynchronize(h->stream));
   h->n = total_h;
   return 0;
+}
+
+int run_sort(pic_handle *h, bool withStaged) {
+  if (int rc = enqueue_sort(h, h->n, withStaged)) return rc;
+  return read_count(h);
 }
 
 int require_particles(pic_handle *h) {
@@ -1358,6 +1387,7 @@ int pic_destroy(pic_handle *h) {
   if (!h) return PIC_OK;
   (void)hipSetDevice(h->device);
   if (h->stream) (void)hipStreamSynchronize(h->stream);
+  drop_graph(h);
   for (void *p : h->allocs) (void)hipFree(p);
   for (auto &e : h->ev)
     if (e) (void)hipEventDestroy(e);
@@ -1509,6 +1539,7 @@ int pic_set_particles(pic_handle *h, int64_t n, const double *PartState, const i
   h->n = nin;
   h->nTotal = n;
   h->stagedValid = false;
+  drop_graph(h);   // recorded for the previous particle arrays
   if (refmap && nin > 0) {
     LAUNCH_NG(k_rm_init, h->hm.N, h->hm.NGeo, dim3(grid_for(nin)), h->M, h->P, nin);
     HIPCHK(hipGetLastError());
@@ -1582,39 +1613,96 @@ int pic_perform_tracking(pic_handle *h) {
   return PIC_OK;
 }
 
+}  // extern "C"
+
+namespace {
+// One 508 step without the field solve, enqueued on the stage's stream with
+// no host synchronisation: Deposition; LastPartPos <- x, interpolate, push,
+// track; re-sort.  nGrid bounds the particle count (the kernels read the
+// count itself from the segment table), so the same launches stay valid
+// while particles are removed; this makes the step capturable in a graph.
+int enqueue_step(pic_handle *h, double dt, int64_t nGrid, bool timed) {
+  if (int rc = run_deposition(h, timed)) return rc;
+  if (timed) HIPCHK(hipEventRecord(h->ev[3], h->stream));
+  const int ne = h->hm.nElems;
+  const int32_t *nDev = h->start + ne;
+  const bool mixed = h->par.precision == PIC_PREC_MIXED;   // N = 3
+  const dim3 grid(grid_for(std::max<int64_t>(nGrid, 1)));
+  if (h->par.tracking == PIC_TRACK_REFMAPPING) {
+    if (mixed && h->hm.NGeo == 1)
+      hipLaunchKernelGGL((k_rm_step<3, 2, true>), grid, dim3(kBlock), 0, h->stream, h->M, h->P, nGrid, dt, h->newElem,
+                         h->lostCount, h->insideById, h->stateById, nDev);
+    else if (mixed)
+      hipLaunchKernelGGL((k_rm_step<3, 3, true>), grid, dim3(kBlock), 0, h->stream, h->M, h->P, nGrid, dt, h->newElem,
+                         h->lostCount, h->insideById, h->stateById, nDev);
+    else
+      LAUNCH_NG(k_rm_step, h->hm.N, h->hm.NGeo, grid, h->M, h->P, nGrid, dt, h->newElem, h->lostCount, h->insideById,
+                h->stateById, nDev);
+  } else {
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(kBlock), 0, h->stream, h->mv.count, int64_t(1), 0);
+    if (mixed)
+      hipLaunchKernelGGL((k_push_inside<3, true>), grid, dim3(kBlock), 0, h->stream, h->M, h->P, nGrid, dt, h->newElem,
+                         h->mv, nDev);
+    else { LAUNCH_N(k_push_inside, h->hm.N, grid, h->M, h->P, nGrid, dt, h->newElem, h->mv, nDev); }
+    hipLaunchKernelGGL(k_track_movers, dim3(1024), dim3(kBlock), 0, h->stream, h->M, h->P, h->newElem, h->mv,
+                       h->lostCount, h->insideById, h->stateById);
+  }
+  HIPCHK(hipGetLastError());
+  if (timed) HIPCHK(hipEventRecord(h->ev[4], h->stream));
+  return enqueue_sort(h, nGrid, false);
+}
+
+void drop_graph(pic_handle *h) {
+  if (h->graphExec) (void)hipGraphExecDestroy(h->graphExec);
+  if (h->graph) (void)hipGraphDestroy(h->graph);
+  h->graphExec = nullptr;
+  h->graph = nullptr;
+}
+
+// Two steps captured in one graph: the particle buffers swap every step, so
+// after two the pointers the graph was recorded with are current again.
+int ensure_graph(pic_handle *h, double dt) {
+  if (h->graphExec && h->graphDt == dt && h->graphN >= h->n && h->graphP == h->P.x) return 0;
+  drop_graph(h);
+  const int64_t nGrid = h->n;
+  HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+  int rc = enqueue_step(h, dt, nGrid, false);
+  if (!rc) rc = enqueue_step(h, dt, nGrid, false);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(h->stream, &g);
+  if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+  if (e != hipSuccess) return fail(PIC_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  h->graph = g;
+  HIPCHK(hipGraphInstantiate(&h->graphExec, g, nullptr, nullptr, 0));
+  h->graphDt = dt;
+  h->graphN = nGrid;
+  h->graphP = h->P.x;
+  return 0;
+}
+}  // namespace
+
+extern "C" {
+
 int pic_time_step(pic_handle *h, double dt, int32_t nsteps) {
   if (!h) return fail(PIC_EINVAL, "pic_time_step: null handle");
   if (int rc = require_particles(h)) return rc;
   if (nsteps < 0) return fail(PIC_EINVAL, "pic_time_step: nsteps < 0");
   if (set_device(h)) return PIC_EHIP;
-  for (int it = 0; it < nsteps; ++it) {
-    if (int rc = run_deposition(h, true)) return rc;
-    // LastPartPos <- PartState, interpolate, push, track (one kernel)
-    HIPCHK(hipEventRecord(h->ev[3], h->stream));
-    const bool mixed = h->par.precision == PIC_PREC_MIXED;   // N = 3
-    if (h->n > 0 && h->par.tracking == PIC_TRACK_REFMAPPING) {
-      if (mixed && h->hm.NGeo == 1)
-        hipLaunchKernelGGL((k_rm_step<3, 2, true>), dim3(grid_for(h->n)), dim3(kBlock), 0, h->stream, h->M, h->P, h->n, dt,
-                           h->newElem, h->lostCount, h->insideById, h->stateById);
-      else if (mixed)
-        hipLaunchKernelGGL((k_rm_step<3, 3, true>), dim3(grid_for(h->n)), dim3(kBlock), 0, h->stream, h->M, h->P, h->n, dt,
-                           h->newElem, h->lostCount, h->insideById, h->stateById);
-      else
-        LAUNCH_NG(k_rm_step, h->hm.N, h->hm.NGeo, dim3(grid_for(h->n)), h->M, h->P, h->n, dt, h->newElem, h->lostCount,
-                  h->insideById, h->stateById);
-    } else if (h->n > 0) {
-      HIPCHK(hipMemsetAsync(h->mv.count, 0, sizeof(int32_t), h->stream));
-      if (mixed)
-        hipLaunchKernelGGL((k_push_inside<3, true>), dim3(grid_for(h->n)), dim3(kBlock), 0, h->stream, h->M, h->P, h->n, dt,
-                           h->newElem, h->mv);
-      else { LAUNCH_N(k_push_inside, h->hm.N, dim3(grid_for(h->n)), h->M, h->P, h->n, dt, h->newElem, h->mv); }
-      hipLaunchKernelGGL(k_track_movers, dim3(1024), dim3(kBlock), 0, h->stream, h->M, h->P, h->newElem, h->mv,
-                         h->lostCount, h->insideById, h->stateById);
-    }
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipEventRecord(h->ev[4], h->stream));
-    if (h->n > 0)
-      if (int rc = run_sort(h, false)) return rc;  // synchronises the stream
+  int it = 0;
+  if (h->useGraph && nsteps >= 2 && h->n > 0) {
+    // replay the two-step graph; the device time of the replays is booked
+    // as push/track time (the stages are not timed one by one)
+    if (int rc = ensure_graph(h, dt)) return rc;
+    HIPCHK(hipEventRecord(h->ev[6], h->stream));
+    for (; it + 1 < nsteps; it += 2) HIPCHK(hipGraphLaunch(h->graphExec, h->stream));
+    HIPCHK(hipEventRecord(h->ev[7], h->stream));
+    if (int rc = read_count(h)) return rc;
+    h->msPushTrack += elapsed(h->ev[6], h->ev[7]);
+    h->nSteps += it;
+  }
+  for (; it < nsteps; ++it) {
+    if (int rc = enqueue_step(h, dt, h->n, true)) return rc;
+    if (int rc = read_count(h)) return rc;
     HIPCHK(hipEventRecord(h->ev[5], h->stream));
     HIPCHK(hipEventSynchronize(h->ev[5]));
     h->msDeposit += elapsed(h->ev[0], h->ev[2]);
@@ -1624,6 +1712,15 @@ int pic_time_step(pic_handle *h, double dt, int32_t nsteps) {
     h->nSteps++;
   }
   h->stagedValid = false;
+  return PIC_OK;
+}
+
+int pic_set_tuning(pic_handle *h, const pic_tuning *t) {
+  if (!h || !t) return fail(PIC_EINVAL, "pic_set_tuning: null argument");
+  if (t->sf_kernel < 0 || t->sf_kernel > 2) return fail(PIC_EINVAL, "pic_set_tuning: sf_kernel must be 0, 1 or 2");
+  h->useGraph = t->use_graph != 0;
+  if (t->sf_kernel != 0) h->sfMatrix = t->sf_kernel == 2;
+  if (!h->useGraph && set_device(h) == 0) drop_graph(h);
   return PIC_OK;
 }
 

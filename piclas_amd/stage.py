@@ -56,6 +56,10 @@ class PicStats(ctypes.Structure):
                 ("n_deposit_launches", ctypes.c_int32), ("ms_deposit_main", ctypes.c_double)]
 
 
+class PicTuning(ctypes.Structure):
+    _fields_ = [("use_graph", ctypes.c_int32), ("sf_kernel", ctypes.c_int32)]
+
+
 # Entry points declared in include/pic_stage.h: name -> (restype, argtypes)
 _H = ctypes.c_void_p
 EXPORTS = {
@@ -78,6 +82,7 @@ EXPORTS = {
     "pic_part_source_device": (ctypes.c_int, [_H, ctypes.POINTER(ctypes.c_void_p)]),
     "pic_deposited_charge": (ctypes.c_int, [_H, _f64p]),
     "pic_field_energy": (ctypes.c_int, [_H, _f64p]),
+    "pic_set_tuning": (ctypes.c_int, [_H, ctypes.POINTER(PicTuning)]),
     "pic_get_stats": (ctypes.c_int, [_H, ctypes.POINTER(PicStats)]),
     "pic_reset_stats": (ctypes.c_int, [_H]),
     "pic_synchronize": (ctypes.c_int, [_H]),
@@ -243,6 +248,11 @@ class PICStage:
 
     CalcDepositedCharge = deposited_charge
     CalcPotentialEnergy = field_energy
+
+    def set_tuning(self, use_graph=False, sf_kernel=0):
+        """Execution tuning: graph replay of pic_time_step, shape-function kernel (0 default, 1 VALU, 2 matrix)."""
+        t = PicTuning(int(bool(use_graph)), int(sf_kernel))
+        _chk(load_library().pic_set_tuning(self.h, ctypes.byref(t)), "pic_set_tuning")
 
     def stats(self):
         st = PicStats()
