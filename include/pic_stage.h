@@ -151,6 +151,26 @@ int  pic_deposited_charge(pic_handle *h, double *PartCharge);
 /* CalcPotentialEnergy (src/analyze/analyzefield_tools.f90:99-228, HDG with PP_nVar=1):
  * WEl = eps0/2 sum_e sum_ijk w_i w_j w_k / sJ |E|^2 of the field last set. */
 int  pic_field_energy(pic_handle *h, double *WEl);
+/* Self-consistent steps through a fixed linear field operator, and their
+ * adjoint (no counterpart in the reference, where HDG() supplies E; these
+ * differentiate the particle stage).  G[3*nUniqueNodes][nElems] (row
+ * 3*(GlobalNodeID-1)+c) maps the element means of rho,
+ * rbar_e = sum_ijk w_i w_j w_k rho_e,ijk / 8, to E at the unique corner nodes;
+ * E at the Gauss points is their trilinear interpolation.  Shape-function
+ * deposition, TriaTracking, FP64, periodic meshes.
+ *   pic_sc_time_step: nsteps of deposit, field, interpolate, push, track; then,
+ *                     if loss is given, the field energy (CalcPotentialEnergy)
+ *                     of the field of the final deposition.
+ *   pic_adjoint:      K such steps and the loss, and the gradients of the loss
+ *                     with respect to the positions and velocities at entry
+ *                     (caller order, [nPart][3]); the states are recomputed
+ *                     from checkpoints every checkpoint_every steps.  Leaves the
+ *                     stage in the state after K steps. */
+int  pic_set_field_operator(pic_handle *h, const double *G);
+int  pic_sc_time_step(pic_handle *h, double dt, int32_t nsteps, double *loss);
+int  pic_adjoint(pic_handle *h, double dt, int32_t K, int32_t checkpoint_every, double *loss, double *dLdx0,
+                 double *dLdv0);
+
 /* Execution tuning (no counterpart in the reference; results do not depend on
  * it beyond round-off in the order of the shape-function sums):
  *   use_graph  pic_time_step replays a captured two-step HIP graph (no host

@@ -71,6 +71,9 @@ def lib():
         L.om_get_elem_bary.argtypes = [ctypes.c_void_p, _f64p, _f64p]
         L.om_get_xieta_basis.argtypes = [ctypes.c_void_p, _f64p, _f64p]
         L.om_get_part_pos_ref.argtypes = [ctypes.c_void_p, _f64p]
+        L.om_sc_set_operator.argtypes = [ctypes.c_void_p, _f64p]
+        L.om_sc_field_from_source.argtypes = [ctypes.c_void_p]
+        L.om_sc_forward.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int32, _f64p]
         L.om_get_refmapping_tables.argtypes = [ctypes.c_void_p, _u8p, _f64p]
         L.om_position_in_ref_elem.argtypes = [ctypes.c_void_p, ctypes.c_int32, _f64p, _i32p, _f64p, _u8p]
         L.om_eval_field_at_ref_pos.argtypes = [ctypes.c_void_p, ctypes.c_int32, _f64p, _i32p, _f64p]
@@ -235,6 +238,19 @@ class Oracle:
         s = np.zeros((self.nElems, 6))
         lib().om_get_xieta_basis(self.h, _p(b, _f64p), _p(s, _f64p))
         return b, s
+
+    def sc_set_operator(self, G):
+        self._G = np.ascontiguousarray(G, np.float64)
+        self._check(lib().om_sc_set_operator(self.h, _p(self._G, _f64p)), "sc_set_operator")
+
+    def sc_field_from_source(self):
+        self._check(lib().om_sc_field_from_source(self.h), "sc_field_from_source")
+
+    def sc_forward(self, dt, K):
+        """K self-consistent steps through the field operator; returns the final field energy."""
+        out = np.zeros(1)
+        self._check(lib().om_sc_forward(self.h, dt, K, _p(out, _f64p)), "sc_forward")
+        return float(out[0])
 
     def part_pos_ref(self):
         n = lib().om_get_nparticles(self.h)

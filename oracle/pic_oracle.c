@@ -174,6 +174,9 @@ struct om_state {
   double *epsOneCell;      /* [e] ElemEpsOneCell                              */
   uint8_t *openElem;       /* [e] element has an open particle boundary side  */
   double gmin[3], gmax[3]; /* GEO%x/y/zminglob, maxglob                       */
+  /* self-consistent field through a fixed linear operator (adjoint workload) */
+  double *Gop;             /* [3*nUnique][nElems]: unique-node E from element-mean rho */
+  double *rhoBar, *Enode;  /* [nElems], [nUnique][3]                           */
   int dirPV[3];            /* GEO%DirPeriodicVectors (0-based axis)           */
   int32_t *perOff, *perNodes; /* twins of each unique node (CSR) */
   double *NodeVolume;      /* [nUnique] */
@@ -1069,7 +1072,7 @@ void om_destroy(om_state *s) {
                   s->bary, s->radius, s->xiBasis, s->slen, s->xGPe, s->sJ, s->perOff, s->perNodes,
                   s->NodeVolume, s->binOff, s->binElems, s->E, s->PS, s->NodeSource, s->PartState,
                   s->LastPartPos, s->Field, s->elem, s->lastElem, s->species, s->inside, s->isNew,
-                  s->curved, s->dXCL1, s->epsOneCell, s->openElem, s->PartPosRef};
+                  s->curved, s->dXCL1, s->epsOneCell, s->openElem, s->PartPosRef, s->Gop, s->rhoBar, s->Enode};
   for (size_t i = 0; i < sizeof ptrs / sizeof ptrs[0]; i++) free(ptrs[i]);
   free(s);
 }
@@ -1898,6 +1901,80 @@ int om_time_step(om_state *s, double dt, int32_t nsteps) {
         if (track_one(s, i) != 0) err |= 1;
     if (err) return -1;
   }
+  return 0;
+}
+
+/* ================================================================ self-consistent field
+ * A fixed linear field operator closes the loop for the adjoint workload:
+ *   rhoBar_e = sum_ijk w_i w_j w_k rho_e,ijk / 8       (reference-space mean)
+ *   E_node   = G rhoBar                                  (unique corner nodes)
+ *   E_dof    = trilinear interpolation of the 8 corner nodes at the Gauss points
+ * G is supplied by the caller (e.g. a spectral periodic Poisson solve). */
+int om_sc_set_operator(om_state *s, const double *G) {
+  const size_t rows = (size_t)3 * s->nUnique, cols = (size_t)s->nElems;
+  free(s->Gop); free(s->rhoBar); free(s->Enode);
+  s->Gop = xcalloc(rows * cols, sizeof(double));
+  memcpy(s->Gop, G, rows * cols * sizeof(double));
+  s->rhoBar = xcalloc(cols, sizeof(double));
+  s->Enode = xcalloc(rows, sizeof(double));
+  return 0;
+}
+
+int om_sc_field_from_source(om_state *s) {
+  if (!s->Gop) { set_err("no field operator set"); return -1; }
+  const int ne = s->nElems, n1 = s->N + 1, nd = s->nDof;
+  for (int e = 0; e < ne; e++) {
+    double acc = 0.0;
+    for (int k = 0; k < n1; k++) for (int j = 0; j < n1; j++) for (int i = 0; i < n1; i++) {
+      int r = (k * n1 + j) * n1 + i;
+      acc = acc + s->wGP[i] * s->wGP[j] * s->wGP[k] * s->PS[((size_t)e * nd + r) * 4 + 3];
+    }
+    s->rhoBar[e] = acc / 8.0;
+  }
+  const size_t rows = (size_t)3 * s->nUnique;
+#pragma omp parallel for num_threads(s->nthreads) schedule(static)
+  for (size_t r = 0; r < rows; r++) {
+    const double *g = &s->Gop[r * ne];
+    double acc = 0.0;
+    for (int e = 0; e < ne; e++) acc = acc + g[e] * s->rhoBar[e];
+    s->Enode[r] = acc;
+  }
+  for (int e = 0; e < ne; e++) {
+    int uid[8];
+    for (int c = 0; c < 8; c++) uid[c] = s->NodeInfo[s->ElemNodeID[e * 8 + c]];
+    for (int k = 0; k < n1; k++) for (int j = 0; j < n1; j++) for (int i = 0; i < n1; i++) {
+      int r = (k * n1 + j) * n1 + i;
+      double a1 = (s->xGP[i] + 1.0) / 2.0, a2 = (s->xGP[j] + 1.0) / 2.0, a3 = (s->xGP[k] + 1.0) / 2.0;
+      double w[8] = {(1 - a1) * (1 - a2) * (1 - a3), a1 * (1 - a2) * (1 - a3), a1 * a2 * (1 - a3), (1 - a1) * a2 * (1 - a3),
+                     (1 - a1) * (1 - a2) * a3, a1 * (1 - a2) * a3, a1 * a2 * a3, (1 - a1) * a2 * a3};
+      for (int c = 0; c < 3; c++) {
+        double v = 0.0;
+        for (int q = 0; q < 8; q++) v = v + w[q] * s->Enode[(size_t)uid[q] * 3 + c];
+        s->E[((size_t)e * nd + r) * 3 + c] = v;
+      }
+    }
+  }
+  return 0;
+}
+
+/* K self-consistent steps (deposit, field, interpolate, push, track), then the
+ * loss: field energy of the field of the final deposition. */
+int om_sc_forward(om_state *s, double dt, int32_t K, double *loss) {
+  for (int it = 0; it < K; it++) {
+    if (om_deposition(s) != 0) return -1;
+    if (om_sc_field_from_source(s) != 0) return -1;
+    om_interpolate_field_to_particle(s);
+    om_push_boris_leapfrog(s, dt);
+    int err = 0;
+#pragma omp parallel for num_threads(s->nthreads) schedule(static) reduction(| : err)
+    for (int i = 0; i < s->n; i++)
+      if (s->inside[i] && s->lastElem[i] >= 0)
+        if (track_one(s, i) != 0) err |= 1;
+    if (err) return -1;
+  }
+  if (om_deposition(s) != 0) return -1;
+  if (om_sc_field_from_source(s) != 0) return -1;
+  *loss = om_field_energy(s);
   return 0;
 }
 

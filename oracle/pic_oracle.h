@@ -112,6 +112,13 @@ double om_deposited_charge(const om_state *s);
 double om_particle_charge(const om_state *s);
 double om_field_energy(const om_state *s);
 
+/* Self-consistent loop through a fixed linear field operator (adjoint workload):
+ * G[3*nUniqueNodes][nElems] maps element-mean rho to E at the unique corner
+ * nodes; E at the Gauss points is their trilinear interpolation. */
+int  om_sc_set_operator(om_state *s, const double *G);
+int  om_sc_field_from_source(om_state *s);
+int  om_sc_forward(om_state *s, double dt, int32_t K, double *loss);
+
 /* 1D node / basis helpers (basis.f90), exposed for closed-form checks */
 void om_legendre_gauss_nodes(int32_t N, double *x, double *w);
 void om_cheby_gauss_lobatto_nodes(int32_t N, double *x);

@@ -555,6 +555,7 @@ std::string build_host_mesh(const pic_mesh_desc &m, const pic_params &par, HostM
   }
   hm.nClasses = ncl;
   hm.elemClass.assign(size_t(ne) * 8, 0);
+  hm.elemNode.assign(size_t(ne) * 8, 0);
   std::vector<double> nodeVol(maxid, 0.0);
   {
     const int n1 = m.N + 1;
@@ -565,6 +566,7 @@ std::string build_host_mesh(const pic_mesh_desc &m, const pic_params &par, HostM
       for (int c = 0; c < 8; ++c) {
         uid[c] = m.GlobalNodeIDs[fn + corner_node(cgns_t[c])] - 1;
         hm.elemClass[size_t(e) * 8 + c] = cls[uid[c]];
+        hm.elemNode[size_t(e) * 8 + c] = uid[c];
       }
       // CalcCellLocNodeVolumes (pic_depo_tools.f90:291-303)
       for (int i = 0; i < n1; ++i) for (int j = 0; j < n1; ++j) for (int k = 0; k < n1; ++k) {
@@ -594,6 +596,19 @@ std::string build_host_mesh(const pic_mesh_desc &m, const pic_params &par, HostM
     for (size_t q = 0; q < hm.elemClass.size(); ++q) {
       int c = hm.elemClass[q];
       hm.classInc[hm.classOff[c] + fill[c]++] = int32_t(q);
+    }
+  }
+
+  // unique node -> (element, corner) incidences, in element order
+  hm.nodeOff.assign(size_t(maxid) + 1, 0);
+  for (size_t q = 0; q < hm.elemNode.size(); ++q) hm.nodeOff[hm.elemNode[q] + 1]++;
+  for (int u = 0; u < maxid; ++u) hm.nodeOff[u + 1] += hm.nodeOff[u];
+  hm.nodeInc.assign(hm.elemNode.size(), 0);
+  {
+    std::vector<int32_t> fill(maxid, 0);
+    for (size_t q = 0; q < hm.elemNode.size(); ++q) {
+      const int u = hm.elemNode[q];
+      hm.nodeInc[hm.nodeOff[u] + fill[u]++] = int32_t(q);
     }
   }
 
@@ -708,6 +723,19 @@ std::string build_host_mesh(const pic_mesh_desc &m, const pic_params &par, HostM
       for (int64_t v : cand) { hm.sfSrc.push_back(int32_t(v / 32)); hm.sfShift.push_back(uint8_t(v % 32)); }
       hm.sfOff[t + 1] = int32_t(hm.sfSrc.size());
     }
+    // inverse lists: source element -> (target, shift), targets ascending
+    hm.sfSrcOff.assign(ne + 1, 0);
+    for (int32_t sv : hm.sfSrc) hm.sfSrcOff[sv + 1]++;
+    for (int e = 0; e < ne; ++e) hm.sfSrcOff[e + 1] += hm.sfSrcOff[e];
+    hm.sfTgt.assign(hm.sfSrc.size(), 0);
+    hm.sfTgtShift.assign(hm.sfSrc.size(), 0);
+    std::vector<int32_t> fill(ne, 0);
+    for (int t = 0; t < ne; ++t)
+      for (int q = hm.sfOff[t]; q < hm.sfOff[t + 1]; ++q) {
+        const int sv = hm.sfSrc[q], slot = hm.sfSrcOff[sv] + fill[sv]++;
+        hm.sfTgt[slot] = t;
+        hm.sfTgtShift[slot] = hm.sfShift[q];
+      }
   }
   return "";
 }

@@ -207,14 +207,21 @@ enum RmStatus { kRmInside = 0, kRmRemoved = 1, kRmNotFound = 2 };
 // decide alike unless a trajectory leaves and re-enters within one step.
 // Returns the new owner, or -1 with status kRmRemoved / kRmNotFound (where
 // the reference aborts).  xi: PartPosRef in and out.
+// first part: periodic movement and the mapping in the last element
 template <int N, int G>
-__device__ inline int rm_track(const DevMesh &M, int e0, double x[3], double last[3], double xi[3], int &status) {
-  status = kRmInside;
+__device__ inline bool rm_track_head(const DevMesh &M, int e0, double x[3], double last[3], double xi[3]) {
   if (M.nPV > 0)
     for (int g = 0; g < 4; ++g)
       if (!periodic_movement(M, x, last)) break;
   rm_position<N, G>(M, e0, x, xi);
-  if (max_abs3(xi) < 1.0) return e0;
+  return max_abs3(xi) < 1.0;
+}
+
+// second part, for particles that left e0: the candidate search; xi holds
+// the mapping in e0 on entry (oldXi)
+template <int N, int G>
+__device__ inline int rm_search(const DevMesh &M, int e0, const double x[3], double xi[3], int &status) {
+  status = kRmInside;
   // FIBGM cell (:180-186)
   int b3[3];
 #pragma unroll
@@ -263,6 +270,13 @@ __device__ inline int rm_track(const DevMesh &M, int e0, double x[3], double las
     return -1;
   }
   return test;
+}
+
+template <int N, int G>
+__device__ inline int rm_track(const DevMesh &M, int e0, double x[3], double last[3], double xi[3], int &status) {
+  status = kRmInside;
+  if (rm_track_head<N, G>(M, e0, x, last, xi)) return e0;
+  return rm_search<N, G>(M, e0, x, xi, status);
 }
 
 // SinglePointToElement for RefMapping (particles/tracking/particle_localization.f90:81-203)

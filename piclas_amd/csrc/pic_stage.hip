@@ -285,6 +285,67 @@ __global__ __launch_bounds__(kBlock) void k_rm_step(DevMesh M, DevParts P, int64
   if (e < 0) rm_removed(P, i, x, v, status, lostCount, insideById, stateById);
 }
 
+// Split form of k_rm_step for throughput (as for TriaTracking): kernel A
+// interpolates, pushes and maps the particle in its last element; the few
+// that left it go to a list that kernel B searches, from the same state.
+template <int N, int G, bool MIXED = false>
+__global__ __launch_bounds__(kBlock) void k_rm_push_inside(DevMesh M, DevParts P, int64_t n, double dt, int32_t *newElem,
+                                                           Movers mv, const int32_t *nDev) {
+  const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (nDev) n = *nDev;
+  bool mover = false;
+  if (i < n) {
+    const int e0 = P.elem[i];
+    if (e0 < 0) {
+      newElem[i] = -1;
+    } else {
+      double x[3] = {P.x[i], P.y[i], P.z[i]};
+      double v[3] = {P.vx[i], P.vy[i], P.vz[i]};
+      double last[3] = {x[0], x[1], x[2]};
+      double xi[3] = {P.rx[i], P.ry[i], P.rz[i]};
+      const int sp = P.spec[i];
+      double F[6] = {0, 0, 0, 0, 0, 0};
+      if (fabs(M.charge[sp]) > 0.0) field_at_ref<N, MIXED>(M, e0, xi, F);
+      boris_push(M.charge[sp], M.mass[sp], P.isNew[i] != 0, F, dt, x, v);
+      const bool in = rm_track_head<N, G>(M, e0, x, last, xi);
+      P.x[i] = x[0]; P.y[i] = x[1]; P.z[i] = x[2];
+      P.vx[i] = v[0]; P.vy[i] = v[1]; P.vz[i] = v[2];
+      P.rx[i] = xi[0]; P.ry[i] = xi[1]; P.rz[i] = xi[2];
+      P.isNew[i] = 0;
+      if (in) newElem[i] = e0;
+      else mover = true;
+    }
+  }
+  const unsigned long long m = __ballot(mover);
+  if (m == 0) return;
+  const int lane = lane_id();
+  const int leader = __ffsll((long long)m) - 1;
+  int base = 0;
+  if (lane == leader) base = atomicAdd(mv.count, __popcll(m));
+  base = __shfl(base, leader, kWave);
+  if (mover) mv.idx[base + __popcll(m & ((1ull << lane) - 1ull))] = int32_t(i);
+}
+
+template <int N, int G>
+__global__ __launch_bounds__(kBlock) void k_rm_movers(DevMesh M, DevParts P, int32_t *newElem, Movers mv,
+                                                      unsigned long long *lostCount, uint8_t *insideById,
+                                                      double *stateById) {
+  const int cnt = *mv.count;
+  for (int k = blockIdx.x * kBlock + threadIdx.x; k < cnt; k += gridDim.x * kBlock) {
+    const int i = mv.idx[k];
+    const double x[3] = {P.x[i], P.y[i], P.z[i]};
+    double xi[3] = {P.rx[i], P.ry[i], P.rz[i]};
+    int status;
+    const int e = rm_search<N, G>(M, P.elem[i], x, xi, status);
+    P.rx[i] = xi[0]; P.ry[i] = xi[1]; P.rz[i] = xi[2];
+    newElem[i] = e;
+    if (e < 0) {
+      const double v[3] = {P.vx[i], P.vy[i], P.vz[i]};
+      rm_removed(P, i, x, v, status, lostCount, insideById, stateById);
+    }
+  }
+}
+
 // staged InterpolateFieldToParticle with RefMapping
 template <int N, bool MIXED = false>
 __global__ __launch_bounds__(kBlock) void k_rm_interp(DevMesh M, DevParts P, int64_t n) {
@@ -1013,6 +1074,24 @@ __global__ void k_elem_of(int nElems, const int32_t *start, int32_t *elemOf) {
 
 }  // namespace
 
+namespace {
+struct StepState {       // a stored particle state, sorted by element
+  double *x = nullptr;   // [6][cap]: x y z vx vy vz
+  int32_t *elem = nullptr, *id = nullptr, *start = nullptr;
+  uint8_t *spec = nullptr;
+  double *E = nullptr;   // field of the step [nElems][nDof][3]
+  int64_t n = 0;
+};
+// self-consistent field operator and adjoint buffers
+struct Adjoint {
+  double *G = nullptr, *GT = nullptr, *rbar = nullptr, *lrbar = nullptr, *En = nullptr, *lEn = nullptr, *lE = nullptr;
+  double *lx = nullptr, *lv = nullptr, *lF = nullptr, *xi = nullptr;
+  int32_t *elemNode = nullptr, *nodeOff = nullptr, *nodeInc = nullptr, *srcOff = nullptr, *tgt = nullptr;
+  uint8_t *tgtShift = nullptr;
+  std::vector<StepState> ck, seg;
+};
+}  // namespace
+
 // ======================================================================= handle
 struct pic_handle {
   pic_mesh_desc meshDesc{};
@@ -1039,6 +1118,7 @@ struct pic_handle {
   double *sJ = nullptr, *elemPart = nullptr, *sumOut = nullptr;
   unsigned long long *specCnt = nullptr;
   SFArgs sf{};
+  Adjoint adj;
   bool sfMatrix = true;   // N = 3 shape function: accumulate on the matrix cores
   bool useGraph = false;  // replay captured two-step graphs in pic_time_step
   hipGraph_t graph = nullptr;
@@ -1629,15 +1709,17 @@ int enqueue_step(pic_handle *h, double dt, int64_t nGrid, bool timed) {
   const bool mixed = h->par.precision == PIC_PREC_MIXED;   // N = 3
   const dim3 grid(grid_for(std::max<int64_t>(nGrid, 1)));
   if (h->par.tracking == PIC_TRACK_REFMAPPING) {
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(kBlock), 0, h->stream, h->mv.count, int64_t(1), 0);
     if (mixed && h->hm.NGeo == 1)
-      hipLaunchKernelGGL((k_rm_step<3, 2, true>), grid, dim3(kBlock), 0, h->stream, h->M, h->P, nGrid, dt, h->newElem,
-                         h->lostCount, h->insideById, h->stateById, nDev);
+      hipLaunchKernelGGL((k_rm_push_inside<3, 2, true>), grid, dim3(kBlock), 0, h->stream, h->M, h->P, nGrid, dt,
+                         h->newElem, h->mv, nDev);
     else if (mixed)
-      hipLaunchKernelGGL((k_rm_step<3, 3, true>), grid, dim3(kBlock), 0, h->stream, h->M, h->P, nGrid, dt, h->newElem,
-                         h->lostCount, h->insideById, h->stateById, nDev);
+      hipLaunchKernelGGL((k_rm_push_inside<3, 3, true>), grid, dim3(kBlock), 0, h->stream, h->M, h->P, nGrid, dt,
+                         h->newElem, h->mv, nDev);
     else
-      LAUNCH_NG(k_rm_step, h->hm.N, h->hm.NGeo, grid, h->M, h->P, nGrid, dt, h->newElem, h->lostCount, h->insideById,
-                h->stateById, nDev);
+      LAUNCH_NG(k_rm_push_inside, h->hm.N, h->hm.NGeo, grid, h->M, h->P, nGrid, dt, h->newElem, h->mv, nDev);
+    LAUNCH_NG(k_rm_movers, h->hm.N, h->hm.NGeo, dim3(1024), h->M, h->P, h->newElem, h->mv, h->lostCount,
+              h->insideById, h->stateById);
   } else {
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(kBlock), 0, h->stream, h->mv.count, int64_t(1), 0);
     if (mixed)
@@ -1904,3 +1986,5 @@ int pic_synchronize(pic_handle *h) {
 }
 
 }  // extern "C"
+
+#include "pic_adjoint_impl.h"

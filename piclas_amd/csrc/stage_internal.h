@@ -60,6 +60,9 @@ struct HostMesh {
   double xGP[kMaxN + 1] = {0}, wGP[kMaxN + 1] = {0}, wBaryGP[kMaxN + 1] = {0};
   // cell_volweight_mean: node classes (a unique node and its periodic twins)
   std::vector<int32_t> elemClass;  // [e][8] class of each CGNS corner
+  std::vector<int32_t> elemNode;   // [e][8] unique node (GlobalNodeIDs - 1) of each CGNS corner
+  std::vector<int32_t> nodeOff;    // [nUnique+1] CSR over (e*8+corner) incidences of each unique node
+  std::vector<int32_t> nodeInc;
   std::vector<int32_t> classOff;   // [nClasses+1] CSR over (e*8+corner) incidences
   std::vector<int32_t> classInc;
   std::vector<double> classVolume; // NodeVolume of the class
@@ -68,6 +71,9 @@ struct HostMesh {
   std::vector<int32_t> sfOff;      // [nElems+1]
   std::vector<int32_t> sfSrc;      // source element
   std::vector<uint8_t> sfShift;    // index into shift table (27 combos)
+  // the same pairs by source element: source -> (target element, shift)
+  std::vector<int32_t> sfSrcOff, sfTgt;
+  std::vector<uint8_t> sfTgtShift;
   double shift[27][3] = {{0}};
   int nPV = 0;
   double PV[3][3] = {{0}};

@@ -83,6 +83,9 @@ EXPORTS = {
     "pic_deposited_charge": (ctypes.c_int, [_H, _f64p]),
     "pic_field_energy": (ctypes.c_int, [_H, _f64p]),
     "pic_set_tuning": (ctypes.c_int, [_H, ctypes.POINTER(PicTuning)]),
+    "pic_set_field_operator": (ctypes.c_int, [_H, _f64p]),
+    "pic_sc_time_step": (ctypes.c_int, [_H, ctypes.c_double, ctypes.c_int32, _f64p]),
+    "pic_adjoint": (ctypes.c_int, [_H, ctypes.c_double, ctypes.c_int32, ctypes.c_int32, _f64p, _f64p, _f64p]),
     "pic_get_stats": (ctypes.c_int, [_H, ctypes.POINTER(PicStats)]),
     "pic_reset_stats": (ctypes.c_int, [_H]),
     "pic_synchronize": (ctypes.c_int, [_H]),
@@ -248,6 +251,26 @@ class PICStage:
 
     CalcDepositedCharge = deposited_charge
     CalcPotentialEnergy = field_energy
+
+    # ---------------------------------------------------------------- self-consistent loop and adjoint
+    def set_field_operator(self, G):
+        self._G = np.ascontiguousarray(G, np.float64)
+        _chk(load_library().pic_set_field_operator(self.h, _p(self._G, _f64p)), "pic_set_field_operator")
+
+    def sc_time_step(self, dt, nsteps, with_loss=True):
+        out = np.zeros(1)
+        _chk(load_library().pic_sc_time_step(self.h, dt, nsteps, _p(out, _f64p) if with_loss else None),
+             "pic_sc_time_step")
+        return float(out[0]) if with_loss else None
+
+    def adjoint(self, dt, K, every=10, gradients=True):
+        """K self-consistent steps: (loss, dL/dx0, dL/dv0) with respect to the state at entry."""
+        loss = np.zeros(1)
+        gx = np.zeros((self.nPart, 3)) if gradients else None
+        gv = np.zeros((self.nPart, 3)) if gradients else None
+        _chk(load_library().pic_adjoint(self.h, dt, K, every, _p(loss, _f64p), None if gx is None else _p(gx, _f64p),
+                                        None if gv is None else _p(gv, _f64p)), "pic_adjoint")
+        return float(loss[0]), gx, gv
 
     def set_tuning(self, use_graph=False, sf_kernel=0):
         """Execution tuning: graph replay of pic_time_step, shape-function kernel (0 default, 1 VALU, 2 matrix)."""
