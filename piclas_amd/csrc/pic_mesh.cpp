@@ -505,6 +505,54 @@ std::string build_host_mesh(const pic_mesh_desc &m, const pic_params &par, HostM
         return "RefMapping: a background cell holds more than " + std::to_string(kMaxBinElems) + " elements";
   }
 
+  if (!refmap) {
+    // background cells for SinglePointToElement with TriaTracking: elements
+    // binned by the bounding box of their nodes; cell size = largest extent
+    std::vector<double> emin(size_t(ne) * 3, 1e300), emax(size_t(ne) * 3, -1e300);
+    double ext = 0.0, bmax[3] = {-1e300, -1e300, -1e300};
+    for (int c = 0; c < 3; ++c) hm.bmin[c] = 1e300;
+    for (int e = 0; e < ne; ++e) {
+      const int fn = EI(e, 4), ln = EI(e, 5);
+      for (int q = fn; q < ln; ++q)
+        for (int c = 0; c < 3; ++c) {
+          emin[e * 3 + c] = std::min(emin[e * 3 + c], m.NodeCoords[size_t(q) * 3 + c]);
+          emax[e * 3 + c] = std::max(emax[e * 3 + c], m.NodeCoords[size_t(q) * 3 + c]);
+        }
+      for (int c = 0; c < 3; ++c) {
+        hm.bmin[c] = std::min(hm.bmin[c], emin[e * 3 + c]);
+        bmax[c] = std::max(bmax[c], emax[e * 3 + c]);
+        ext = std::max(ext, emax[e * 3 + c] - emin[e * 3 + c]);
+      }
+    }
+    for (int c = 0; c < 3; ++c) {
+      const double L = bmax[c] - hm.bmin[c];
+      hm.nb[c] = std::max(1, int(std::ceil(L / ext)));
+      hm.bdelta[c] = L / hm.nb[c];
+      if (!(hm.bdelta[c] > 0)) hm.bdelta[c] = 1.0;
+    }
+    const size_t nbins = size_t(hm.nb[0]) * hm.nb[1] * hm.nb[2];
+    hm.binOff.assign(nbins + 1, 0);
+    for (int pass = 0; pass < 2; ++pass) {
+      std::vector<int32_t> fill(pass ? nbins : 0, 0);
+      for (int e = 0; e < ne; ++e) {
+        int lo3[3], hi3[3];
+        for (int c = 0; c < 3; ++c) {
+          lo3[c] = std::max(0, int(std::floor((emin[e * 3 + c] - hm.bmin[c]) / hm.bdelta[c])));
+          hi3[c] = std::min(hm.nb[c] - 1, int(std::floor((emax[e * 3 + c] - hm.bmin[c]) / hm.bdelta[c])));
+        }
+        for (int z = lo3[2]; z <= hi3[2]; ++z) for (int y = lo3[1]; y <= hi3[1]; ++y) for (int x = lo3[0]; x <= hi3[0]; ++x) {
+          const size_t b = (size_t(z) * hm.nb[1] + y) * hm.nb[0] + x;
+          if (!pass) hm.binOff[b + 1]++;
+          else hm.binElems[hm.binOff[b] + fill[b]++] = e;
+        }
+      }
+      if (!pass) {
+        for (size_t q = 0; q < nbins; ++q) hm.binOff[q + 1] += hm.binOff[q];
+        hm.binElems.resize(hm.binOff[nbins]);
+      }
+    }
+  }
+
   // ---- cell_volweight_mean node classes ------------------------------------
   int maxid = 0;
   for (int i = 0; i < m.nNodes; ++i) maxid = std::max(maxid, m.GlobalNodeIDs[i]);

@@ -302,4 +302,27 @@ __device__ inline int rm_locate(const DevMesh &M, const double x[3], double xi[3
   return -1;
 }
 
+// SinglePointToElement with TriaTracking (particles/tracking/particle_localization.f90:81-203):
+// candidates within ElemRadius, nearest first, ParticleInsideQuad3D
+__device__ inline int tria_locate(const DevMesh &M, const double x[3]) {
+  int b3[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    int v = int(ceil((x[c] - M.bmin[c]) / M.bdelta[c]));
+    v = v > M.nb[c] ? M.nb[c] : v;
+    v = v < 1 ? 1 : v;
+    b3[c] = v - 1;
+  }
+  const int b = (b3[2] * M.nb[1] + b3[1]) * M.nb[0] + b3[0];
+  const int b0 = M.binOff[b], n = M.binOff[b + 1] - b0;
+  double dPrev = -HUGE_VAL;
+  int qPrev = -1, q;
+  while ((q = next_candidate(M, b0, n, x, -1, false, dPrev, qPrev)) >= 0) {
+    const int e = M.binElems[b0 + q];
+    double det[6][2];
+    if (inside_quad(M, e, x, det)) return e;
+  }
+  return -1;
+}
+
 }  // namespace pic

@@ -409,6 +409,13 @@ __global__ __launch_bounds__(kBlock) void k_rm_locate(DevMesh M, int64_t n, cons
   double xi[3];
   elem[i] = rm_locate<N, G>(M, x, xi);
 }
+__global__ __launch_bounds__(kBlock) void k_tria_locate(DevMesh M, int64_t n, const double *xyz, int32_t *elem) {
+  const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (i >= n) return;
+  if (elem[i] >= 0) return;
+  const double x[3] = {xyz[i * 3], xyz[i * 3 + 1], xyz[i * 3 + 2]};
+  elem[i] = tria_locate(M, x);
+}
 
 // ---- sort by element --------------------------------------------------------
 // One wave per old element segment: stayers get their rank among the
@@ -1384,9 +1391,9 @@ int pic_create(const pic_mesh_desc *mesh, const pic_params *par, pic_handle **ou
     rc |= upload(h, const_cast<uint8_t **>(&M.curved), hm.curved);
     rc |= upload(h, const_cast<double **>(&M.epsOneCell), hm.epsOneCell);
     rc |= upload(h, const_cast<uint8_t **>(&M.openElem), hm.openElem);
-    rc |= upload(h, const_cast<int32_t **>(&M.binOff), hm.binOff);
-    rc |= upload(h, const_cast<int32_t **>(&M.binElems), hm.binElems);
   }
+  rc |= upload(h, const_cast<int32_t **>(&M.binOff), hm.binOff);
+  rc |= upload(h, const_cast<int32_t **>(&M.binElems), hm.binElems);
   for (int i = 0; i <= hm.N; ++i) { M.xGP[i] = hm.xGP[i]; M.wBaryGP[i] = hm.wBaryGP[i]; }
   for (int v = 0; v < 3; ++v)
     for (int c = 0; c < 3; ++c) M.PV[v][c] = hm.PV[v][c];
@@ -1506,40 +1513,6 @@ int pic_set_field_device(pic_handle *h, const double *E_dev) {
   return PIC_OK;
 }
 
-// SinglePointToElement on the host (TriaTracking inside test), used for
-// particles handed over without an element.
-static int locate_host(const HostMesh &hm, const double x[3]) {
-  // candidate elements by barycentre distance <= radius, nearest first
-  std::vector<std::pair<double, int>> cand;
-  for (int e = 0; e < hm.nElems; ++e) {
-    const double d0 = x[0] - hm.bary[e * 3], d1 = x[1] - hm.bary[e * 3 + 1], d2 = x[2] - hm.bary[e * 3 + 2];
-    const double d2s = d0 * d0 + d1 * d1 + d2 * d2;
-    if (d2s <= hm.radius[e] * hm.radius[e]) cand.emplace_back(d2s, e);
-  }
-  std::stable_sort(cand.begin(), cand.end(), [](auto &a, auto &b) { return a.first < b.first; });
-  for (auto &c : cand) {
-    const int e = c.second;
-    bool in = true;
-    for (int l = 0; l < 6 && in; ++l) {
-      const uint8_t *sn = &hm.sideNode[(e * 6 + l) * 4];
-      const double *C = &hm.corner[size_t(e) * 24];
-      double A[3][4];
-      for (int q = 0; q < 4; ++q)
-        for (int k = 0; k < 3; ++k) A[k][q] = C[sn[q] * 3 + k] - x[k];
-      double cp0 = A[1][0] * A[2][2] - A[2][0] * A[1][2];
-      double cp1 = A[2][0] * A[0][2] - A[0][0] * A[2][2];
-      double cp2 = A[0][0] * A[1][2] - A[1][0] * A[0][2];
-      double d1 = -(cp0 * A[0][1] + cp1 * A[1][1] + cp2 * A[2][1]);
-      double d2 = cp0 * A[0][3] + cp1 * A[1][3] + cp2 * A[2][3];
-      bool neg = d1 < 0 || d2 < 0, pos = !(d1 < 0) || !(d2 < 0);
-      if ((hm.concave[e] >> l) & 1) { if (!pos) in = false; }
-      else if (neg) in = false;
-    }
-    if (in) return e;
-  }
-  return -1;
-}
-
 int pic_set_particles(pic_handle *h, int64_t n, const double *PartState, const int32_t *elem, const int32_t *species,
                       const uint8_t *is_new) {
   if (!h || (n > 0 && !PartState)) return fail(PIC_EINVAL, "pic_set_particles: null argument");
@@ -1553,10 +1526,7 @@ int pic_set_particles(pic_handle *h, int64_t n, const double *PartState, const i
   for (int64_t i = 0; i < n; ++i) {
     int e = elem ? elem[i] : 0;
     if (e < 0 || e > ne) return fail(PIC_EINVAL, "pic_set_particles: element id out of range");
-    if (e == 0) {
-      if (refmap) pending.push_back(i);
-      else e = locate_host(h->hm, &PartState[i * 6]) + 1;
-    }
+    if (e == 0) pending.push_back(i);
     el[i] = e - 1;
     sp[i] = species ? species[i] - 1 : 0;
     if (sp[i] < 0 || sp[i] >= h->par.nSpecies) return fail(PIC_EINVAL, "pic_set_particles: species out of range");
@@ -1570,7 +1540,8 @@ int pic_set_particles(pic_handle *h, int64_t n, const double *PartState, const i
       for (int c = 0; c < 3; ++c) xyz[k * 3 + c] = PartState[pending[k] * 6 + c];
     HIPCHK(hipMemcpy(h->mv.last, xyz.data(), np * 3 * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(h->newElem, pe.data(), np * sizeof(int32_t), hipMemcpyHostToDevice));
-    LAUNCH_NG(k_rm_locate, h->hm.N, h->hm.NGeo, dim3(grid_for(np)), h->M, np, h->mv.last, h->newElem);
+    if (refmap) { LAUNCH_NG(k_rm_locate, h->hm.N, h->hm.NGeo, dim3(grid_for(np)), h->M, np, h->mv.last, h->newElem); }
+    else hipLaunchKernelGGL(k_tria_locate, dim3(grid_for(np)), dim3(kBlock), 0, h->stream, h->M, np, h->mv.last, h->newElem);
     HIPCHK(hipGetLastError());
     // the stage's stream does not synchronise with the null stream's copies
     HIPCHK(hipStreamSynchronize(h->stream));
