@@ -38,16 +38,18 @@ def sf_flops_per_particle(R_over_h, rad_over_h, dofs_per_elem):
     return 8.0 * cand + 4.0 * inrad, cand, inrad
 
 
-def measured_traffic(kernel):
-    """HBM bytes per launch of `kernel` for the 48^3 workload, as recorded by
-    the FETCH_SIZE / WRITE_SIZE counter passes in profiles/ (None if absent)."""
-    path = os.path.join(ROOT, "profiles", "k_sf_gather_cube48_N3_sf_hbm_traffic.md")
+def measured_traffic(kernel, workload="cube48_N3_sf"):
+    """HBM bytes per launch of `kernel` on a workload, as recorded by the
+    FETCH_SIZE / WRITE_SIZE counter passes in profiles/ (None if absent)."""
+    pdir = os.path.join(ROOT, "profiles")
     try:
-        for line in open(path):
+        names = sorted(f for f in os.listdir(pdir) if f.endswith("_hbm_traffic.md") and workload in f)
+    except OSError:
+        return None
+    for f in names:
+        for line in open(os.path.join(pdir, f)):
             if line.startswith("traffic_bytes_per_launch %s:" % kernel):
                 return int(float(line.split(":")[1]))
-    except OSError:
-        pass
     return None
 
 

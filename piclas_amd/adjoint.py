@@ -77,3 +77,34 @@ def landau_box(n_el=16, n_part=2_000_000, alpha=0.01, T_eV=8.0, seed=20261206, s
     dx0 = np.zeros((n_part, 3))
     dx0[:, 0] = dxda
     return w, dx0
+
+
+def stage_loss(s, dt, K, every=10, species=None, locate=None):
+    """The loss of K self-consistent steps of stage ``s`` (whose field operator
+    is set) as a function of the particle state, for torch.autograd:
+
+        f = stage_loss(s, dt, K, species=w.species)
+        L = f(x0, v0)            # x0, v0: (nPart, 3) float64 tensors
+        L.backward()             # x0.grad, v0.grad from pic_adjoint
+
+    Every call sets the particles from (x0, v0) and runs pic_adjoint forward
+    and backward; ``locate(x) -> 1-based element ids`` gives the elements,
+    else the stage locates the particles on the device."""
+    import torch
+
+    class _StageLoss(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, x0, v0):
+            ps = torch.cat([x0, v0], 1).detach().cpu().numpy()
+            elem = None if locate is None else locate(ps[:, :3])
+            s.set_particles(ps, elem=elem, species=species)
+            loss, gx, gv = s.adjoint(dt, K, every)
+            ctx.save_for_backward(torch.from_numpy(gx).to(x0.device), torch.from_numpy(gv).to(v0.device))
+            return torch.tensor(loss, dtype=torch.float64, device=x0.device)
+
+        @staticmethod
+        def backward(ctx, g):
+            gx, gv = ctx.saved_tensors
+            return g * gx, g * gv
+
+    return _StageLoss.apply

@@ -96,38 +96,26 @@ def test_adjoint_matches_finite_differences(oracle_mod):
 
 
 def test_autograd_function_wrapper():
-    """torch.autograd.Function around pic_adjoint: backward returns the
-    adjoint gradients, and a random directional derivative matches central
-    differences of the forward."""
+    """piclas_amd.adjoint.stage_loss (a torch.autograd.Function around
+    pic_adjoint): backward returns the adjoint gradients, and a random
+    directional derivative matches central differences of the forward."""
     import torch
 
     w, _ = A.landau_box(n_el=6, n_part=3000, steps=K)
     G = A.periodic_poisson_operator(w.mesh)
     s = make_stage(w, G)
 
-    class StageLoss(torch.autograd.Function):
-        @staticmethod
-        def forward(ctx, x0, v0):
-            ps = torch.cat([x0, v0], 1).detach().cpu().numpy()
-            s.set_particles(ps, elem=w.mesh.element_of(ps[:, :3]), species=w.species)
-            loss, gx, gv = s.adjoint(w.dt, K, EVERY)
-            ctx.save_for_backward(torch.from_numpy(gx), torch.from_numpy(gv))
-            return torch.tensor(loss, dtype=torch.float64)
-
-        @staticmethod
-        def backward(ctx, g):
-            gx, gv = ctx.saved_tensors
-            return g * gx, g * gv
+    StageLoss = A.stage_loss(s, w.dt, K, EVERY, species=w.species, locate=w.mesh.element_of)
 
     x0 = torch.tensor(w.PartState[:, :3], dtype=torch.float64, requires_grad=True)
     v0 = torch.tensor(w.PartState[:, 3:], dtype=torch.float64, requires_grad=True)
-    L = StageLoss.apply(x0, v0)
+    L = StageLoss(x0, v0)
     L.backward()
     rng = np.random.default_rng(11)
     dv = torch.tensor(rng.normal(size=v0.shape) * 1e-6 * float(v0.detach().std()), dtype=torch.float64)
     with torch.no_grad():
-        lp = StageLoss.apply(x0, v0 + dv)
-        lm = StageLoss.apply(x0, v0 - dv)
+        lp = StageLoss(x0, v0 + dv)
+        lm = StageLoss(x0, v0 - dv)
     fd = float(lp - lm) / 2.0
     ad = float((v0.grad * dv).sum())
     print("directional derivative adjoint %.10e  FD %.10e" % (ad, fd))
