@@ -168,17 +168,20 @@ def main():
     fpp, cand, inrad = sf_flops_per_particle(w.sf_radius / h, math.sqrt(3) / 2, (w.N + 1) ** 3)
     avg_ms = st["ms_deposit_main"] / max(st["n_deposit_launches"], 1)
     achieved = fpp * n_now / (avg_ms * 1e-3) / 1e12
-    if w.N == 3 and os.environ.get("PIC_SF_KERNEL", "") != "valu":
-        kname = "k_sf_gather_mfma<%d>" % (w.sf_alpha if w.sf_alpha == 2 else 0)
-    else:
-        kname = "k_sf_gather<%d, %d>" % (w.N, w.sf_alpha if w.sf_alpha == 2 else 0)
+    a = w.sf_alpha if w.sf_alpha == 2 else 0
+    kname, note = {
+        stage.KERNEL_SF_CARTESIAN: ("k_sf_gather_cart<%d>" % a,
+                                    "FP64 issue bound: per pair a subtraction, the clamp and the square on the VALU "
+                                    "(axis-aligned tensor grid of Gauss points), products summed on FP64 MFMA (4x4x4)"),
+        stage.KERNEL_SF_MATRIX: ("k_sf_gather_mfma<%d, true>" % a,
+                                 "FP64 issue bound: S on the VALU, its products summed on FP64 MFMA (4x4x4)"),
+    }.get(st["deposit_kernel"], ("k_sf_gather<%d, %d>" % (w.N, a), "FP64 issue bound: all on the VALU"))
+    note += "; peak = MI355X FP64 vector peak (the FP64 matrix peak is the same 78.6 TFLOP/s)"
     traffic = measured_traffic(kname) if (args.nx, args.ppc) == (48, 200) else None
     roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
             "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": traffic,
             "kernel": kname, "avg_launch_ms": round(avg_ms, 4),
-            "algorithmic_flops_per_particle": round(fpp, 1),
-            "note": "FP64 issue bound: S on the VALU, its products summed on FP64 MFMA (4x4x4); "
-                    "peak = MI355X FP64 vector peak (the FP64 matrix peak is the same 78.6 TFLOP/s)"}
+            "algorithmic_flops_per_particle": round(fpp, 1), "note": note}
     ms_step = t / args.steps * 1e3
     out = {"metric": METRIC, "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",

@@ -58,6 +58,15 @@ enum { PIC_TRACK_REFMAPPING = 1, PIC_TRACK_TRIA = 3 };         /* TrackingMethod
 enum { PIC_DEPO_NONE = 0, PIC_DEPO_CVWM = 1, PIC_DEPO_SF = 2 }; /* PIC-Deposition-Type                  */
 enum { PIC_BC_OPEN = 1, PIC_BC_REFLECTIVE = 2, PIC_BC_PERIODIC = 3 }; /* PartBound%TargetBoundCond      */
 enum { PIC_PREC_FP64 = 0, PIC_PREC_MIXED = 1 };
+/* dominant deposition kernel, reported in pic_stats.deposit_kernel */
+enum {
+  PIC_KERNEL_NONE = 0,
+  PIC_KERNEL_CVWM = 1,          /* k_cvwm_elem                                   */
+  PIC_KERNEL_SF_VALU = 2,       /* k_sf_gather: shape function on the vector ALUs */
+  PIC_KERNEL_SF_MATRIX = 3,     /* k_sf_gather_mfma: sums on the matrix cores     */
+  PIC_KERNEL_SF_CARTESIAN = 4,  /* k_sf_gather_cart: axis-aligned tensor grids    */
+  PIC_KERNEL_SF_MIXED = 5       /* k_sf_gather_mixed: FP32 sums                   */
+};
 
 /* Mesh in HOPR layout, as the reference's mesh reader holds it
  * (src/mesh/mesh_readin.f90; index constants src/piclas.h:136-176). */
@@ -103,6 +112,7 @@ typedef struct {
   double  ms_sort;                /* device time in the element sort              */
   int32_t n_deposit_launches;     /* launches of the dominant deposition kernel   */
   double  ms_deposit_main;        /* device time of that kernel alone             */
+  int32_t deposit_kernel;         /* that kernel: PIC_KERNEL_*                    */
 } pic_stats;
 
 typedef struct pic_handle pic_handle;

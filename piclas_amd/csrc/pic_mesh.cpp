@@ -736,6 +736,10 @@ std::string build_host_mesh(const pic_mesh_desc &m, const pic_params &par, HostM
     hm.sfOff.assign(ne + 1, 0);
     hm.sfSrc.clear();
     hm.sfShift.clear();
+    hm.sfRunOff.assign(ne + 1, 0);
+    hm.sfRunFirst.clear();
+    hm.sfRunEnd.clear();
+    hm.sfRunShift.clear();
     std::vector<int64_t> cand;
     for (int t = 0; t < ne; ++t) {
       cand.clear();
@@ -770,6 +774,20 @@ std::string build_host_mesh(const pic_mesh_desc &m, const pic_params &par, HostM
       cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
       for (int64_t v : cand) { hm.sfSrc.push_back(int32_t(v / 32)); hm.sfShift.push_back(uint8_t(v % 32)); }
       hm.sfOff[t + 1] = int32_t(hm.sfSrc.size());
+      // runs: by shift, then element; consecutive elements of one shift merge
+      for (int64_t &v : cand) v = (v % 32) * (int64_t(1) << 32) + v / 32;
+      std::sort(cand.begin(), cand.end());
+      for (size_t a = 0; a < cand.size();) {
+        const int64_t sh = cand[a] >> 32;
+        const int32_t first = int32_t(cand[a] & 0xffffffff);
+        size_t b = a + 1;
+        while (b < cand.size() && (cand[b] >> 32) == sh && int32_t(cand[b] & 0xffffffff) == first + int32_t(b - a)) ++b;
+        hm.sfRunFirst.push_back(first);
+        hm.sfRunEnd.push_back(first + int32_t(b - a));
+        hm.sfRunShift.push_back(uint8_t(sh));
+        a = b;
+      }
+      hm.sfRunOff[t + 1] = int32_t(hm.sfRunFirst.size());
     }
     // inverse lists: source element -> (target, shift), targets ascending
     hm.sfSrcOff.assign(ne + 1, 0);

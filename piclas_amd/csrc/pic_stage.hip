@@ -441,28 +441,60 @@ __global__ __launch_bounds__(kBlock) void k_seg_rank(int nElems, const int32_t *
   if (lane == 0) stayCnt[w] = stay;
 }
 
-// exclusive scan of stayCnt+arrCnt into newStart[0..nElems], single workgroup
+// exclusive scan of stayCnt+arrCnt into newStart[0..nElems], single
+// workgroup: tiles of 4096 counts, 4 consecutive per thread (one 16-byte
+// load each), wave scans, wave totals through LDS, the next tile's loads in
+// flight during the current tile's scan.
 __global__ __launch_bounds__(1024) void k_scan(int nElems, const int32_t *stayCnt, const int32_t *arrCnt, int32_t *newStart) {
-  __shared__ int32_t part[1024];
-  const int t = threadIdx.x;
-  const int per = (nElems + 1023) / 1024;
-  const int a = t * per, b = min(nElems, a + per);
-  int s = 0;
-  for (int e = a; e < b; ++e) s += stayCnt[e] + arrCnt[e];
-  part[t] = s;
-  __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {
-    int v = t >= off ? part[t - off] : 0;
+  __shared__ int32_t wsum[1024 / kWave];
+  const int t = threadIdx.x, lane = lane_id(), w = t / kWave;
+  auto load4 = [&](int base, int4 &a, int4 &b) {
+    const int e = base + 4 * t;
+    if (e + 3 < nElems) {
+      a = *reinterpret_cast<const int4 *>(stayCnt + e);
+      b = *reinterpret_cast<const int4 *>(arrCnt + e);
+    } else {
+      a = make_int4(e < nElems ? stayCnt[e] : 0, e + 1 < nElems ? stayCnt[e + 1] : 0,
+                    e + 2 < nElems ? stayCnt[e + 2] : 0, 0);
+      b = make_int4(e < nElems ? arrCnt[e] : 0, e + 1 < nElems ? arrCnt[e + 1] : 0,
+                    e + 2 < nElems ? arrCnt[e + 2] : 0, 0);
+    }
+  };
+  int carry = 0;
+  int4 sa, sb;
+  load4(0, sa, sb);
+  for (int base = 0; base < nElems; base += 4096) {
+    const int v0 = sa.x + sb.x, v1 = sa.y + sb.y, v2 = sa.z + sb.z, v3 = sa.w + sb.w;
+    if (base + 4096 < nElems) load4(base + 4096, sa, sb);
+    const int s = v0 + v1 + v2 + v3;
+    int x = s;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+      const int y = __shfl_up(x, off, kWave);
+      if (lane >= off) x += y;
+    }
+    if (lane == kWave - 1) wsum[w] = x;
     __syncthreads();
-    part[t] += v;
+    int before = 0, tot = 0;
+#pragma unroll
+    for (int k = 0; k < 1024 / kWave; ++k) {
+      const int ws = wsum[k];
+      before += k < w ? ws : 0;
+      tot += ws;
+    }
     __syncthreads();
+    const int e = base + 4 * t;
+    int run = carry + before + x - s;
+    if (e < nElems) newStart[e] = run;
+    run += v0;
+    if (e + 1 < nElems) newStart[e + 1] = run;
+    run += v1;
+    if (e + 2 < nElems) newStart[e + 2] = run;
+    run += v2;
+    if (e + 3 < nElems) newStart[e + 3] = run;
+    carry += tot;
   }
-  int run = t > 0 ? part[t - 1] : 0;
-  for (int e = a; e < b; ++e) {
-    newStart[e] = run;
-    run += stayCnt[e] + arrCnt[e];
-  }
-  if (t == 1023) newStart[nElems] = part[1023];
+  if (t == 0) newStart[nElems] = carry;
 }
 
 __global__ __launch_bounds__(kBlock) void k_dest(int64_t n, const int32_t *oldElemOf, const int32_t *newElem,
@@ -653,8 +685,15 @@ __global__ __launch_bounds__(kBlock) void k_cvwm_dof(DevMesh M, const int32_t *e
 struct SFArgs {
   const int32_t *off, *src;
   const uint8_t *shift;
+  const int32_t *roff, *rfirst, *rend;   // the lists as runs of consecutive source elements
+  const uint8_t *rshift;
   double sh[27][3];
   double r_sf, r2_sf, r2_sf_inv, w_sf;
+  double fsp[8];   // per species: ChargeIC * MacroParticleFactor * w_sf
+  const double4 *rec;   // per sorted particle: (x, y, z, 0), (f vx, f vy, f vz, f)
+  double r_inv;         // 1 / r_sf
+  int nspec;
+  int cartesian;        // every element's Gauss points form an axis-aligned tensor grid
   int alpha;
   int linear;   // NGeo == 1: the barycentre cull is implied by the DOF-box test
 };
@@ -791,19 +830,35 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
 // A_b[i][k]; B: lane 16k+4b+j holds B_b[k][j]; D: lane 16i+4b+j holds
 // D_b[i][j]) puts particle k = lane/16 and DOF 16Q + lane%16 on each lane for
 // the operands, and DOF 16Q + 4((lane/4)%4) + lane/16, component lane%4 on
-// the accumulator.
-#ifndef SF_GROUP_UNROLL
-#define SF_GROUP_UNROLL 1
-#endif
-template <int ALPHA>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE_MX))) void k_sf_gather_mfma(DevMesh M, DevParts P, const int32_t *start, SFArgs A, double *PS) {
+// the accumulator.  The wave is one target element; the segment of every
+// (source element, shift) of its list is walked in chunks of 64 particle
+// records.
+// Deposition records of the sorted particles for the shape-function
+// gathers: (x, y, z, 0), (f vx, f vy, f vz, f) with f = ChargeIC *
+// MacroParticleFactor * w_sf of the particle's species, so that a gather
+// reads one 64-byte record per particle and target.
+__global__ __launch_bounds__(kBlock) void k_sf_records(DevParts P, const int32_t *count, SFArgs A, double4 *rec) {
+  const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (i >= *count) return;
+  const double f = A.fsp[P.spec[i]];
+  rec[2 * i] = make_double4(P.x[i], P.y[i], P.z[i], 0.0);
+  rec[2 * i + 1] = make_double4(P.vx[i] * f, P.vy[i] * f, P.vz[i] * f, f);
+}
+
+template <int ALPHA, bool LINEAR>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE_MX))) void k_sf_gather_mfma(DevMesh M, const int32_t *start, SFArgs A, double *PS) {
 #pragma clang fp contract(fast)
   constexpr int nd = 64;
-  __shared__ double lds[kBlock / kWave][kWave + 4][8];
+  // Per wave a ring of 128 staged records (x, y, z relative to the target
+  // barycentre, |x|^2/R^2, f vx, f vy, f vz, f): particles that pass the cull
+  // are appended, and groups of 4 are consumed once 64 are pending, so groups
+  // span source chunks and only the very last group is padded.
+  constexpr int kRing = 128;
+  __shared__ double lds[kBlock / kWave][kRing][8];
   const int wib = threadIdx.x / kWave;
   const int lane = lane_id();
   const int lb = xcd_block(blockIdx.x, gridDim.x);
-  const int t = lb * (kBlock / kWave) + wib;
+  const int t = __builtin_amdgcn_readfirstlane(lb * (kBlock / kWave) + wib);
   if (t >= M.nElems) return;
   const double bx = M.bary[t * 3], by = M.bary[t * 3 + 1], bz = M.bary[t * 3 + 2];
   const double R2i0 = A.r2_sf_inv;
@@ -832,67 +887,208 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE_MX))
   const double R2box = A.r2_sf * (1.0 + 1e-12);
   double (*L)[8] = lds[wib];
   const int kp = lane >> 4, jc = lane & 3;
-  for (int q = A.off[t]; q < A.off[t + 1]; ++q) {
-    const int s = A.src[q];
-    const int si = A.shift[q];
-    const double shx = A.sh[si][0], shy = A.sh[si][1], shz = A.sh[si][2];
-    const int s0 = start[s], s1 = start[s + 1];
-    for (int base = s0; base < s1; base += kWave) {
-      const int i = base + lane;
-      bool pass = false;
-      double px = 0, py = 0, pz = 0, f = 0, vx = 0, vy = 0, vz = 0;
-      if (i < s1) {
-        const int sp = P.spec[i];
-        const double q0 = M.charge[sp];
-        px = (P.x[i] + shx) - bx; py = (P.y[i] + shy) - by; pz = (P.z[i] + shz) - bz;
+  // groups read 64 consecutive ring records from a multiple of 64 (no wrap)
+  auto group = [&](const double (*Lh)[8], int g) {
+    const double *rec = Lh[4 * g + kp];
+    const double qx = rec[0], qy = rec[1], qz = rec[2], qq = rec[3];
+    const double bval = rec[4 + jc];
+#pragma unroll
+    for (int Q = 0; Q < 4; ++Q) {
+      const double S = fmax(mx[Q] * qx + (my[Q] * qy + (mz[Q] * qz + (cd[Q] - qq))), 0.0);
+      double S1 = S * S;
+      if (ALPHA == 0) {
+        for (int ex = 3; ex <= A.alpha; ++ex) S1 = S * S1;
+      } else {
+#pragma unroll
+        for (int ex = 3; ex <= ALPHA; ++ex) S1 = S * S1;
+      }
+      acc[Q] = __builtin_amdgcn_mfma_f64_4x4x4f64(S1, bval, acc[Q], 0, 0, 0);
+    }
+  };
+  int head = 0, tail = 0;   // ring positions in records (head multiple of 64)
+  // The target's list, as runs of consecutive source elements with one
+  // shift (contiguous particle segments), is read 64 runs at a time, one per
+  // lane (segment bounds and shift); the walk takes them with readlane.
+  const int q0 = A.roff[t], qend = A.roff[t + 1];
+  for (int qb = q0; qb < qend; qb += kWave) {
+    int ms0 = 0, ms1 = 0, msi = 0;
+    if (qb + lane < qend) {
+      ms0 = start[A.rfirst[qb + lane]]; ms1 = start[A.rend[qb + lane]]; msi = A.rshift[qb + lane];
+    }
+    const int jn = min(kWave, qend - qb);
+    for (int j = 0; j < jn; ++j) {
+      const int s0 = __builtin_amdgcn_readlane(ms0, j), s1 = __builtin_amdgcn_readlane(ms1, j);
+      const int si = __builtin_amdgcn_readlane(msi, j);
+      // shifted position (GetPartPosShifted) relative to the target barycentre
+      const double ox = A.sh[si][0] - bx, oy = A.sh[si][1] - by, oz = A.sh[si][2] - bz;
+      for (int base = s0; base < s1; base += kWave) {
+        const int i = base + lane;
+        const bool valid = i < s1;
+        const size_t ic = size_t(valid ? i : s1 - 1);
+        const double4 ra = A.rec[2 * ic], rb = A.rec[2 * ic + 1];
+        const double px = ra.x + ox, py = ra.y + oy, pz = ra.z + oz;
+        // the exact test that some DOF of the target lies within r_sf, and on
+        // curved elements the reference's element cull |x - bary| > r_sf + ElemRadius
         const double gx = fmax(0.0, fmax(blo[0] - px, px - bhi[0]));
         const double gy = fmax(0.0, fmax(blo[1] - py, py - bhi[1]));
         const double gz = fmax(0.0, fmax(blo[2] - pz, pz - bhi[2]));
-        pass = (fabs(q0) > 0.0) && (gx * gx + gy * gy + gz * gz <= R2box) &&
-               (A.linear || !(sqrt(px * px + py * py + pz * pz) > thr));
-        f = q0 * M.mpf[sp] * A.w_sf;
-        vx = P.vx[i]; vy = P.vy[i]; vz = P.vz[i];
-      }
-      const unsigned long long m = __ballot(pass);
-      const int cnt = __popcll(m);
-      if (cnt == 0) continue;
-      if (pass) {
-        const int slot = __popcll(m & ((1ull << lane) - 1ull));
-        L[slot][0] = px; L[slot][1] = py; L[slot][2] = pz;
-        L[slot][3] = R2i0 * (px * px + py * py + pz * pz);
-        L[slot][4] = vx * f; L[slot][5] = vy * f; L[slot][6] = vz * f; L[slot][7] = f;
-      }
-      // pad the last group of 4 with weightless records
-      const int ng = (cnt + 3) >> 2;
-      if (lane >= cnt && lane < 4 * ng) {
-#pragma unroll
-        for (int c = 0; c < 8; ++c) L[lane][c] = 0.0;
-      }
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-#if SF_GROUP_UNROLL > 1
-#pragma unroll SF_GROUP_UNROLL
-#endif
-      for (int g = 0; g < ng; ++g) {
-        const double *rec = L[4 * g + kp];
-        const double qx = rec[0], qy = rec[1], qz = rec[2], qq = rec[3];
-        const double bval = rec[4 + jc];
-#pragma unroll
-        for (int Q = 0; Q < 4; ++Q) {
-          const double S = fmax(mx[Q] * qx + (my[Q] * qy + (mz[Q] * qz + (cd[Q] - qq))), 0.0);
-          double S1 = S * S;
-          if (ALPHA == 0) {
-            for (int ex = 3; ex <= A.alpha; ++ex) S1 = S * S1;
-          } else {
-#pragma unroll
-            for (int ex = 3; ex <= ALPHA; ++ex) S1 = S * S1;
-          }
-          acc[Q] = __builtin_amdgcn_mfma_f64_4x4x4f64(S1, bval, acc[Q], 0, 0, 0);
+        bool pass = valid && (rb.w != 0.0) && (gx * gx + gy * gy + gz * gz <= R2box);
+        // (on linear elements every DOF lies within ElemRadius of the
+        // barycentre, so the box test implies the cull)
+        if (!LINEAR) pass = pass && !(sqrt(px * px + py * py + pz * pz) > thr);
+        const unsigned long long m = __ballot(pass);
+        if (m == 0) continue;
+        if (pass) {
+          double *r = L[(tail + __popcll(m & ((1ull << lane) - 1ull))) & (kRing - 1)];
+          r[0] = px; r[1] = py; r[2] = pz;
+          r[3] = R2i0 * (px * px + py * py + pz * pz);
+          r[4] = rb.x; r[5] = rb.y; r[6] = rb.z; r[7] = rb.w;
         }
+        tail += __popcll(m);
+        if (tail - head < kWave) continue;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        for (int g = 0; g < kWave / 4; ++g) group(L + (head & (kRing - 1)), g);
+        head += kWave;
+        __builtin_amdgcn_wave_barrier();
       }
-      __builtin_amdgcn_wave_barrier();
     }
+  }
+  // the rest, with the last group padded by weightless records
+  const int rest = tail - head;
+  if (rest > 0) {
+    const int ng = (rest + 3) >> 2;
+    if (lane >= rest && lane < 4 * ng) {
+      double *r = L[(head + lane) & (kRing - 1)];
+#pragma unroll
+      for (int c = 0; c < 8; ++c) r[c] = 0.0;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (int g = 0; g < ng; ++g) group(L + (head & (kRing - 1)), g);
+  }
+  const int dofLo = 4 * ((lane >> 2) & 3) + (lane >> 4);
+#pragma unroll
+  for (int Q = 0; Q < 4; ++Q) PS[((size_t)t * nd + 16 * Q + dofLo) * 4 + jc] = acc[Q];
+}
+
+// The same gather for targets whose Gauss points form an axis-aligned tensor
+// grid (Cartesian elements): with u = x/r_sf, S = 1 - (u_i - pu)^2 -
+// (v_j - pv)^2 - (w_k - pw)^2 splits into a per-(i,j) part, formed once per
+// lane and group of 4 particles, and the per-particle (w_k - pw)^2, formed
+// once at staging; a pair then costs a subtraction, the clamp and the square.
+// Lane and accumulator layout as k_sf_gather_mfma, with quarter Q = k and
+// lane%16 = i + 4j.
+template <int ALPHA>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE_MX))) void k_sf_gather_cart(DevMesh M, const int32_t *start, SFArgs A, double *PS) {
+#pragma clang fp contract(fast)
+  constexpr int nd = 64;
+  // ring of 128 staged records: pu, pv, (w_k - pw)^2 for k = 0..3, f vx, f vy, f vz, f
+  constexpr int kRing = 128, kRec = 10;
+  __shared__ double lds[kBlock / kWave][kRing][kRec];
+  const int wib = threadIdx.x / kWave;
+  const int lane = lane_id();
+  const int lb = xcd_block(blockIdx.x, gridDim.x);
+  const int t = __builtin_amdgcn_readfirstlane(lb * (kBlock / kWave) + wib);
+  if (t >= M.nElems) return;
+  const double bx = M.bary[t * 3], by = M.bary[t * 3 + 1], bz = M.bary[t * 3 + 2];
+  const double Ri = A.r_inv;
+  const double *xg = M.xGPe + (size_t)t * nd * 3;
+  // scaled DOF coordinates relative to the barycentre: u_i = x(i,0,0), v_j = y(0,j,0), w_k = z(0,0,k)
+  double u[4], v[4], w[4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    u[a] = (xg[a * 3 + 0] - bx) * Ri;
+    v[a] = (xg[4 * a * 3 + 1] - by) * Ri;
+    w[a] = (xg[16 * a * 3 + 2] - bz) * Ri;
+  }
+  const double ulo = fmin(fmin(u[0], u[1]), fmin(u[2], u[3])), uhi = fmax(fmax(u[0], u[1]), fmax(u[2], u[3]));
+  const double vlo = fmin(fmin(v[0], v[1]), fmin(v[2], v[3])), vhi = fmax(fmax(v[0], v[1]), fmax(v[2], v[3]));
+  const double wlo = fmin(fmin(w[0], w[1]), fmin(w[2], w[3])), whi = fmax(fmax(w[0], w[1]), fmax(w[2], w[3]));
+  const double ui = (xg[(lane & 3) * 3 + 0] - bx) * Ri;
+  const double vj = (xg[4 * ((lane >> 2) & 3) * 3 + 1] - by) * Ri;
+  const double box2 = 1.0 + 1e-12;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  double (*L)[kRec] = lds[wib];
+  const int kp = lane >> 4, jc = lane & 3;
+  auto group = [&](const double (*Lh)[kRec], int g) {
+    const double *rec = Lh[4 * g + kp];
+    const double a = ui - rec[0], b = vj - rec[1];
+    const double W = fma(-b, b, fma(-a, a, 1.0));
+    const double bval = rec[6 + jc];
+#pragma unroll
+    for (int Q = 0; Q < 4; ++Q) {
+      const double S = fmax(W - rec[2 + Q], 0.0);
+      double S1 = S * S;
+      if (ALPHA == 0) {
+        for (int ex = 3; ex <= A.alpha; ++ex) S1 = S * S1;
+      } else {
+#pragma unroll
+        for (int ex = 3; ex <= ALPHA; ++ex) S1 = S * S1;
+      }
+      acc[Q] = __builtin_amdgcn_mfma_f64_4x4x4f64(S1, bval, acc[Q], 0, 0, 0);
+    }
+  };
+  int head = 0, tail = 0;   // ring positions in records (head multiple of 64)
+  const int q0 = A.roff[t], qend = A.roff[t + 1];
+  for (int qb = q0; qb < qend; qb += kWave) {
+    int ms0 = 0, ms1 = 0, msi = 0;
+    if (qb + lane < qend) {
+      ms0 = start[A.rfirst[qb + lane]]; ms1 = start[A.rend[qb + lane]]; msi = A.rshift[qb + lane];
+    }
+    const int jn = min(kWave, qend - qb);
+    for (int j = 0; j < jn; ++j) {
+      const int s0 = __builtin_amdgcn_readlane(ms0, j), s1 = __builtin_amdgcn_readlane(ms1, j);
+      const int si = __builtin_amdgcn_readlane(msi, j);
+      const double ox = A.sh[si][0] - bx, oy = A.sh[si][1] - by, oz = A.sh[si][2] - bz;
+      for (int base = s0; base < s1; base += kWave) {
+        const int i = base + lane;
+        const bool valid = i < s1;
+        const size_t ic = size_t(valid ? i : s1 - 1);
+        const double4 ra = A.rec[2 * ic], rb = A.rec[2 * ic + 1];
+        // shifted position (GetPartPosShifted) relative to the barycentre, scaled
+        const double pu = (ra.x + ox) * Ri, pv = (ra.y + oy) * Ri, pw = (ra.z + oz) * Ri;
+        const double gx = fmax(0.0, fmax(ulo - pu, pu - uhi));
+        const double gy = fmax(0.0, fmax(vlo - pv, pv - vhi));
+        const double gz = fmax(0.0, fmax(wlo - pw, pw - whi));
+        const bool pass = valid && (rb.w != 0.0) && (gx * gx + gy * gy + gz * gz <= box2);
+        const unsigned long long m = __ballot(pass);
+        if (m == 0) continue;
+        if (pass) {
+          double *r = L[(tail + __popcll(m & ((1ull << lane) - 1ull))) & (kRing - 1)];
+          const double c0 = w[0] - pw, c1 = w[1] - pw, c2 = w[2] - pw, c3 = w[3] - pw;
+          r[0] = pu; r[1] = pv; r[2] = c0 * c0; r[3] = c1 * c1; r[4] = c2 * c2; r[5] = c3 * c3;
+          r[6] = rb.x; r[7] = rb.y; r[8] = rb.z; r[9] = rb.w;
+        }
+        tail += __popcll(m);
+        if (tail - head < kWave) continue;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        for (int g = 0; g < kWave / 4; ++g) group(L + (head & (kRing - 1)), g);
+        head += kWave;
+        __builtin_amdgcn_wave_barrier();
+      }
+    }
+  }
+  // the rest, with the last group padded by weightless records far outside
+  const int rest = tail - head;
+  if (rest > 0) {
+    const int ng = (rest + 3) >> 2;
+    if (lane >= rest && lane < 4 * ng) {
+      double *r = L[(head + lane) & (kRing - 1)];
+      r[0] = 0.0; r[1] = 0.0;
+#pragma unroll
+      for (int c = 2; c < 6; ++c) r[c] = 4.0;
+#pragma unroll
+      for (int c = 6; c < kRec; ++c) r[c] = 0.0;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (int g = 0; g < ng; ++g) group(L + (head & (kRing - 1)), g);
   }
   const int dofLo = 4 * ((lane >> 2) & 3) + (lane >> 4);
 #pragma unroll
@@ -1137,7 +1333,7 @@ struct pic_handle {
   bool stagedValid = false;   // LastPartPos/FieldAtParticle refer to the current order
   std::vector<std::pair<int64_t, std::vector<double>>> unplaced;  // particles no element holds at set-up
   // statistics
-  hipEvent_t ev[8] = {};
+  hipEvent_t ev[9] = {};
   double msDeposit = 0, msPushTrack = 0, msSort = 0, msDepositMain = 0;
   int nDepositLaunches = 0;
   int64_t nSteps = 0;
@@ -1224,11 +1420,25 @@ int set_device(pic_handle *h) {
 
 void drop_graph(pic_handle *h);
 
+// the dominant kernel run_deposition launches (PIC_KERNEL_*)
+int deposit_kernel(const pic_handle *h) {
+  if (h->par.deposition == PIC_DEPO_NONE) return PIC_KERNEL_NONE;
+  if (h->par.deposition == PIC_DEPO_CVWM) return PIC_KERNEL_CVWM;
+  if (h->hm.N == 3 && h->par.precision == PIC_PREC_MIXED) return PIC_KERNEL_SF_MIXED;
+  if (h->hm.N == 3 && h->sfMatrix) return h->sf.cartesian ? PIC_KERNEL_SF_CARTESIAN : PIC_KERNEL_SF_MATRIX;
+  return PIC_KERNEL_SF_VALU;
+}
+
 // Deposition() on the current sorted particles
 int run_deposition(pic_handle *h, bool timed) {
   const int ne = h->hm.nElems, N = h->hm.N;
   const size_t nps = size_t(ne) * h->hm.nDof * 4;
   if (timed) HIPCHK(hipEventRecord(h->ev[0], h->stream));
+  // ev[8] marks the start of the dominant kernel (after the record pass of the
+  // matrix-core gather)
+  const bool records = h->par.deposition == PIC_DEPO_SF && N == 3 && h->sfMatrix &&
+                       h->par.precision != PIC_PREC_MIXED;
+  if (timed && !records) HIPCHK(hipEventRecord(h->ev[8], h->stream));
   if (h->par.deposition == PIC_DEPO_NONE) {
     hipLaunchKernelGGL(k_fill_f64, dim3(grid_for(int64_t(nps))), dim3(kBlock), 0, h->stream, h->PS, int64_t(nps));
   } else if (h->par.deposition == PIC_DEPO_CVWM) {
@@ -1246,10 +1456,22 @@ int run_deposition(pic_handle *h, bool timed) {
       else
         hipLaunchKernelGGL((k_sf_gather_mixed<0>), dim3(nblk), dim3(kBlock), 0, h->stream, h->M, h->P, h->start, h->sf, h->PS);
     } else if (N == 3 && h->sfMatrix) {
-      if (h->sf.alpha == 2)
-        hipLaunchKernelGGL((k_sf_gather_mfma<2>), dim3(nblk), dim3(kBlock), 0, h->stream, h->M, h->P, h->start, h->sf, h->PS);
+      hipLaunchKernelGGL(k_sf_records, dim3(grid_for(std::max<int64_t>(h->n, 1))), dim3(kBlock), 0, h->stream, h->P,
+                         h->start + ne, h->sf, const_cast<double4 *>(h->sf.rec));
+      if (timed) HIPCHK(hipEventRecord(h->ev[8], h->stream));
+      const bool lin = h->sf.linear != 0;
+      if (h->sf.cartesian && h->sf.alpha == 2)
+        hipLaunchKernelGGL((k_sf_gather_cart<2>), dim3(nblk), dim3(kBlock), 0, h->stream, h->M, h->start, h->sf, h->PS);
+      else if (h->sf.cartesian)
+        hipLaunchKernelGGL((k_sf_gather_cart<0>), dim3(nblk), dim3(kBlock), 0, h->stream, h->M, h->start, h->sf, h->PS);
+      else if (h->sf.alpha == 2 && lin)
+        hipLaunchKernelGGL((k_sf_gather_mfma<2, true>), dim3(nblk), dim3(kBlock), 0, h->stream, h->M, h->start, h->sf, h->PS);
+      else if (h->sf.alpha == 2)
+        hipLaunchKernelGGL((k_sf_gather_mfma<2, false>), dim3(nblk), dim3(kBlock), 0, h->stream, h->M, h->start, h->sf, h->PS);
+      else if (lin)
+        hipLaunchKernelGGL((k_sf_gather_mfma<0, true>), dim3(nblk), dim3(kBlock), 0, h->stream, h->M, h->start, h->sf, h->PS);
       else
-        hipLaunchKernelGGL((k_sf_gather_mfma<0>), dim3(nblk), dim3(kBlock), 0, h->stream, h->M, h->P, h->start, h->sf, h->PS);
+        hipLaunchKernelGGL((k_sf_gather_mfma<0, false>), dim3(nblk), dim3(kBlock), 0, h->stream, h->M, h->start, h->sf, h->PS);
     } else if (h->sf.alpha == 2) {
       LAUNCH_N2(k_sf_gather, N, 2, dim3(nblk), h->M, h->P, h->start, h->sf, h->PS);
     } else {
@@ -1421,6 +1643,10 @@ int pic_create(const pic_mesh_desc *mesh, const pic_params *par, pic_handle **ou
     rc |= upload(h, const_cast<int32_t **>(&h->sf.off), hm.sfOff);
     rc |= upload(h, const_cast<int32_t **>(&h->sf.src), hm.sfSrc);
     rc |= upload(h, const_cast<uint8_t **>(&h->sf.shift), hm.sfShift);
+    rc |= upload(h, const_cast<int32_t **>(&h->sf.roff), hm.sfRunOff);
+    rc |= upload(h, const_cast<int32_t **>(&h->sf.rfirst), hm.sfRunFirst);
+    rc |= upload(h, const_cast<int32_t **>(&h->sf.rend), hm.sfRunEnd);
+    rc |= upload(h, const_cast<uint8_t **>(&h->sf.rshift), hm.sfRunShift);
     for (int s = 0; s < 27; ++s)
       for (int c = 0; c < 3; ++c) h->sf.sh[s][c] = hm.shift[s][c];
     const double a = par->sf_alpha, r = par->sf_radius;
@@ -1432,6 +1658,26 @@ int pic_create(const pic_mesh_desc *mesh, const pic_params *par, pic_handle **ou
     h->sf.w_sf = 1. / (2. * BetaFac * a + 2 * BetaFac) * (a + 1.) / (M_PI * (r * r * r));
     h->sf.alpha = par->sf_alpha;
     h->sf.linear = mesh->NGeo == 1;
+    h->sf.nspec = par->nSpecies;
+    h->sf.r_inv = 1. / r;
+    // axis-aligned tensor grids of Gauss points: x depends on i only, y on j, z on k
+    if (hm.N == 3) {
+      bool cart = true;
+      for (int e = 0; e < hm.nElems && cart; ++e) {
+        const double *xg = &hm.xGPe[size_t(e) * 64 * 3];
+        const double tol = 1e-12 * hm.radius[e];
+        for (int k = 0; k < 4 && cart; ++k)
+          for (int j = 0; j < 4 && cart; ++j)
+            for (int i = 0; i < 4 && cart; ++i) {
+              const double *p = xg + (i + 4 * j + 16 * k) * 3;
+              cart = std::fabs(p[0] - xg[i * 3]) <= tol && std::fabs(p[1] - xg[4 * j * 3 + 1]) <= tol &&
+                     std::fabs(p[2] - xg[16 * k * 3 + 2]) <= tol;
+            }
+      }
+      h->sf.cartesian = cart;
+      if (const char *k = std::getenv("PIC_SF_CARTESIAN")) h->sf.cartesian = cart && std::strcmp(k, "0") != 0;
+    }
+    for (int k = 0; k < 8; ++k) h->sf.fsp[k] = k < par->nSpecies ? par->charge[k] * par->mpf[k] * h->sf.w_sf : 0.0;
     // PIC_SF_KERNEL=valu selects the all-VALU gather (for comparisons)
     if (const char *k = std::getenv("PIC_SF_KERNEL")) h->sfMatrix = std::strcmp(k, "valu") != 0;
   }
@@ -1447,6 +1693,11 @@ int pic_create(const pic_mesh_desc *mesh, const pic_params *par, pic_handle **ou
   if (par->tracking == PIC_TRACK_REFMAPPING) {
     rc |= dalloc(h, &h->P.rx, cap); rc |= dalloc(h, &h->P.ry, cap); rc |= dalloc(h, &h->P.rz, cap);
     rc |= dalloc(h, &h->Q.rx, cap); rc |= dalloc(h, &h->Q.ry, cap); rc |= dalloc(h, &h->Q.rz, cap);
+  }
+  if (par->deposition == PIC_DEPO_SF && hm.N == 3) {
+    double4 *rec = nullptr;
+    rc |= dalloc(h, &rec, size_t(cap) * 2);
+    h->sf.rec = rec;
   }
   rc |= dalloc(h, &h->start, hm.nElems + 1);
   rc |= dalloc(h, &h->startNew, hm.nElems + 1);
@@ -1609,7 +1860,7 @@ int pic_deposition(pic_handle *h) {
   if (int rc = run_deposition(h, true)) return rc;
   HIPCHK(hipStreamSynchronize(h->stream));
   h->msDeposit += elapsed(h->ev[0], h->ev[2]);
-  if (h->par.deposition != PIC_DEPO_NONE) { h->msDepositMain += elapsed(h->ev[0], h->ev[1]); h->nDepositLaunches++; }
+  if (h->par.deposition != PIC_DEPO_NONE) { h->msDepositMain += elapsed(h->ev[8], h->ev[1]); h->nDepositLaunches++; }
   return PIC_OK;
 }
 
@@ -1759,7 +2010,7 @@ int pic_time_step(pic_handle *h, double dt, int32_t nsteps) {
     HIPCHK(hipEventRecord(h->ev[5], h->stream));
     HIPCHK(hipEventSynchronize(h->ev[5]));
     h->msDeposit += elapsed(h->ev[0], h->ev[2]);
-    if (h->par.deposition != PIC_DEPO_NONE) { h->msDepositMain += elapsed(h->ev[0], h->ev[1]); h->nDepositLaunches++; }
+    if (h->par.deposition != PIC_DEPO_NONE) { h->msDepositMain += elapsed(h->ev[8], h->ev[1]); h->nDepositLaunches++; }
     h->msPushTrack += elapsed(h->ev[3], h->ev[4]);
     h->msSort += elapsed(h->ev[4], h->ev[5]);
     h->nSteps++;
@@ -1938,6 +2189,7 @@ int pic_get_stats(pic_handle *h, pic_stats *st) {
   st->ms_sort = h->msSort;
   st->n_deposit_launches = h->nDepositLaunches;
   st->ms_deposit_main = h->msDepositMain;
+  st->deposit_kernel = deposit_kernel(h);
   return PIC_OK;
 }
 

@@ -71,6 +71,11 @@ struct HostMesh {
   std::vector<int32_t> sfOff;      // [nElems+1]
   std::vector<int32_t> sfSrc;      // source element
   std::vector<uint8_t> sfShift;    // index into shift table (27 combos)
+  // the same lists as runs of consecutive source elements with one shift,
+  // whose particle segments are contiguous in the element-sorted arrays
+  std::vector<int32_t> sfRunOff;   // [nElems+1]
+  std::vector<int32_t> sfRunFirst, sfRunEnd;   // elements [first, end)
+  std::vector<uint8_t> sfRunShift;
   // the same pairs by source element: source -> (target element, shift)
   std::vector<int32_t> sfSrcOff, sfTgt;
   std::vector<uint8_t> sfTgtShift;

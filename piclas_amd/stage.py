@@ -23,6 +23,8 @@ LIB_PATH = os.environ.get("PICLAS_AMD_LIB", os.path.join(HERE, "libpicstage.so")
 TRACK_REFMAPPING, TRACK_TRIA = 1, 3
 DEPO_NONE, DEPO_CVWM, DEPO_SF = 0, 1, 2
 PREC_FP64, PREC_MIXED = 0, 1
+# dominant deposition kernel (pic_stats.deposit_kernel)
+KERNEL_NONE, KERNEL_CVWM, KERNEL_SF_VALU, KERNEL_SF_MATRIX, KERNEL_SF_CARTESIAN, KERNEL_SF_MIXED = range(6)
 
 ELEMENTARY_CHARGE = 1.602176634e-19   # globals_vars.f90:54
 ELECTRON_MASS = 9.1093826e-31         # globals_vars.f90:57
@@ -53,7 +55,8 @@ class PicParams(ctypes.Structure):
 class PicStats(ctypes.Structure):
     _fields_ = [("n_particles", ctypes.c_int64), ("n_lost", ctypes.c_int64), ("n_steps", ctypes.c_int64),
                 ("ms_deposit", ctypes.c_double), ("ms_push_track", ctypes.c_double), ("ms_sort", ctypes.c_double),
-                ("n_deposit_launches", ctypes.c_int32), ("ms_deposit_main", ctypes.c_double)]
+                ("n_deposit_launches", ctypes.c_int32), ("ms_deposit_main", ctypes.c_double),
+                ("deposit_kernel", ctypes.c_int32)]
 
 
 class PicTuning(ctypes.Structure):
