@@ -49,6 +49,10 @@ struct DevMesh {
   double gmin[3], gmax[3];
   int dirPV[3];
   double xiCLg[3], wBaryCLg[3];  // CL basis of NGeo
+  // affine elements (parallelepipeds): corner-map centre and inverse
+  // Jacobian, [e][12] = c(3), A^-1 (3x3 row-major); null unless every
+  // element is affine
+  const double *affine;
   // mixed precision: E and the Gauss basis in FP32
   const float *E32;         // [e][nDof][3]
   float xGP32[8], wBaryGP32[8];
@@ -249,9 +253,12 @@ __device__ inline void ref_newton_start(const DevMesh &M, int e, const double x[
   }
 }
 
-// EvaluateFieldAtRefPos with NVar=3 (interpolation/eval_xyz.f90:167-295)
+// EvaluateFieldAtRefPos with NVar=3 (interpolation/eval_xyz.f90:167-295),
+// summed by factors: U = sum_k L_k sum_j L_j sum_i L_i U_ijk (the reference
+// sums sum_ijk U_ijk L_i (L_j L_k); the two agree to round-off).
 template <int N>
 __device__ inline void eval_field(const DevMesh &M, int e, const double xi[3], double U[3]) {
+#pragma clang fp contract(fast)
   constexpr int n1 = N + 1;
   double L[3][n1];
 #pragma unroll
@@ -259,18 +266,26 @@ __device__ inline void eval_field(const DevMesh &M, int e, const double xi[3], d
   const double *Ue = M.E + (size_t)e * n1 * n1 * n1 * 3;
   U[0] = U[1] = U[2] = 0.0;
 #pragma unroll 1
-  for (int k = 0; k < n1; ++k)
-#pragma unroll 1
+  for (int k = 0; k < n1; ++k) {
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+#pragma unroll
     for (int j = 0; j < n1; ++j) {
-      const double Lez = L[1][j] * L[2][k];
+      double t0 = 0.0, t1 = 0.0, t2 = 0.0;
 #pragma unroll
       for (int i = 0; i < n1; ++i) {
         const double *u = Ue + ((k * n1 + j) * n1 + i) * 3;
-        U[0] = U[0] + u[0] * L[0][i] * Lez;
-        U[1] = U[1] + u[1] * L[0][i] * Lez;
-        U[2] = U[2] + u[2] * L[0][i] * Lez;
+        t0 += u[0] * L[0][i];
+        t1 += u[1] * L[0][i];
+        t2 += u[2] * L[0][i];
       }
+      s0 += t0 * L[1][j];
+      s1 += t1 * L[1][j];
+      s2 += t2 * L[1][j];
     }
+    U[0] += s0 * L[2][k];
+    U[1] += s1 * L[2][k];
+    U[2] += s2 * L[2][k];
+  }
 }
 
 // GetEMFieldDW (E only), particles/pic/interpolation/pic_interpolation_tools.f90:458-572.
@@ -370,13 +385,29 @@ __device__ inline void eval_field_f32(const DevMesh &M, int e, const double xi[3
 // InterpolateFieldToSingleParticle for the 508 stepper
 // (particles/pic/interpolation/pic_interpolation.f90:325-369):
 // external field + E(xi) with xi from GetPositionInRefElem.
-template <int N, bool MIXED = false>
+// On affine elements (parallelepipeds) the corner map is X(xi) = c + A xi and
+// RefElemNewton converges to xi = A^-1 (x - c), which is formed directly
+// (the two agree to round-off).  Used when every element is affine.
+__device__ inline void affine_xi(const DevMesh &M, int e, const double x[3], double xi[3]) {
+  const double *a = M.affine + (size_t)e * 12;
+  const double d0 = x[0] - a[0], d1 = x[1] - a[1], d2 = x[2] - a[2];
+  xi[0] = a[3] * d0 + a[4] * d1 + a[5] * d2;
+  xi[1] = a[6] * d0 + a[7] * d1 + a[8] * d2;
+  xi[2] = a[9] * d0 + a[10] * d1 + a[11] * d2;
+}
+
+template <int N, bool MIXED = false, bool AFFINE = false>
 __device__ inline void field_at_particle(const DevMesh &M, int e, const double x[3], double F[6]) {
 #pragma unroll
   for (int c = 0; c < 6; ++c) F[c] = M.ext[c];
   double xi[3];
-  ref_newton_start<N>(M, e, x, xi);
-  const bool ok = ref_elem_newton(M, e, x, xi, true);
+  bool ok = true;
+  if (AFFINE) {
+    affine_xi(M, e, x, xi);
+  } else {
+    ref_newton_start<N>(M, e, x, xi);
+    ok = ref_elem_newton(M, e, x, xi, true);
+  }
   double U[3];
   if (!ok && M.deposition == 1) em_field_dw<N>(M, e, x, U);
   else if (MIXED) eval_field_f32<N>(M, e, xi, U);

@@ -60,7 +60,7 @@ __device__ inline int lane_id() { return __lane_id(); }
 // ------------------------------------------------------------------ kernels
 
 // Interpolation only (InterpolateFieldToParticle): F[i] = ext + E(x_i)
-template <int N, bool MIXED = false>
+template <int N, bool MIXED = false, bool AFFINE = false>
 __global__ __launch_bounds__(kBlock) void k_interp(DevMesh M, DevParts P, int64_t n) {
   const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
   if (i >= n) return;
@@ -69,7 +69,7 @@ __global__ __launch_bounds__(kBlock) void k_interp(DevMesh M, DevParts P, int64_
   const int sp = P.spec[i];
   if (e >= 0 && fabs(M.charge[sp]) > 0.0) {
     const double x[3] = {P.x[i], P.y[i], P.z[i]};
-    field_at_particle<N, MIXED>(M, e, x, F);
+    field_at_particle<N, MIXED, AFFINE>(M, e, x, F);
   }
 #pragma unroll
   for (int c = 0; c < 6; ++c) P.F[i * 6 + c] = F[c];
@@ -166,7 +166,7 @@ struct Movers {
   double *last;     // [cap][3] LastPartPos
 };
 
-template <int N, bool MIXED = false>
+template <int N, bool MIXED = false, bool AFFINE = false>
 __global__ __launch_bounds__(kBlock) void k_push_inside(DevMesh M, DevParts P, int64_t n, double dt, int32_t *newElem,
                                                         Movers mv, const int32_t *nDev) {
   const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
@@ -183,7 +183,7 @@ __global__ __launch_bounds__(kBlock) void k_push_inside(DevMesh M, DevParts P, i
       last[0] = x[0]; last[1] = x[1]; last[2] = x[2];
       const int sp = P.spec[i];
       double F[6] = {0, 0, 0, 0, 0, 0};
-      if (fabs(M.charge[sp]) > 0.0) field_at_particle<N, MIXED>(M, e0, x, F);
+      if (fabs(M.charge[sp]) > 0.0) field_at_particle<N, MIXED, AFFINE>(M, e0, x, F);
       boris_push(M.charge[sp], M.mass[sp], P.isNew[i] != 0, F, dt, x, v);
       P.x[i] = x[0]; P.y[i] = x[1]; P.z[i] = x[2];
       P.vx[i] = v[0]; P.vy[i] = v[1]; P.vz[i] = v[2];
@@ -1596,6 +1596,49 @@ int pic_create(const pic_mesh_desc *mesh, const pic_params *par, pic_handle **ou
   rc |= upload(h, const_cast<double **>(&M.xiBasis), hm.xiBasis);
   rc |= upload(h, const_cast<double **>(&M.slen), hm.slen);
   rc |= upload(h, const_cast<double **>(&M.xGPe), hm.xGPe);
+  {
+    // affine elements: the trilinear corner map has no xi.eta, xi.zeta,
+    // eta.zeta or xi.eta.zeta terms (to round-off)
+    std::vector<double> aff(size_t(hm.nElems) * 12, 0.0);
+    bool all = hm.NGeo == 1 && hm.nElems > 0;
+    for (int e = 0; e < hm.nElems && all; ++e) {
+      all = false;
+      const double *X = &hm.corner[size_t(e) * 24];
+      double co[8][3] = {};   // coefficients of 1, xi, eta, xi.eta, zeta, xi.zeta, eta.zeta, xi.eta.zeta
+      for (int t = 0; t < 8; ++t)
+        for (int m = 0; m < 8; ++m) {
+          double sg = 1.0;
+          for (int d = 0; d < 3; ++d)
+            if ((m >> d) & 1) sg *= ((t >> d) & 1) ? 1.0 : -1.0;
+          for (int c = 0; c < 3; ++c) co[m][c] += 0.125 * sg * X[t * 3 + c];
+        }
+      double h1 = 0, hn = 0;
+      for (int c = 0; c < 3; ++c) {
+        h1 = std::max({h1, std::fabs(co[1][c]), std::fabs(co[2][c]), std::fabs(co[4][c])});
+        hn = std::max({hn, std::fabs(co[3][c]), std::fabs(co[5][c]), std::fabs(co[6][c]), std::fabs(co[7][c])});
+      }
+      if (!(hn <= 1e-14 * h1)) continue;
+      const double A[3][3] = {{co[1][0], co[2][0], co[4][0]}, {co[1][1], co[2][1], co[4][1]}, {co[1][2], co[2][2], co[4][2]}};
+      const double det = A[0][0] * (A[1][1] * A[2][2] - A[1][2] * A[2][1]) - A[0][1] * (A[1][0] * A[2][2] - A[1][2] * A[2][0]) +
+                         A[0][2] * (A[1][0] * A[2][1] - A[1][1] * A[2][0]);
+      if (!(std::fabs(det) > 0)) continue;
+      double *a = &aff[size_t(e) * 12];
+      a[0] = co[0][0]; a[1] = co[0][1]; a[2] = co[0][2];
+      a[3] = (A[1][1] * A[2][2] - A[1][2] * A[2][1]) / det;
+      a[4] = (A[0][2] * A[2][1] - A[0][1] * A[2][2]) / det;
+      a[5] = (A[0][1] * A[1][2] - A[0][2] * A[1][1]) / det;
+      a[6] = (A[1][2] * A[2][0] - A[1][0] * A[2][2]) / det;
+      a[7] = (A[0][0] * A[2][2] - A[0][2] * A[2][0]) / det;
+      a[8] = (A[0][2] * A[1][0] - A[0][0] * A[1][2]) / det;
+      a[9] = (A[1][0] * A[2][1] - A[1][1] * A[2][0]) / det;
+      a[10] = (A[0][1] * A[2][0] - A[0][0] * A[2][1]) / det;
+      a[11] = (A[0][0] * A[1][1] - A[0][1] * A[1][0]) / det;
+      all = true;
+    }
+    if (const char *k = std::getenv("PIC_AFFINE")) all = all && std::strcmp(k, "0") != 0;
+    M.affine = nullptr;
+    if (all) rc |= upload(h, const_cast<double **>(&M.affine), aff);
+  }
   rc |= dalloc(h, &h->E, size_t(hm.nElems) * hm.nDof * 3);
   M.E = h->E;
   M.xiCL[0] = -1.0; M.xiCL[1] = 1.0;      // linear CL basis of the corner map
@@ -1874,7 +1917,13 @@ int pic_interpolate_field_to_particle(pic_handle *h) {
       if (mixed) hipLaunchKernelGGL((k_rm_interp<3, true>), dim3(grid_for(h->n)), dim3(kBlock), 0, h->stream, h->M, h->P, h->n);
       else { LAUNCH_N(k_rm_interp, h->hm.N, dim3(grid_for(h->n)), h->M, h->P, h->n); }
     } else {
-      if (mixed) hipLaunchKernelGGL((k_interp<3, true>), dim3(grid_for(h->n)), dim3(kBlock), 0, h->stream, h->M, h->P, h->n);
+      const bool aff = h->M.affine != nullptr && h->hm.N == 3;
+      if (mixed && aff)
+        hipLaunchKernelGGL((k_interp<3, true, true>), dim3(grid_for(h->n)), dim3(kBlock), 0, h->stream, h->M, h->P, h->n);
+      else if (mixed)
+        hipLaunchKernelGGL((k_interp<3, true>), dim3(grid_for(h->n)), dim3(kBlock), 0, h->stream, h->M, h->P, h->n);
+      else if (aff)
+        hipLaunchKernelGGL((k_interp<3, false, true>), dim3(grid_for(h->n)), dim3(kBlock), 0, h->stream, h->M, h->P, h->n);
       else { LAUNCH_N(k_interp, h->hm.N, dim3(grid_for(h->n)), h->M, h->P, h->n); }
     }
   }
@@ -1944,9 +1993,17 @@ int enqueue_step(pic_handle *h, double dt, int64_t nGrid, bool timed) {
               h->insideById, h->stateById);
   } else {
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(kBlock), 0, h->stream, h->mv.count, int64_t(1), 0);
-    if (mixed)
+    // every element affine (N = 3): xi without Newton
+    const bool aff = h->M.affine != nullptr && h->hm.N == 3;
+    if (mixed && aff)
+      hipLaunchKernelGGL((k_push_inside<3, true, true>), grid, dim3(kBlock), 0, h->stream, h->M, h->P, nGrid, dt,
+                         h->newElem, h->mv, nDev);
+    else if (mixed)
       hipLaunchKernelGGL((k_push_inside<3, true>), grid, dim3(kBlock), 0, h->stream, h->M, h->P, nGrid, dt, h->newElem,
                          h->mv, nDev);
+    else if (aff)
+      hipLaunchKernelGGL((k_push_inside<3, false, true>), grid, dim3(kBlock), 0, h->stream, h->M, h->P, nGrid, dt,
+                         h->newElem, h->mv, nDev);
     else { LAUNCH_N(k_push_inside, h->hm.N, grid, h->M, h->P, nGrid, dt, h->newElem, h->mv, nDev); }
     hipLaunchKernelGGL(k_track_movers, dim3(1024), dim3(kBlock), 0, h->stream, h->M, h->P, h->newElem, h->mv,
                        h->lostCount, h->insideById, h->stateById);
