@@ -1099,96 +1099,153 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE_MX))
 // FP32, summed in FP32 on the matrix cores (v_mfma_f32_4x4x1, 16 blocks of
 // 4x4: lane 4b+i gives A_b[i] = S^alpha of DOF lane, lane 4b+j gives
 // B_b[j] = f (v,1)_j of the particle, lane 4b+j holds D_b[i][j] in register
-// i), then added into FP64 at the end of every source chunk of <= 64
-// particles.  Positions relative to the target barycentre are formed in FP64.
+// i), then added into FP64 after every batch of 64 staged particles.
+// Positions relative to the target barycentre and the cull are formed in
+// FP64 from the same records and runs as the FP64 gathers.  Records in LDS
+// (FP32): general elements px, py, pz, |p|^2/R^2, f (v,1); axis-aligned
+// tensor grids (CART) 1 - (u_i - pu)^2 - (v_j - pv)^2 for the 16 (i,j),
+// (w_k - pw)^2 for k = 0..3 and f (v,1), so that a pair costs a
+// subtraction, the clamp and the square.
 typedef float v4f __attribute__((ext_vector_type(4)));
-template <int ALPHA>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) void k_sf_gather_mixed(DevMesh M, DevParts P, const int32_t *start, SFArgs A, double *PS) {
+template <int ALPHA, bool CART>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE_MX))) void k_sf_gather_mixed(DevMesh M, const int32_t *start, SFArgs A, double *PS) {
 #pragma clang fp contract(fast)
-  constexpr int nd = 64;
-  __shared__ float lds[kBlock / kWave][kWave][8];
+  constexpr int nd = 64, kRing = 128;
+  constexpr int kRec = CART ? 24 : 8;
+  __shared__ float lds[kBlock / kWave][kRing][kRec];
   const int wib = threadIdx.x / kWave;
   const int lane = lane_id();
   const int lb = xcd_block(blockIdx.x, gridDim.x);
-  const int t = lb * (kBlock / kWave) + wib;
+  const int t = __builtin_amdgcn_readfirstlane(lb * (kBlock / kWave) + wib);
   if (t >= M.nElems) return;
   const double bx = M.bary[t * 3], by = M.bary[t * 3 + 1], bz = M.bary[t * 3 + 2];
-  const double R2i0 = A.r2_sf_inv;
-  float mx, my, mz, cd;
+  const double R2i0 = A.r2_sf_inv, Ri = A.r_inv;
+  const double *xg = M.xGPe + (size_t)t * nd * 3;
+  // general elements: S = cd - qq + m.q for DOF lane
+  float mx = 0.f, my = 0.f, mz = 0.f, cd = 0.f;
+  // tensor grids: scaled DOF coordinates u_i, v_j, w_k
+  double u[4], v[4], w[4];
   double blo[3], bhi[3];
-  {
-    const double *xg = M.xGPe + ((size_t)t * nd + lane) * 3;
-    const double dx = xg[0] - bx, dy = xg[1] - by, dz = xg[2] - bz;
+  if (CART) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      u[q] = (xg[q * 3 + 0] - bx) * Ri;
+      v[q] = (xg[4 * q * 3 + 1] - by) * Ri;
+      w[q] = (xg[16 * q * 3 + 2] - bz) * Ri;
+    }
+    blo[0] = fmin(fmin(u[0], u[1]), fmin(u[2], u[3])); bhi[0] = fmax(fmax(u[0], u[1]), fmax(u[2], u[3]));
+    blo[1] = fmin(fmin(v[0], v[1]), fmin(v[2], v[3])); bhi[1] = fmax(fmax(v[0], v[1]), fmax(v[2], v[3]));
+    blo[2] = fmin(fmin(w[0], w[1]), fmin(w[2], w[3])); bhi[2] = fmax(fmax(w[0], w[1]), fmax(w[2], w[3]));
+  } else {
+    const double dx = xg[lane * 3] - bx, dy = xg[lane * 3 + 1] - by, dz = xg[lane * 3 + 2] - bz;
     mx = float(2.0 * R2i0 * dx); my = float(2.0 * R2i0 * dy); mz = float(2.0 * R2i0 * dz);
     cd = float(1.0 - R2i0 * (dx * dx + dy * dy + dz * dz));
     blo[0] = bhi[0] = dx; blo[1] = bhi[1] = dy; blo[2] = bhi[2] = dz;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        blo[c] = fmin(blo[c], __shfl_xor(blo[c], off, kWave));
+        bhi[c] = fmax(bhi[c], __shfl_xor(bhi[c], off, kWave));
+      }
   }
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1)
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      blo[a] = fmin(blo[a], __shfl_xor(blo[a], off, kWave));
-      bhi[a] = fmax(bhi[a], __shfl_xor(bhi[a], off, kWave));
-    }
   const double thr = A.r_sf + M.radius[t];
-  const double R2box = A.r2_sf * (1.0 + 1e-12);
-  float (*L)[8] = lds[wib];
-  const int jc = lane & 3;
+  // box test threshold: in scaled coordinates 1, else R^2
+  const double box2 = (CART ? 1.0 : A.r2_sf) * (1.0 + 1e-12);
+  float (*L)[kRec] = lds[wib];
+  const int jc = lane & 3, wq = lane & 15, zk = lane >> 4;
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int q = A.off[t]; q < A.off[t + 1]; ++q) {
-    const int s = A.src[q];
-    const int si = A.shift[q];
-    const double shx = A.sh[si][0], shy = A.sh[si][1], shz = A.sh[si][2];
-    const int s0 = start[s], s1 = start[s + 1];
-    for (int base = s0; base < s1; base += kWave) {
-      const int i = base + lane;
-      bool pass = false;
-      double px = 0, py = 0, pz = 0, f = 0, vx = 0, vy = 0, vz = 0;
-      if (i < s1) {
-        const int sp = P.spec[i];
-        const double q0 = M.charge[sp];
-        px = (P.x[i] + shx) - bx; py = (P.y[i] + shy) - by; pz = (P.z[i] + shz) - bz;
+  // the batch of n staged records from ring position h0 (no wrap)
+  auto batch = [&](int h0, int n) {
+    const float (*Lh)[kRec] = L + (h0 & (kRing - 1));
+    v4f c = {0.f, 0.f, 0.f, 0.f};
+    for (int p = 0; p < n; ++p) {
+      const float *r = Lh[p];
+      float S;
+      if (CART) {
+        S = fmaxf(r[wq] - r[16 + zk], 0.f);
+      } else {
+        S = fmaxf(mx * r[0] + (my * r[1] + (mz * r[2] + (cd - r[3]))), 0.f);
+      }
+      float S1 = S * S;
+      if (ALPHA == 0) {
+        for (int ex = 3; ex <= A.alpha; ++ex) S1 = S * S1;
+      } else {
+#pragma unroll
+        for (int ex = 3; ex <= ALPHA; ++ex) S1 = S * S1;
+      }
+      c = __builtin_amdgcn_mfma_f32_4x4x1f32(S1, r[(CART ? 20 : 4) + jc], c, 0, 0, 0);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) acc[q] += double(c[q]);
+  };
+  int head = 0, tail = 0;   // ring positions in records (head multiple of 64)
+  const int q0 = A.roff[t], qend = A.roff[t + 1];
+  for (int qb = q0; qb < qend; qb += kWave) {
+    int ms0 = 0, ms1 = 0, msi = 0;
+    if (qb + lane < qend) {
+      ms0 = start[A.rfirst[qb + lane]]; ms1 = start[A.rend[qb + lane]]; msi = A.rshift[qb + lane];
+    }
+    const int jn = min(kWave, qend - qb);
+    for (int j = 0; j < jn; ++j) {
+      const int s0 = __builtin_amdgcn_readlane(ms0, j), s1 = __builtin_amdgcn_readlane(ms1, j);
+      const int si = __builtin_amdgcn_readlane(msi, j);
+      const double ox = A.sh[si][0] - bx, oy = A.sh[si][1] - by, oz = A.sh[si][2] - bz;
+      for (int base = s0; base < s1; base += kWave) {
+        const int i = base + lane;
+        const bool valid = i < s1;
+        const size_t ic = size_t(valid ? i : s1 - 1);
+        const double4 ra = A.rec[2 * ic], rb = A.rec[2 * ic + 1];
+        double px = ra.x + ox, py = ra.y + oy, pz = ra.z + oz;
+        if (CART) { px *= Ri; py *= Ri; pz *= Ri; }
         const double gx = fmax(0.0, fmax(blo[0] - px, px - bhi[0]));
         const double gy = fmax(0.0, fmax(blo[1] - py, py - bhi[1]));
         const double gz = fmax(0.0, fmax(blo[2] - pz, pz - bhi[2]));
-        pass = (fabs(q0) > 0.0) && (gx * gx + gy * gy + gz * gz <= R2box) &&
-               (A.linear || !(sqrt(px * px + py * py + pz * pz) > thr));
-        f = q0 * M.mpf[sp] * A.w_sf;
-        vx = P.vx[i]; vy = P.vy[i]; vz = P.vz[i];
-      }
-      const unsigned long long m = __ballot(pass);
-      const int cnt = __popcll(m);
-      if (cnt == 0) continue;
-      if (pass) {
-        const int slot = __popcll(m & ((1ull << lane) - 1ull));
-        L[slot][0] = float(px); L[slot][1] = float(py); L[slot][2] = float(pz);
-        L[slot][3] = float(R2i0 * (px * px + py * py + pz * pz));
-        L[slot][4] = float(vx * f); L[slot][5] = float(vy * f); L[slot][6] = float(vz * f); L[slot][7] = float(f);
-      }
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      v4f c = {0.f, 0.f, 0.f, 0.f};
-      for (int j = 0; j < cnt; ++j) {
-        const float qx = L[j][0], qy = L[j][1], qz = L[j][2], qq = L[j][3];
-        const float bval = L[j][4 + jc];
-        const float S = fmaxf(mx * qx + (my * qy + (mz * qz + (cd - qq))), 0.f);
-        float S1 = S * S;
-        if (ALPHA == 0) {
-          for (int ex = 3; ex <= A.alpha; ++ex) S1 = S * S1;
-        } else {
+        bool pass = valid && (rb.w != 0.0) && (gx * gx + gy * gy + gz * gz <= box2);
+        if (!CART && !A.linear) pass = pass && !(sqrt(px * px + py * py + pz * pz) > thr);
+        const unsigned long long m = __ballot(pass);
+        if (m == 0) continue;
+        if (pass) {
+          float *r = L[(tail + __popcll(m & ((1ull << lane) - 1ull))) & (kRing - 1)];
+          if (CART) {
+            float X[4], Y[4];
 #pragma unroll
-          for (int ex = 3; ex <= ALPHA; ++ex) S1 = S * S1;
+            for (int q = 0; q < 4; ++q) {
+              const float a = float(u[q] - px), b = float(v[q] - py), c = float(w[q] - pz);
+              X[q] = 1.f - a * a;
+              Y[q] = b * b;
+              r[16 + q] = c * c;
+            }
+#pragma unroll
+            for (int jj = 0; jj < 4; ++jj)
+#pragma unroll
+              for (int ii = 0; ii < 4; ++ii) r[ii + 4 * jj] = X[ii] - Y[jj];
+            r[20] = float(rb.x); r[21] = float(rb.y); r[22] = float(rb.z); r[23] = float(rb.w);
+          } else {
+            r[0] = float(px); r[1] = float(py); r[2] = float(pz);
+            r[3] = float(R2i0 * (px * px + py * py + pz * pz));
+            r[4] = float(rb.x); r[5] = float(rb.y); r[6] = float(rb.z); r[7] = float(rb.w);
+          }
         }
-        c = __builtin_amdgcn_mfma_f32_4x4x1f32(S1, bval, c, 0, 0, 0);
+        tail += __popcll(m);
+        if (tail - head < kWave) continue;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        batch(head, kWave);
+        head += kWave;
+        __builtin_amdgcn_wave_barrier();
       }
-#pragma unroll
-      for (int r = 0; r < 4; ++r) acc[r] += double(c[r]);
-      __builtin_amdgcn_wave_barrier();
     }
   }
+  if (tail > head) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    batch(head, tail - head);
+  }
 #pragma unroll
-  for (int r = 0; r < 4; ++r) PS[((size_t)t * nd + 4 * (lane >> 2) + r) * 4 + jc] = acc[r];
+  for (int q = 0; q < 4; ++q) PS[((size_t)t * nd + 4 * (lane >> 2) + q) * 4 + jc] = acc[q];
 }
 
 // ---- analysis integrals ------------------------------------------------------
@@ -1436,8 +1493,8 @@ int run_deposition(pic_handle *h, bool timed) {
   if (timed) HIPCHK(hipEventRecord(h->ev[0], h->stream));
   // ev[8] marks the start of the dominant kernel (after the record pass of the
   // matrix-core gather)
-  const bool records = h->par.deposition == PIC_DEPO_SF && N == 3 && h->sfMatrix &&
-                       h->par.precision != PIC_PREC_MIXED;
+  const bool mixed = h->par.precision == PIC_PREC_MIXED;
+  const bool records = h->par.deposition == PIC_DEPO_SF && N == 3 && (h->sfMatrix || mixed);
   if (timed && !records) HIPCHK(hipEventRecord(h->ev[8], h->stream));
   if (h->par.deposition == PIC_DEPO_NONE) {
     hipLaunchKernelGGL(k_fill_f64, dim3(grid_for(int64_t(nps))), dim3(kBlock), 0, h->stream, h->PS, int64_t(nps));
@@ -1450,15 +1507,22 @@ int run_deposition(pic_handle *h, bool timed) {
   } else {
     const int waves = kBlock / kWave;
     const unsigned nblk = xcd_grid((ne + waves - 1) / waves);
-    if (N == 3 && h->par.precision == PIC_PREC_MIXED) {
-      if (h->sf.alpha == 2)
-        hipLaunchKernelGGL((k_sf_gather_mixed<2>), dim3(nblk), dim3(kBlock), 0, h->stream, h->M, h->P, h->start, h->sf, h->PS);
-      else
-        hipLaunchKernelGGL((k_sf_gather_mixed<0>), dim3(nblk), dim3(kBlock), 0, h->stream, h->M, h->P, h->start, h->sf, h->PS);
-    } else if (N == 3 && h->sfMatrix) {
+    if (records) {
       hipLaunchKernelGGL(k_sf_records, dim3(grid_for(std::max<int64_t>(h->n, 1))), dim3(kBlock), 0, h->stream, h->P,
                          h->start + ne, h->sf, const_cast<double4 *>(h->sf.rec));
       if (timed) HIPCHK(hipEventRecord(h->ev[8], h->stream));
+    }
+    if (N == 3 && mixed) {
+      const bool cart = h->sf.cartesian != 0;
+      if (h->sf.alpha == 2 && cart)
+        hipLaunchKernelGGL((k_sf_gather_mixed<2, true>), dim3(nblk), dim3(kBlock), 0, h->stream, h->M, h->start, h->sf, h->PS);
+      else if (h->sf.alpha == 2)
+        hipLaunchKernelGGL((k_sf_gather_mixed<2, false>), dim3(nblk), dim3(kBlock), 0, h->stream, h->M, h->start, h->sf, h->PS);
+      else if (cart)
+        hipLaunchKernelGGL((k_sf_gather_mixed<0, true>), dim3(nblk), dim3(kBlock), 0, h->stream, h->M, h->start, h->sf, h->PS);
+      else
+        hipLaunchKernelGGL((k_sf_gather_mixed<0, false>), dim3(nblk), dim3(kBlock), 0, h->stream, h->M, h->start, h->sf, h->PS);
+    } else if (N == 3 && h->sfMatrix) {
       const bool lin = h->sf.linear != 0;
       if (h->sf.cartesian && h->sf.alpha == 2)
         hipLaunchKernelGGL((k_sf_gather_cart<2>), dim3(nblk), dim3(kBlock), 0, h->stream, h->M, h->start, h->sf, h->PS);
