@@ -6,6 +6,12 @@
       HBM bytes per dispatch of every kernel from FETCH_SIZE / WRITE_SIZE
       (KiB per dispatch), each scaled by the factor that makes the same
       counter read the known size of a plain device copy (hbm_copy_probe).
+  python -m piclas_amd.profiling issue <counters.csv> [kernel-substring ...]
+      per dispatch: instruction counts (SQ_INSTS_VALU, _MFMA, _LDS, _SALU,
+      _VMEM_RD), SQ_VALU_MFMA_BUSY_CYCLES and the share of the SIMD cycles
+      that VALU issue (4 cycles per wave instruction on a 16-lane SIMD) and
+      the matrix cores take together; GRBM_GUI_ACTIVE counts the cycles of
+      each of the 8 XCDs (MI355X: 8 XCDs, 1024 SIMDs).
 
 Both print Markdown tables.
 """
@@ -70,9 +76,33 @@ def traffic(fetch_csv, write_csv, copy_fetch_csv, copy_write_csv, copy_bytes=1 <
     return "\n".join(lines)
 
 
+def issue(path, names=(), xcds=8, simds=1024):
+    agg, disp = OrderedDict(), OrderedDict()
+    for r in csv.DictReader(open(path)):
+        k = short(r["Kernel_Name"])
+        if names and not any(n in k for n in names):
+            continue
+        agg.setdefault(k, {}).setdefault(r["Counter_Name"], 0.0)
+        agg[k][r["Counter_Name"]] += float(r["Counter_Value"])
+        disp.setdefault(k, set()).add(r["Dispatch_Id"])
+    lines = ["| kernel | dispatches | cycles per XCD | VALU | MFMA | LDS | SALU | VMEM rd | MFMA busy cycles | "
+             "VALU+MFMA share of SIMD cycles |", "|---|---|---|---|---|---|---|---|---|---|"]
+    for k, c in agg.items():
+        n = len(disp[k])
+        g = lambda name: c.get(name, 0.0) / n
+        cyc = g("GRBM_GUI_ACTIVE") / xcds
+        share = (4.0 * g("SQ_INSTS_VALU") + g("SQ_VALU_MFMA_BUSY_CYCLES")) / (cyc * simds) if cyc > 0 else 0.0
+        lines.append("| `%s` | %d | %.4g | %.4g | %.4g | %.4g | %.4g | %.4g | %.4g | %.2f |" % (
+            k, n, cyc, g("SQ_INSTS_VALU"), g("SQ_INSTS_MFMA"), g("SQ_INSTS_LDS"), g("SQ_INSTS_SALU"),
+            g("SQ_INSTS_VMEM_RD"), g("SQ_VALU_MFMA_BUSY_CYCLES"), share))
+    return "\n".join(lines)
+
+
 def main(argv):
     if len(argv) >= 2 and argv[0] == "stats":
         print(kernel_stats(argv[1]))
+    elif len(argv) >= 2 and argv[0] == "issue":
+        print(issue(argv[1], tuple(argv[2:])))
     elif len(argv) >= 5 and argv[0] == "traffic":
         nb = int(argv[5]) if len(argv) > 5 else (1 << 30)
         print(traffic(argv[1], argv[2], argv[3], argv[4], nb))
