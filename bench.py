@@ -141,8 +141,12 @@ def main():
     ap.add_argument("--autotune", action="store_true", help="pick graph replay / shape-function kernel by measurement")
     ap.add_argument("--cpu-fraction", type=float, default=1.0 / 16)
     ap.add_argument("--cpu-steps", type=int, default=3)
-    ap.add_argument("--workload", choices=["cube", "annulus"], default="cube",
-                    help="cube: 48^3 linear box (default); annulus: 24x96x24 curved NGeo=2, RefMapping, 30M particles")
+    ap.add_argument("--workload", choices=["cube", "annulus", "adjoint"], default="cube",
+                    help="cube: 48^3 linear box (default); annulus: 24x96x24 curved NGeo=2, RefMapping, 30M "
+                         "particles; adjoint: 16^3 perturbed plasma, 2M particles, K self-consistent steps and "
+                         "their adjoint")
+    ap.add_argument("--adjoint-steps", type=int, default=50, help="adjoint workload: K")
+    ap.add_argument("--adjoint-every", type=int, default=10, help="adjoint workload: checkpoint interval")
     ap.add_argument("--particles", type=int, default=30_000_000, help="annulus workload only")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -153,6 +157,8 @@ def main():
     from piclas_amd import stage, workloads as W
     if args.workload == "annulus":
         return bench_annulus(args)
+    if args.workload == "adjoint":
+        return bench_adjoint(args)
     w = W.thermal_cube(nx=args.nx, ppc=args.ppc)
     n = len(w.PartState)
     print("bench: %s, %d particles, %d warmup + %d timed steps" % (w.name, n, args.warmup, args.steps),
@@ -245,6 +251,57 @@ def bench_annulus(args):
                                  "interp_push_track": st["ms_push_track"] / args.steps,
                                  "sort": st["ms_sort"] / args.steps},
            "n_removed_or_lost": {"lost": st["n_lost"], "inside_after": st["n_particles"]}}
+    print(json.dumps(out), flush=True)
+
+
+def bench_adjoint(args):
+    """The differentiable workload at full size: periodic 16^3 box, N = 3,
+    2 M electrons with density n0 (1 + alpha cos 2 pi x), field closed by the
+    fixed spectral Poisson operator, loss = field energy after K
+    self-consistent steps, shape function alpha = 3.  Times the forward
+    alone and the forward with checkpoints plus the reverse sweep
+    (pic_adjoint), and checks dL/dalpha against central differences of the
+    forward (h = 1e-6)."""
+    from piclas_amd import adjoint as A, stage
+    K, every = args.adjoint_steps, args.adjoint_every
+    w, dxda = A.landau_box(n_el=16, n_part=2_000_000, steps=K)
+    n = len(w.PartState)
+    print("bench: %s, %d particles, K = %d, checkpoint every %d" % (w.name, n, K, every), file=sys.stderr, flush=True)
+    G = A.periodic_poisson_operator(w.mesh)
+    s = stage.PICStage(w.mesh, w.N, max_particles=n, deposition=w.deposition, sf_radius=w.sf_radius,
+                       sf_alpha=w.sf_alpha, charge=w.charge, mass=w.mass, mpf=w.mpf)
+    s.set_field_operator(G)
+
+    def forward(ws):
+        s.set_particles(ws.PartState, elem=ws.elem, species=ws.species)
+        s.synchronize()
+        t0 = time.perf_counter()
+        loss = s.sc_time_step(ws.dt, K)
+        return loss, time.perf_counter() - t0
+
+    forward(w)   # warm-up
+    loss, t_fwd = forward(w)
+    s.set_particles(w.PartState, elem=w.elem, species=w.species)
+    s.synchronize()
+    t0 = time.perf_counter()
+    loss_a, gx, gv = s.adjoint(w.dt, K, every)
+    t_adj = time.perf_counter() - t0
+    dLda = float((gx * dxda).sum())
+    h = 1e-6
+    wp, _ = A.landau_box(n_el=16, n_part=2_000_000, alpha=0.01 + h, steps=K)
+    wm, _ = A.landau_box(n_el=16, n_part=2_000_000, alpha=0.01 - h, steps=K)
+    fd = (forward(wp)[0] - forward(wm)[0]) / (2 * h)
+    s.close()
+    out = {"metric": "particle-steps/s of forward + adjoint (K self-consistent steps, checkpointed reverse sweep)",
+           "value": n * K / t_adj, "unit": "particle-steps/s", "n_gpus": 1, "steps": K, "warmup": 1,
+           "ms_per_step": t_adj / K * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64", "data": "synthetic",
+           "config": {"workload": w.name, "elements": w.mesh.nElems, "N": w.N, "particles": n,
+                      "deposition": "shape_function R=1.5h alpha=3", "tracking": "triatracking",
+                      "checkpoint_every": every, "parallelism": "1 GPU"},
+           "forward_ms_per_step": t_fwd / K * 1e3, "adjoint_over_forward": t_adj / t_fwd,
+           "loss": loss_a, "loss_forward": loss,
+           "dL_dalpha": {"adjoint": dLda, "central_difference": fd, "rel_err": abs(dLda - fd) / abs(fd)}}
     print(json.dumps(out), flush=True)
 
 
