@@ -347,3 +347,45 @@ def test_graph_replay_matches_eager(case):
         assert counts[-1] < len(w.PartState)
     sg.close()
     se.close()
+
+
+@pytest.mark.parametrize("env", ["PIC_SF_CARTESIAN", "PIC_AFFINE"])
+def test_general_kernels_on_cartesian_mesh(oracle_mod, env, monkeypatch):
+    """The general shape-function gather and the Newton interpolation, forced
+    on a Cartesian mesh (where the tensor-grid gather and the affine
+    reference coordinates would run), against the oracle."""
+    monkeypatch.setenv(env, "0")
+    w = W.thermal_cube(nx=6, ppc=24, steps=10)
+    s, o = run_pair(oracle_mod, w, 10)
+    if env == "PIC_SF_CARTESIAN":
+        assert s.stats()["deposit_kernel"] == stage.KERNEL_SF_MATRIX
+    compare(s, o, 1.0, mesh=w.mesh)
+    s.close()
+
+
+def deformed_box(w, amp=0.12):
+    """Displace the nodes of a generated periodic unit box by a smooth
+    periodic field that keeps the boundary planes (faces become bilinear, no
+    element is affine, no Gauss-point grid is a tensor grid)."""
+    X = w.mesh.NodeCoords
+    x, y, z = X[:, 0].copy(), X[:, 1].copy(), X[:, 2].copy()
+    a = amp / w.mesh.shape[0]
+    s, c = (lambda u: np.sin(2 * np.pi * u)), (lambda u: np.cos(2 * np.pi * u))
+    X[:, 0] = x + a * s(x) * c(y) * c(z)
+    X[:, 1] = y + a * c(x) * s(y) * c(z)
+    X[:, 2] = z + a * c(x) * c(y) * s(z)
+    w.elem = np.zeros(len(w.PartState), np.int32)   # located by the stage and the oracle
+    return w
+
+
+@pytest.mark.parametrize("depo", [stage.DEPO_SF, stage.DEPO_CVWM])
+def test_deformed_box_parity(oracle_mod, depo):
+    """Linear hexahedra with bilinear faces (ConcaveElemSide, Newton on the
+    trilinear map, the general gather): device localization and 10 steps
+    against the oracle, ownership bit-exact."""
+    w = deformed_box(W.thermal_cube(nx=5, ppc=30, deposition=depo, steps=10))
+    s, o = run_pair(oracle_mod, w, 10, sf_radius=w.sf_radius if depo == stage.DEPO_SF else 0.0)
+    if depo == stage.DEPO_SF:
+        assert s.stats()["deposit_kernel"] == stage.KERNEL_SF_MATRIX
+    assert compare(s, o, 1.0) == 0
+    s.close()
