@@ -676,6 +676,12 @@ __global__ __launch_bounds__(kBlock) void k_cvwm_dof(DevMesh M, const int32_t *e
 }
 
 // ---- shape_function ---------------------------------------------------------
+// S = 1 - r^2/R^2 <= 1, so max(S, 0) = min(max(S, 0), 1): written this way the
+// clamp becomes the clamp bit of the instruction that forms S (one operation
+// less per particle-DOF pair).
+__device__ inline double clamp01(double x) { return fmin(fmax(x, 0.0), 1.0); }
+__device__ inline float clamp01f(float x) { return fminf(fmaxf(x, 0.f), 1.f); }
+
 // Gather form of depoChargeOnDOFsSF (pic_depo_shapefunction_tools.f90:292-422):
 // one wave per target element, one lane per DOF.  For every (source element,
 // periodic shift) of the target's list the wave streams the source segment,
@@ -790,7 +796,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
 #pragma unroll
         for (int c = 0; c < nch; ++c) {
           // radius2 <= r2_sf  <=>  S >= 0 (up to round-off, where S^alpha ~ 0)
-          const double S = fmax(mx[c] * qx + (my[c] * qy + (mz[c] * qz + (cd[c] - qq))), 0.0);
+          const double S = clamp01(mx[c] * qx + (my[c] * qy + (mz[c] * qz + (cd[c] - qq))));
           double S1 = S * S;
           if (ALPHA == 0) {
             for (int ex = 3; ex <= A.alpha; ++ex) S1 = S * S1;
@@ -894,7 +900,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE_MX))
     const double bval = rec[4 + jc];
 #pragma unroll
     for (int Q = 0; Q < 4; ++Q) {
-      const double S = fmax(mx[Q] * qx + (my[Q] * qy + (mz[Q] * qz + (cd[Q] - qq))), 0.0);
+      const double S = clamp01(mx[Q] * qx + (my[Q] * qy + (mz[Q] * qz + (cd[Q] - qq))));
       double S1 = S * S;
       if (ALPHA == 0) {
         for (int ex = 3; ex <= A.alpha; ++ex) S1 = S * S1;
@@ -1020,7 +1026,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE_MX))
     const double bval = rec[6 + jc];
 #pragma unroll
     for (int Q = 0; Q < 4; ++Q) {
-      const double S = fmax(W - rec[2 + Q], 0.0);
+      const double S = clamp01(W - rec[2 + Q]);
       double S1 = S * S;
       if (ALPHA == 0) {
         for (int ex = 3; ex <= A.alpha; ++ex) S1 = S * S1;
@@ -1163,9 +1169,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE_MX))
       const float *r = Lh[p];
       float S;
       if (CART) {
-        S = fmaxf(r[wq] - r[16 + zk], 0.f);
+        S = clamp01f(r[wq] - r[16 + zk]);
       } else {
-        S = fmaxf(mx * r[0] + (my * r[1] + (mz * r[2] + (cd - r[3]))), 0.f);
+        S = clamp01f(mx * r[0] + (my * r[1] + (mz * r[2] + (cd - r[3]))));
       }
       float S1 = S * S;
       if (ALPHA == 0) {
