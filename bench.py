@@ -108,6 +108,35 @@ def mixed_report(w, args, r64):
                     "into FP64; FP64 positions, Newton and face tests"}
 
 
+def exchange_timing(w, reps=10):
+    """The rho/E hand-over between the stage and a field solver: U_N%E from
+    device memory (a CUDA tensor, device-to-device) and from host memory, and
+    PartSource to host memory (the device path hands over a pointer)."""
+    import numpy as np
+    from piclas_amd import stage, workloads as W
+    s = stage.PICStage(w.mesh, w.N, max_particles=1, **W.stage_kwargs(w))
+    E_host = np.ascontiguousarray(w.E)
+    E_dev = stage.DeviceBuffer.from_array(E_host)
+    s.set_particles(w.PartState[:1], elem=w.elem[:1], species=w.species[:1])
+    s.deposition()
+
+    def timed(f):   # every call returns after its copies have finished
+        f()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            f()
+        return (time.perf_counter() - t0) / reps * 1e3
+
+    out = {"field_bytes": int(E_host.nbytes), "source_bytes": int(w.mesh.nElems * (w.N + 1) ** 3 * 4 * 8),
+           "set_field_device_ms": timed(lambda: s.set_field_device(E_dev.ptr)),
+           "set_field_host_ms": timed(lambda: s.set_field(E_host)),
+           "part_source_host_ms": timed(s.part_source),
+           "part_source_device": "pointer (pic_part_source_device), no copy"}
+    s.close()
+    E_dev.free()
+    return out
+
+
 def cpu_baseline(w, fraction, steps, threads):
     """The FP64 CPU oracle (the reference's algorithm restated in C, OpenMP)
     on a bounded sample of the same workload."""
@@ -202,6 +231,7 @@ def main():
            "roofline": roof}
     if not args.no_mixed:
         out["mixed"] = mixed_report(w, args, r64)
+    out["exchange"] = exchange_timing(w)
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(w, args.cpu_fraction, args.cpu_steps, threads)

@@ -112,6 +112,69 @@ def load_library(path=LIB_PATH):
     return _lib
 
 
+def _hip_runtime_paths():
+    """Paths of the HIP runtime libraries mapped into this process."""
+    try:
+        with open("/proc/self/maps") as f:
+            return {line.split()[-1] for line in f if "libamdhip64" in line}
+    except OSError:
+        return set()
+
+
+def _check_one_hip_runtime():
+    if len(_hip_runtime_paths()) > 1:
+        raise RuntimeError("two HIP runtimes are loaded (the stage library's and torch's): import torch before "
+                           "piclas_amd loads the stage library, or hand device memory over with DeviceBuffer")
+
+
+def _hip():
+    """The HIP runtime the stage library is linked against (already loaded)."""
+    load_library()
+    hip = ctypes.CDLL("libamdhip64.so.7")
+    hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+    hip.hipFree.argtypes = [ctypes.c_void_p]
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    return hip
+
+
+class DeviceBuffer:
+    """Device memory of the HIP runtime the stage uses, for handing U_N%E and
+    PartSource over without host copies (pic_set_field_device,
+    pic_part_source_device), e.g. to and from a field solver on the GPU."""
+    H2D, D2H = 1, 2
+
+    def __init__(self, nbytes):
+        self.hip = _hip()
+        p = ctypes.c_void_p()
+        if self.hip.hipMalloc(ctypes.byref(p), int(nbytes)) != 0:
+            raise StageError("hipMalloc of %d bytes failed" % nbytes)
+        self.ptr, self.nbytes = p.value, int(nbytes)
+
+    @classmethod
+    def from_array(cls, a):
+        a = np.ascontiguousarray(a, np.float64)
+        b = cls(a.nbytes)
+        if b.hip.hipMemcpy(ctypes.c_void_p(b.ptr), a.ctypes.data_as(ctypes.c_void_p), a.nbytes, cls.H2D) != 0:
+            raise StageError("hipMemcpy to the device failed")
+        return b
+
+    def free(self):
+        if self.ptr:
+            self.hip.hipFree(ctypes.c_void_p(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        self.free()
+
+
+def device_to_host(ptr, shape):
+    """float64 array of the given shape copied from device address ptr."""
+    out = np.empty(shape, np.float64)
+    if _hip().hipMemcpy(out.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(ptr), out.nbytes, DeviceBuffer.D2H) != 0:
+        raise StageError("hipMemcpy from the device failed")
+    return out
+
+
 class StageError(RuntimeError):
     pass
 
@@ -177,6 +240,45 @@ class PICStage:
         if E.size != self.nElems * self.nDof * 3:
             raise ValueError("E must have nElems*(N+1)^3*3 entries")
         _chk(load_library().pic_set_field(self.h, _p(E, _f64p)), "pic_set_field")
+
+    def set_field_device(self, ptr):
+        """U_N%E from device memory: ptr is the device address of
+        [nElems][(N+1)^3][3] float64 values (copied on the stage's stream)."""
+        _chk(load_library().pic_set_field_device(self.h, ctypes.c_void_p(int(ptr))), "pic_set_field_device")
+
+    def set_field_tensor(self, E):
+        """U_N%E from a contiguous float64 tensor on the stage's GPU, without a
+        host round trip; waits for the work torch has queued on its stream.
+        torch and the stage must share one HIP runtime: import torch before
+        this module loads the stage library."""
+        import torch
+        _check_one_hip_runtime()
+        if not (E.is_cuda and E.dtype == torch.float64 and E.is_contiguous()):
+            raise ValueError("E must be a contiguous float64 CUDA tensor")
+        if E.numel() != self.nElems * self.nDof * 3:
+            raise ValueError("E must have nElems*(N+1)^3*3 entries")
+        torch.cuda.current_stream(E.device).synchronize()
+        self.set_field_device(E.data_ptr())
+
+    def part_source_device(self):
+        """Device address of PartSource [nElems][(N+1)^3][4] (float64), valid
+        until the next deposition."""
+        out = ctypes.c_void_p()
+        _chk(load_library().pic_part_source_device(self.h, ctypes.byref(out)), "pic_part_source_device")
+        return out.value
+
+    def part_source_tensor(self, device=None):
+        """PartSource as a torch tensor viewing the stage's device memory (no
+        copy), shape (nElems, (N+1)^3, 4); valid until the next deposition."""
+        import torch
+        _check_one_hip_runtime()
+
+        class _View:
+            pass
+        v = _View()
+        v.__cuda_array_interface__ = {"shape": (self.nElems, self.nDof, 4), "typestr": "<f8",
+                                      "data": (self.part_source_device(), False), "version": 2, "strides": None}
+        return torch.as_tensor(v, device=device or torch.device("cuda", 0))
 
     def set_particles(self, PartState, elem=None, species=None, is_new=None):
         ps = np.ascontiguousarray(PartState, np.float64).reshape(-1, 6)

@@ -389,3 +389,33 @@ def test_deformed_box_parity(oracle_mod, depo):
         assert s.stats()["deposit_kernel"] == stage.KERNEL_SF_MATRIX
     assert compare(s, o, 1.0) == 0
     s.close()
+
+
+def test_device_resident_field_and_source():
+    """pic_set_field_device / pic_part_source_device: the field handed over
+    from device memory gives the same steps as the host hand-over
+    (bit-identical), and the source read at the device address equals the
+    host copy."""
+    w = W.thermal_cube(nx=5, ppc=20, steps=3)
+    kw = W.stage_kwargs(w)
+    E_dev = stage.DeviceBuffer.from_array(w.E)
+    res = []
+    for device_path in (False, True):
+        s = stage.PICStage(w.mesh, w.N, max_particles=len(w.PartState), **kw)
+        if device_path:
+            s.set_field_device(E_dev.ptr)
+        else:
+            s.set_field(w.E)
+        s.set_particles(w.PartState, elem=w.elem, species=w.species)
+        s.time_step(w.dt, 3)
+        s.deposition()
+        ps, el, ins = s.particles()
+        src = s.part_source()
+        if device_path:
+            dev = stage.device_to_host(s.part_source_device(), src.shape)
+            assert np.array_equal(dev, src)
+        res.append((ps, el, src))
+        s.close()
+    E_dev.free()
+    assert np.array_equal(res[0][0], res[1][0]) and np.array_equal(res[0][1], res[1][1])
+    assert np.array_equal(res[0][2], res[1][2])
