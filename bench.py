@@ -170,10 +170,11 @@ def main():
     ap.add_argument("--autotune", action="store_true", help="pick graph replay / shape-function kernel by measurement")
     ap.add_argument("--cpu-fraction", type=float, default=1.0 / 16)
     ap.add_argument("--cpu-steps", type=int, default=3)
-    ap.add_argument("--workload", choices=["cube", "annulus", "adjoint"], default="cube",
+    ap.add_argument("--workload", choices=["cube", "annulus", "adjoint", "wave"], default="cube",
                     help="cube: 48^3 linear box (default); annulus: 24x96x24 curved NGeo=2, RefMapping, 30M "
                          "particles; adjoint: 16^3 perturbed plasma, 2M particles, K self-consistent steps and "
-                         "their adjoint")
+                         "their adjoint; wave: 16x4x4 plasma-wave box, 16,384 particles, CVWM (launch-bound: "
+                         "eager steps against graph replay)")
     ap.add_argument("--adjoint-steps", type=int, default=50, help="adjoint workload: K")
     ap.add_argument("--adjoint-every", type=int, default=10, help="adjoint workload: checkpoint interval")
     ap.add_argument("--particles", type=int, default=30_000_000, help="annulus workload only")
@@ -188,6 +189,8 @@ def main():
         return bench_annulus(args)
     if args.workload == "adjoint":
         return bench_adjoint(args)
+    if args.workload == "wave":
+        return bench_wave(args)
     w = W.thermal_cube(nx=args.nx, ppc=args.ppc)
     n = len(w.PartState)
     print("bench: %s, %d particles, %d warmup + %d timed steps" % (w.name, n, args.warmup, args.steps),
@@ -281,6 +284,31 @@ def bench_annulus(args):
                                  "interp_push_track": st["ms_push_track"] / args.steps,
                                  "sort": st["ms_sort"] / args.steps},
            "n_removed_or_lost": {"lost": st["n_lost"], "inside_after": st["n_particles"]}}
+    print(json.dumps(out), flush=True)
+
+
+def bench_wave(args):
+    """A small, launch-bound step: 16x4x4 periodic box of [0, 4 pi] x [0,1]^2,
+    N = 3, 16,384 electrons with density n0 (1 + 0.01 cos(x/2)), CVWM,
+    TriaTracking, frozen field of the perturbation.  Times eager steps and
+    the replay of the captured two-step graph (the tuner's table)."""
+    from piclas_amd import stage, workloads as W
+    from piclas_amd.autotune import autotune
+    w = W.plasma_wave_box()
+    n = len(w.PartState)
+    print("bench: %s, %d particles, %d timed steps per candidate" % (w.name, n, args.steps), file=sys.stderr,
+          flush=True)
+    best, table = autotune(w, steps=args.steps, warmup=max(args.warmup, 2), verbose=True)
+    ms = {("graph" if c["use_graph"] else "eager"): t for c, t in table}
+    out = {"metric": "particle-steps/s (interp+push+track+deposit), launch-bound small step",
+           "value": n / (ms[("graph" if best["use_graph"] else "eager")] * 1e-3), "unit": "particle-steps/s",
+           "n_gpus": 1, "steps": args.steps, "warmup": max(args.warmup, 2),
+           "ms_per_step": ms[("graph" if best["use_graph"] else "eager")], "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": w.name, "elements": w.mesh.nElems, "N": w.N, "particles": n,
+                      "deposition": "cell_volweight_mean", "tracking": "triatracking", "parallelism": "1 GPU"},
+           "tuning": best, "ms_per_step_eager": ms["eager"], "ms_per_step_graph": ms["graph"],
+           "graph_speedup": ms["eager"] / ms["graph"]}
     print(json.dumps(out), flush=True)
 
 

@@ -16,17 +16,17 @@ from . import stage
 from . import workloads as W
 
 
-def candidates(N):
+def candidates(N, deposition=stage.DEPO_SF):
     out = []
     for g in (False, True):
-        for k in ((1, 2) if N == 3 else (1,)):
+        for k in ((1, 2) if N == 3 and deposition == stage.DEPO_SF else (0,)):
             out.append({"use_graph": g, "sf_kernel": k})
     return out
 
 
 def autotune(w, steps=4, warmup=2, precision=stage.PREC_FP64, cands=None, verbose=False):
     """Returns (best tuning dict, [(tuning, ms per step), ...])."""
-    cands = cands or candidates(w.N)
+    cands = cands or candidates(w.N, w.deposition)
     n = len(w.PartState)
     s = stage.PICStage(w.mesh, w.N, max_particles=n, precision=precision, **W.stage_kwargs(w))
     s.set_field(w.E)

@@ -381,6 +381,7 @@ int restore_state(pic_handle *h, const StepState &S) {
   HIPCHK(hipMemcpyAsync(h->P.id, S.id, n * sizeof(int32_t), hipMemcpyDeviceToDevice, h->stream));
   HIPCHK(hipMemcpyAsync(h->P.spec, S.spec, n, hipMemcpyDeviceToDevice, h->stream));
   HIPCHK(hipMemcpyAsync(h->start, S.start, (h->hm.nElems + 1) * sizeof(int32_t), hipMemcpyDeviceToDevice, h->stream));
+  h->recFresh = false;
   hipLaunchKernelGGL(k_fill_u8, dim3(grid_for(int64_t(n))), dim3(kBlock), 0, h->stream, h->P.isNew, int64_t(n), uint8_t(0));
   h->n = S.n;
   return 0;
@@ -424,6 +425,7 @@ int sc_step(pic_handle *h, double dt) {
   const int32_t *nDev = h->start + h->hm.nElems;
   hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(kBlock), 0, h->stream, h->mv.count, int64_t(1), 0);
   LAUNCH_N(k_push_inside, h->hm.N, dim3(grid_for(std::max<int64_t>(n, 1))), h->M, h->P, n, dt, h->newElem, h->mv, nDev);
+  h->recFresh = false;
   hipLaunchKernelGGL(k_track_movers, dim3(1024), dim3(kBlock), 0, h->stream, h->M, h->P, h->newElem, h->mv,
                      h->lostCount, h->insideById, h->stateById);
   HIPCHK(hipGetLastError());

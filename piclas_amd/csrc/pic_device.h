@@ -20,6 +20,7 @@ struct DevMesh {
   const int32_t *sideNb;    // [e][6]
   const int32_t *sideGID;   // [e][6]
   const uint8_t *sideNode;  // [e][6][4]
+  const double *sideXYZ;    // [e][6][4][3] corner coordinates of every side in ElemSideNodeID order
   const uint8_t *sideBC;    // [e][6]
   const int8_t *sidePV;     // [e][6]
   const uint8_t *concave;   // [e]
@@ -460,17 +461,16 @@ __device__ inline void boris_push(double q, double m, bool isNew, const double F
 
 // ParticleInsideQuad3D, particles/particle_mesh/particle_mesh_tools.f90:78-222
 __device__ inline bool inside_quad(const DevMesh &M, int e, const double x[3], double det[6][2]) {
-  const double *C = M.corner + (size_t)e * 24;
   const uint8_t cc = M.concave[e];
   bool in = true;
 #pragma unroll 1
   for (int l = 0; l < 6; ++l) {
-    const uint8_t *sn = M.sideNode + ((size_t)e * 6 + l) * 4;
+    const double *sx = M.sideXYZ + ((size_t)e * 6 + l) * 12;
     double A[3][4];
 #pragma unroll
     for (int q = 0; q < 4; ++q)
 #pragma unroll
-      for (int c = 0; c < 3; ++c) A[c][q] = C[sn[q] * 3 + c] - x[c];
+      for (int c = 0; c < 3; ++c) A[c][q] = sx[q * 3 + c] - x[c];
     const double cp0 = A[1][0] * A[2][2] - A[2][0] * A[1][2];
     const double cp1 = A[2][0] * A[0][2] - A[0][0] * A[2][2];
     const double cp2 = A[0][0] * A[1][2] - A[1][0] * A[0][2];
@@ -489,18 +489,17 @@ __device__ inline bool inside_quad(const DevMesh &M, int e, const double x[3], d
 
 // ParticleThroughSideCheck3DFast, particles/tracking/particle_intersection.f90:167-280
 __device__ inline bool through_side_fast(const DevMesh &M, int e, int l, int tri, const double P[3], const double V[3]) {
-  const double *C = M.corner + (size_t)e * 24;
-  const uint8_t *sn = M.sideNode + ((size_t)e * 6 + l) * 4;
+  const double *sx = M.sideXYZ + ((size_t)e * 6 + l) * 12;
   double Ax[3], Ay[3], Az[3];
-  Ax[0] = C[sn[0] * 3 + 0] - P[0];
-  Ay[0] = C[sn[0] * 3 + 1] - P[1];
-  Az[0] = C[sn[0] * 3 + 2] - P[2];
+  Ax[0] = sx[0] - P[0];
+  Ay[0] = sx[1] - P[1];
+  Az[0] = sx[2] - P[2];
 #pragma unroll
   for (int n = 1; n < 3; ++n) {
     const int node = n + tri - 1;  // 0-based side node for n = 1,2
-    Ax[n] = C[sn[node] * 3 + 0] - P[0];
-    Ay[n] = C[sn[node] * 3 + 1] - P[1];
-    Az[n] = C[sn[node] * 3 + 2] - P[2];
+    Ax[n] = sx[node * 3 + 0] - P[0];
+    Ay[n] = sx[node * 3 + 1] - P[1];
+    Az[n] = sx[node * 3 + 2] - P[2];
   }
   double dc[3][3];
   dc[0][0] = (Ay[0] * V[2] - Az[0] * V[1]) * Ax[2];
@@ -529,15 +528,14 @@ __device__ inline bool through_side_fast(const DevMesh &M, int e, int l, int tri
 
 // ParticleThroughSideLastPosCheck (regular side), particle_intersection.f90:430-512
 __device__ inline bool through_side_last_pos(const DevMesh &M, int e, int l, int tri, const double P[3], double &det) {
-  const double *C = M.corner + (size_t)e * 24;
-  const uint8_t *sn = M.sideNode + ((size_t)e * 6 + l) * 4;
-  const int nodes[3] = {sn[0], sn[tri], sn[tri + 1]};
+  const double *sx = M.sideXYZ + ((size_t)e * 6 + l) * 12;
+  const int nodes[3] = {0, tri, tri + 1};
   double Ax[3], Ay[3], Az[3];
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
-    Ax[q] = C[nodes[q] * 3 + 0] - P[0];
-    Ay[q] = C[nodes[q] * 3 + 1] - P[1];
-    Az[q] = C[nodes[q] * 3 + 2] - P[2];
+    Ax[q] = sx[nodes[q] * 3 + 0] - P[0];
+    Ay[q] = sx[nodes[q] * 3 + 1] - P[1];
+    Az[q] = sx[nodes[q] * 3 + 2] - P[2];
   }
   det = ((Ay[0] * Az[1] - Az[0] * Ay[1]) * Ax[2] + (Az[0] * Ax[1] - Ax[0] * Az[1]) * Ay[2] +
          (Ax[0] * Ay[1] - Ay[0] * Ax[1]) * Az[2]);
@@ -546,12 +544,11 @@ __device__ inline bool through_side_last_pos(const DevMesh &M, int e, int l, int
 
 // IntersectionWithWall, particle_intersection.f90:79-164
 __device__ inline double intersection_with_wall(const DevMesh &M, int e, int l, int tri, const double P[3], const double traj[3]) {
-  const double *C = M.corner + (size_t)e * 24;
-  const uint8_t *sn = M.sideNode + ((size_t)e * 6 + l) * 4;
-  const double xN = C[sn[0] * 3], yN = C[sn[0] * 3 + 1], zN = C[sn[0] * 3 + 2];
-  const int n1 = sn[tri], n2 = sn[tri + 1];
-  const double V1[3] = {C[n1 * 3] - xN, C[n1 * 3 + 1] - yN, C[n1 * 3 + 2] - zN};
-  const double V2[3] = {C[n2 * 3] - xN, C[n2 * 3 + 1] - yN, C[n2 * 3 + 2] - zN};
+  const double *sx = M.sideXYZ + ((size_t)e * 6 + l) * 12;
+  const double xN = sx[0], yN = sx[1], zN = sx[2];
+  const int n1 = tri, n2 = tri + 1;
+  const double V1[3] = {sx[n1 * 3] - xN, sx[n1 * 3 + 1] - yN, sx[n1 * 3 + 2] - zN};
+  const double V2[3] = {sx[n2 * 3] - xN, sx[n2 * 3 + 1] - yN, sx[n2 * 3 + 2] - zN};
   double nx = V1[1] * V2[2] - V1[2] * V2[1];
   double ny = V1[2] * V2[0] - V1[0] * V2[2];
   double nz = V1[0] * V2[1] - V1[1] * V2[0];

@@ -539,16 +539,28 @@ __global__ __launch_bounds__(kBlock) void k_gather(const int32_t *total, const i
 }
 
 // six FP64 state arrays + id/spec/isNew/elem in one pass
+// (and, when rec is given, the shape-function deposition records of the new
+// order; fsp holds ChargeIC * MacroParticleFactor * w_sf per species)
+struct SpeciesFactor {
+  double fsp[8];
+};
 __global__ __launch_bounds__(kBlock) void k_gather_state(const int32_t *total, const int32_t *perm, DevParts s, DevParts d,
-                                                         const int32_t *newElem) {
+                                                         const int32_t *newElem, SpeciesFactor sf, double4 *rec) {
   const int64_t j = int64_t(blockIdx.x) * kBlock + threadIdx.x;
   if (j >= *total) return;
   const int32_t i = perm[j];
-  d.x[j] = s.x[i]; d.y[j] = s.y[i]; d.z[j] = s.z[i];
-  d.vx[j] = s.vx[i]; d.vy[j] = s.vy[i]; d.vz[j] = s.vz[i];
-  d.id[j] = s.id[i]; d.spec[j] = s.spec[i]; d.isNew[j] = s.isNew[i];
+  const double x = s.x[i], y = s.y[i], z = s.z[i], vx = s.vx[i], vy = s.vy[i], vz = s.vz[i];
+  const uint8_t sp = s.spec[i];
+  d.x[j] = x; d.y[j] = y; d.z[j] = z;
+  d.vx[j] = vx; d.vy[j] = vy; d.vz[j] = vz;
+  d.id[j] = s.id[i]; d.spec[j] = sp; d.isNew[j] = s.isNew[i];
   d.elem[j] = newElem[i];
   if (s.rx) { d.rx[j] = s.rx[i]; d.ry[j] = s.ry[i]; d.rz[j] = s.rz[i]; }
+  if (rec) {
+    const double f = sf.fsp[sp];
+    rec[2 * j] = make_double4(x, y, z, 0.0);
+    rec[2 * j + 1] = make_double4(vx * f, vy * f, vz * f, f);
+  }
 }
 
 // ---- cell_volweight_mean ----------------------------------------------------
@@ -1386,6 +1398,7 @@ struct pic_handle {
   SFArgs sf{};
   Adjoint adj;
   bool sfMatrix = true;   // N = 3 shape function: accumulate on the matrix cores
+  bool recFresh = false;  // sf.rec holds the records of the current particle arrays
   bool useGraph = false;  // replay captured two-step graphs in pic_time_step
   hipGraph_t graph = nullptr;
   hipGraphExec_t graphExec = nullptr;
@@ -1514,8 +1527,12 @@ int run_deposition(pic_handle *h, bool timed) {
     const int waves = kBlock / kWave;
     const unsigned nblk = xcd_grid((ne + waves - 1) / waves);
     if (records) {
-      hipLaunchKernelGGL(k_sf_records, dim3(grid_for(std::max<int64_t>(h->n, 1))), dim3(kBlock), 0, h->stream, h->P,
-                         h->start + ne, h->sf, const_cast<double4 *>(h->sf.rec));
+      // the element sort writes the records; after other changes of the
+      // particle arrays they are formed here
+      if (!h->recFresh)
+        hipLaunchKernelGGL(k_sf_records, dim3(grid_for(std::max<int64_t>(h->n, 1))), dim3(kBlock), 0, h->stream, h->P,
+                           h->start + ne, h->sf, const_cast<double4 *>(h->sf.rec));
+      h->recFresh = true;
       if (timed) HIPCHK(hipEventRecord(h->ev[8], h->stream));
     }
     if (N == 3 && mixed) {
@@ -1572,7 +1589,12 @@ int enqueue_sort(pic_handle *h, int64_t nGrid, bool withStaged) {
   hipLaunchKernelGGL(k_fix_arrivals, dim3(grid_for(ne)), dim3(kBlock), 0, h->stream, ne, h->startNew, h->stayCnt,
                      h->arrCnt, h->P.id, h->perm);
   const int32_t *total = h->startNew + ne;
-  hipLaunchKernelGGL(k_gather_state, dim3(grid_for(n)), dim3(kBlock), 0, h->stream, total, h->perm, h->P, h->Q, h->newElem);
+  // records for the shape-function gathers that read them
+  const bool useRec = h->sf.rec && (h->sfMatrix || h->par.precision == PIC_PREC_MIXED);
+  SpeciesFactor sfac{};
+  for (int k = 0; k < 8; ++k) sfac.fsp[k] = h->sf.fsp[k];
+  hipLaunchKernelGGL(k_gather_state, dim3(grid_for(n)), dim3(kBlock), 0, h->stream, total, h->perm, h->P, h->Q, h->newElem,
+                     sfac, useRec ? const_cast<double4 *>(h->sf.rec) : nullptr);
   if (withStaged) {
     // keep LastPartPos / LastElem / FieldAtParticle in the new order
     hipLaunchKernelGGL(k_gather<double>, dim3(grid_for(n)), dim3(kBlock), 0, h->stream, total, h->perm, h->P.lx, h->Q.lx, 1);
@@ -1591,6 +1613,7 @@ int enqueue_sort(pic_handle *h, int64_t nGrid, bool withStaged) {
   std::swap(h->P.spec, h->Q.spec); std::swap(h->P.isNew, h->Q.isNew);
   std::swap(h->P.rx, h->Q.rx); std::swap(h->P.ry, h->Q.ry); std::swap(h->P.rz, h->Q.rz);
   std::swap(h->start, h->startNew);
+  h->recFresh = useRec;   // the gather wrote the records of the new order
   return 0;
 }
 
@@ -1657,6 +1680,16 @@ int pic_create(const pic_mesh_desc *mesh, const pic_params *par, pic_handle **ou
   rc |= upload(h, const_cast<int32_t **>(&M.sideNb), hm.sideNb);
   rc |= upload(h, const_cast<int32_t **>(&M.sideGID), hm.sideGID);
   rc |= upload(h, const_cast<uint8_t **>(&M.sideNode), hm.sideNode);
+  {
+    std::vector<double> sxyz(size_t(hm.nElems) * 72);
+    for (int e = 0; e < hm.nElems; ++e)
+      for (int l = 0; l < 6; ++l)
+        for (int q = 0; q < 4; ++q)
+          for (int c = 0; c < 3; ++c)
+            sxyz[((size_t(e) * 6 + l) * 4 + q) * 3 + c] =
+                hm.corner[(size_t(e) * 8 + hm.sideNode[(size_t(e) * 6 + l) * 4 + q]) * 3 + c];
+    rc |= upload(h, const_cast<double **>(&M.sideXYZ), sxyz);
+  }
   rc |= upload(h, const_cast<uint8_t **>(&M.sideBC), hm.sideBC);
   rc |= upload(h, const_cast<int8_t **>(&M.sidePV), hm.sidePV);
   rc |= upload(h, const_cast<uint8_t **>(&M.concave), hm.concave);
@@ -1879,6 +1912,7 @@ int pic_set_field_device(pic_handle *h, const double *E_dev) {
 
 int pic_set_particles(pic_handle *h, int64_t n, const double *PartState, const int32_t *elem, const int32_t *species,
                       const uint8_t *is_new) {
+  if (h) h->recFresh = false;
   if (!h || (n > 0 && !PartState)) return fail(PIC_EINVAL, "pic_set_particles: null argument");
   if (n < 0 || n > h->cap) return fail(PIC_ECAPACITY, "pic_set_particles: more particles than max_particles");
   if (set_device(h)) return PIC_EHIP;
@@ -2009,6 +2043,7 @@ int pic_push_boris_leapfrog(pic_handle *h, double dt) {
   if (!h->stagedValid) return fail(PIC_ESTATE, "pic_push_boris_leapfrog: call pic_interpolate_field_to_particle first");
   if (set_device(h)) return PIC_EHIP;
   if (h->n > 0) hipLaunchKernelGGL(k_push, dim3(grid_for(h->n)), dim3(kBlock), 0, h->stream, h->M, h->P, h->n, dt);
+  h->recFresh = false;
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(h->stream));
   return PIC_OK;
@@ -2044,6 +2079,7 @@ namespace {
 // while particles are removed; this makes the step capturable in a graph.
 int enqueue_step(pic_handle *h, double dt, int64_t nGrid, bool timed) {
   if (int rc = run_deposition(h, timed)) return rc;
+  h->recFresh = false;   // the push below moves the particles; the re-sort writes new records
   if (timed) HIPCHK(hipEventRecord(h->ev[3], h->stream));
   const int ne = h->hm.nElems;
   const int32_t *nDev = h->start + ne;
@@ -2092,10 +2128,26 @@ void drop_graph(pic_handle *h) {
 
 // Two steps captured in one graph: the particle buffers swap every step, so
 // after two the pointers the graph was recorded with are current again.
+// deposition records of the current arrays, if the gathers read them and
+// the last sort did not write them
+int refresh_records(pic_handle *h) {
+  const bool records = h->par.deposition == PIC_DEPO_SF && h->hm.N == 3 &&
+                       (h->sfMatrix || h->par.precision == PIC_PREC_MIXED) && h->sf.rec;
+  if (!records || h->recFresh) return 0;
+  hipLaunchKernelGGL(k_sf_records, dim3(grid_for(std::max<int64_t>(h->n, 1))), dim3(kBlock), 0, h->stream, h->P,
+                     h->start + h->hm.nElems, h->sf, const_cast<double4 *>(h->sf.rec));
+  HIPCHK(hipGetLastError());
+  h->recFresh = true;
+  return 0;
+}
+
 int ensure_graph(pic_handle *h, double dt) {
   if (h->graphExec && h->graphDt == dt && h->graphN >= h->n && h->graphP == h->P.x) return 0;
   drop_graph(h);
   const int64_t nGrid = h->n;
+  // the captured steps read the records the previous step's sort wrote: form
+  // them now if the arrays changed since, and before every replay
+  if (int rc = refresh_records(h)) return rc;
   HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
   int rc = enqueue_step(h, dt, nGrid, false);
   if (!rc) rc = enqueue_step(h, dt, nGrid, false);
@@ -2124,6 +2176,7 @@ int pic_time_step(pic_handle *h, double dt, int32_t nsteps) {
     // replay the two-step graph; the device time of the replays is booked
     // as push/track time (the stages are not timed one by one)
     if (int rc = ensure_graph(h, dt)) return rc;
+    if (int rc = refresh_records(h)) return rc;
     HIPCHK(hipEventRecord(h->ev[6], h->stream));
     for (; it + 1 < nsteps; it += 2) HIPCHK(hipGraphLaunch(h->graphExec, h->stream));
     HIPCHK(hipEventRecord(h->ev[7], h->stream));
