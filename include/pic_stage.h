@@ -133,6 +133,36 @@ int  pic_set_field_device(pic_handle *h, const double *E_dev);
 int  pic_set_particles(pic_handle *h, int64_t n, const double *PartState, const int32_t *elem,
                        const int32_t *species, const uint8_t *is_new);
 
+/* Particle emission on the device (initial insertion,
+ * particles/emission/particle_position_and_velocity.f90: SetParticlePosition
+ * with SpaceIC 'cuboid' or 'sin_deviation', SetParticleVelocity with
+ * 'constant' or 'maxwell_lpn'): positions and velocities are formed on the
+ * device, located there (SinglePointToElement) and set as the stage's
+ * particles, as pic_set_particles with IsNewPart = T.  Particle ids follow
+ * the inits in order (init 0's particles first); particles no element holds
+ * are kept as not inside.  Random numbers: Philox4x32-10 keyed by seed with
+ * the counter (particle id, draw), so results depend on (seed, id) only; the
+ * reference's RANDOM_NUMBER stream is not reproduced. */
+enum { PIC_SPACE_CUBOID = 1, PIC_SPACE_SIN_DEVIATION = 2 };   /* Part-SpeciesX-InitY-SpaceIC        */
+enum { PIC_VELO_CONSTANT = 1, PIC_VELO_MAXWELL_LPN = 2 };     /* Part-SpeciesX-InitY-velocityDistribution */
+typedef struct {
+  int32_t species;                /* 1-based                                       */
+  int32_t space;                  /* PIC_SPACE_*                                   */
+  int64_t particle_number;        /* ParticleNumber (cuboid)                       */
+  double  base_point[3];          /* BasePointIC (cuboid)                          */
+  double  base_vector1[3];        /* BaseVector1IC (cuboid)                        */
+  double  base_vector2[3];        /* BaseVector2IC (cuboid)                        */
+  double  cuboid_height;          /* CuboidHeightIC (cuboid)                       */
+  int32_t max_particle_number[3]; /* maxParticleNumber-x/y/z (sin_deviation)       */
+  double  amplitude;              /* Amplitude (sin_deviation)                     */
+  double  wave_number;            /* WaveNumber (sin_deviation)                    */
+  int32_t velocity;               /* PIC_VELO_*                                    */
+  double  velo_ic;                /* VeloIC                                        */
+  double  velo_vec[3];            /* VeloVecIC                                     */
+  double  temperature;            /* MWTemperatureIC                               */
+} pic_emission;
+int  pic_emit_particles(pic_handle *h, int32_t n_inits, const pic_emission *inits, uint64_t seed, int64_t *n_emitted);
+
 /* The four stage routines, by reference name. */
 int  pic_deposition(pic_handle *h);                          /* Deposition()                 */
 int  pic_interpolate_field_to_particle(pic_handle *h);       /* InterpolateFieldToParticle() */

@@ -1495,6 +1495,8 @@ int set_device(pic_handle *h) {
 }
 
 void drop_graph(pic_handle *h);
+int set_particles_impl(pic_handle *h, int64_t n, const double *PartState, const int32_t *elem, const int32_t *species,
+                       const uint8_t *is_new, bool outsideMarked);
 
 // the dominant kernel run_deposition launches (PIC_KERNEL_*)
 int deposit_kernel(const pic_handle *h) {
@@ -1912,6 +1914,16 @@ int pic_set_field_device(pic_handle *h, const double *E_dev) {
 
 int pic_set_particles(pic_handle *h, int64_t n, const double *PartState, const int32_t *elem, const int32_t *species,
                       const uint8_t *is_new) {
+  return set_particles_impl(h, n, PartState, elem, species, is_new, false);
+}
+
+}  // extern "C"
+
+namespace {
+// pic_set_particles; with outsideMarked an element id of -1 marks a particle
+// that no element holds (set up as not inside)
+int set_particles_impl(pic_handle *h, int64_t n, const double *PartState, const int32_t *elem, const int32_t *species,
+                       const uint8_t *is_new, bool outsideMarked) {
   if (h) h->recFresh = false;
   if (!h || (n > 0 && !PartState)) return fail(PIC_EINVAL, "pic_set_particles: null argument");
   if (n < 0 || n > h->cap) return fail(PIC_ECAPACITY, "pic_set_particles: more particles than max_particles");
@@ -1923,9 +1935,13 @@ int pic_set_particles(pic_handle *h, int64_t n, const double *PartState, const i
   std::vector<int64_t> pending;
   for (int64_t i = 0; i < n; ++i) {
     int e = elem ? elem[i] : 0;
-    if (e < 0 || e > ne) return fail(PIC_EINVAL, "pic_set_particles: element id out of range");
-    if (e == 0) pending.push_back(i);
-    el[i] = e - 1;
+    if (outsideMarked && e == -1) {
+      el[i] = -1;
+    } else {
+      if (e < 0 || e > ne) return fail(PIC_EINVAL, "pic_set_particles: element id out of range");
+      if (e == 0) pending.push_back(i);
+      el[i] = e - 1;
+    }
     sp[i] = species ? species[i] - 1 : 0;
     if (sp[i] < 0 || sp[i] >= h->par.nSpecies) return fail(PIC_EINVAL, "pic_set_particles: species out of range");
   }
@@ -1997,7 +2013,7 @@ int pic_set_particles(pic_handle *h, int64_t n, const double *PartState, const i
   return PIC_OK;
 }
 
-}  // extern "C"
+}  // namespace
 
 extern "C" {
 
@@ -2391,3 +2407,4 @@ int pic_synchronize(pic_handle *h) {
 }  // extern "C"
 
 #include "pic_adjoint_impl.h"
+#include "pic_emission.h"

@@ -52,6 +52,19 @@ class PicParams(ctypes.Structure):
                 ("precision", ctypes.c_int32), ("max_particles", ctypes.c_int64), ("device", ctypes.c_int32)]
 
 
+SPACE_CUBOID, SPACE_SIN_DEVIATION = 1, 2
+VELO_CONSTANT, VELO_MAXWELL_LPN = 1, 2
+
+
+class PicEmission(ctypes.Structure):
+    _fields_ = [("species", ctypes.c_int32), ("space", ctypes.c_int32), ("particle_number", ctypes.c_int64),
+                ("base_point", ctypes.c_double * 3), ("base_vector1", ctypes.c_double * 3),
+                ("base_vector2", ctypes.c_double * 3), ("cuboid_height", ctypes.c_double),
+                ("max_particle_number", ctypes.c_int32 * 3), ("amplitude", ctypes.c_double),
+                ("wave_number", ctypes.c_double), ("velocity", ctypes.c_int32), ("velo_ic", ctypes.c_double),
+                ("velo_vec", ctypes.c_double * 3), ("temperature", ctypes.c_double)]
+
+
 class PicStats(ctypes.Structure):
     _fields_ = [("n_particles", ctypes.c_int64), ("n_lost", ctypes.c_int64), ("n_steps", ctypes.c_int64),
                 ("ms_deposit", ctypes.c_double), ("ms_push_track", ctypes.c_double), ("ms_sort", ctypes.c_double),
@@ -90,6 +103,8 @@ EXPORTS = {
     "pic_sc_time_step": (ctypes.c_int, [_H, ctypes.c_double, ctypes.c_int32, _f64p]),
     "pic_adjoint": (ctypes.c_int, [_H, ctypes.c_double, ctypes.c_int32, ctypes.c_int32, _f64p, _f64p, _f64p]),
     "pic_get_stats": (ctypes.c_int, [_H, ctypes.POINTER(PicStats)]),
+    "pic_emit_particles": (ctypes.c_int, [_H, ctypes.c_int32, ctypes.POINTER(PicEmission), ctypes.c_uint64,
+                                          ctypes.POINTER(ctypes.c_int64)]),
     "pic_reset_stats": (ctypes.c_int, [_H]),
     "pic_synchronize": (ctypes.c_int, [_H]),
 }
@@ -279,6 +294,26 @@ class PICStage:
         v.__cuda_array_interface__ = {"shape": (self.nElems, self.nDof, 4), "typestr": "<f8",
                                       "data": (self.part_source_device(), False), "version": 2, "strides": None}
         return torch.as_tensor(v, device=device or torch.device("cuda", 0))
+
+    def emit_particles(self, inits, seed=1):
+        """Particle emission on the device (SetParticlePosition 'cuboid' /
+        'sin_deviation', SetParticleVelocity 'constant' / 'maxwell_lpn'):
+        inits is a list of dicts with the PicEmission fields (species is
+        1-based); sets them as the particles, located on the device.
+        Returns the number of particles inside the mesh."""
+        arr = (PicEmission * len(inits))()
+        for a, d in zip(arr, inits):
+            for k, v in d.items():
+                if isinstance(v, (list, tuple, np.ndarray)):
+                    getattr(a, k)[:] = list(v)
+                else:
+                    setattr(a, k, v)
+        out = ctypes.c_int64(0)
+        _chk(load_library().pic_emit_particles(self.h, len(inits), arr, ctypes.c_uint64(seed), ctypes.byref(out)),
+             "pic_emit_particles")
+        self.nPart = sum(d.get("particle_number", 0) if d.get("space", SPACE_CUBOID) == SPACE_CUBOID
+                         else int(np.prod(d["max_particle_number"])) for d in inits)
+        return out.value
 
     def set_particles(self, PartState, elem=None, species=None, is_new=None):
         ps = np.ascontiguousarray(PartState, np.float64).reshape(-1, 6)
