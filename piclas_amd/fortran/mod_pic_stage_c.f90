@@ -41,6 +41,20 @@ TYPE, BIND(C), PUBLIC :: tPicParams
   INTEGER(C_INT32_T) :: device
 END TYPE tPicParams
 
+!> one emission init (Part-SpeciesX-InitY-*), see pic_emit_particles
+TYPE, BIND(C), PUBLIC :: tPicEmission
+  INTEGER(C_INT32_T) :: species, space
+  INTEGER(C_INT64_T) :: particle_number
+  REAL(C_DOUBLE)     :: base_point(3), base_vector1(3), base_vector2(3), cuboid_height
+  INTEGER(C_INT32_T) :: max_particle_number(3)
+  REAL(C_DOUBLE)     :: amplitude, wave_number
+  INTEGER(C_INT32_T) :: velocity
+  REAL(C_DOUBLE)     :: velo_ic, velo_vec(3), temperature
+END TYPE tPicEmission
+
+INTEGER(C_INT32_T),PARAMETER,PUBLIC :: PIC_SPACE_CUBOID = 1, PIC_SPACE_SIN_DEVIATION = 2
+INTEGER(C_INT32_T),PARAMETER,PUBLIC :: PIC_VELO_CONSTANT = 1, PIC_VELO_MAXWELL_LPN = 2
+
 INTERFACE
   INTEGER(C_INT) FUNCTION pic_create(mesh, par, h) BIND(C, NAME='pic_create')
     IMPORT :: C_INT, C_PTR, tPicMeshDesc, tPicParams
@@ -123,6 +137,14 @@ INTERFACE
     TYPE(C_PTR),VALUE          :: h
     REAL(C_DOUBLE),INTENT(OUT) :: WEl
   END FUNCTION pic_field_energy
+  INTEGER(C_INT) FUNCTION pic_emit_particles(h, n_inits, inits, seed, n_emitted) BIND(C, NAME='pic_emit_particles')
+    IMPORT :: C_INT, C_PTR, C_INT32_T, C_INT64_T, tPicEmission
+    TYPE(C_PTR),VALUE                 :: h
+    INTEGER(C_INT32_T),VALUE          :: n_inits
+    TYPE(tPicEmission),INTENT(IN)     :: inits(*)
+    INTEGER(C_INT64_T),VALUE          :: seed
+    INTEGER(C_INT64_T),INTENT(OUT)    :: n_emitted
+  END FUNCTION pic_emit_particles
   INTEGER(C_SIZE_T) FUNCTION c_strlen(s) BIND(C, NAME='strlen')
     IMPORT :: C_SIZE_T, C_PTR
     TYPE(C_PTR),VALUE :: s
@@ -134,7 +156,7 @@ TYPE(C_PTR),SAVE :: stage = C_NULL_PTR
 PUBLIC :: InitPICStage, FinalizePICStage, PICStage_SetField, PICStage_SetParticles, PICStage_GetParticles
 PUBLIC :: Deposition, InterpolateFieldToParticle, PushBorisLeapfrog, PerformTracking, TimeStepPoissonByBorisLeapfrog
 PUBLIC :: PICStage_GetPartSource, PICStage_GetFieldAtParticle, PICStage_GetPartPosRef
-PUBLIC :: CalcDepositedCharge, CalcPotentialEnergy
+PUBLIC :: CalcDepositedCharge, CalcPotentialEnergy, PICStage_EmitParticles
 
 CONTAINS
 
@@ -182,6 +204,15 @@ SUBROUTINE PICStage_SetField(E)
 REAL(C_DOUBLE),INTENT(IN) :: E(:,:,:,:,:)
 CALL Check(pic_set_field(stage, E), 'pic_set_field')
 END SUBROUTINE PICStage_SetField
+
+!> initial particles emitted on the device (SetParticlePosition / SetParticleVelocity
+!> for every init), located there; nEmitted = particles inside the mesh
+SUBROUTINE PICStage_EmitParticles(inits, seed, nEmitted)
+TYPE(tPicEmission),INTENT(IN)   :: inits(:)
+INTEGER(C_INT64_T),INTENT(IN)   :: seed
+INTEGER(C_INT64_T),INTENT(OUT)  :: nEmitted
+CALL Check(pic_emit_particles(stage, INT(SIZE(inits),C_INT32_T), inits, seed, nEmitted), 'pic_emit_particles')
+END SUBROUTINE PICStage_EmitParticles
 
 !> PartState(1:6,1:n), PEM%GlobalElemID (0: locate), PartSpecies, PDM%IsNewPart
 SUBROUTINE PICStage_SetParticles(PartState, GlobalElemID, PartSpecies, IsNewPart)
