@@ -167,6 +167,7 @@ def main():
     ap.add_argument("--ppc", type=int, default=200)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision run")
+    ap.add_argument("--no-exchange", action="store_true", help="skip the rho/E hand-over timing (profiling runs)")
     ap.add_argument("--autotune", action="store_true", help="pick graph replay / shape-function kernel by measurement")
     ap.add_argument("--cpu-fraction", type=float, default=1.0 / 16)
     ap.add_argument("--cpu-steps", type=int, default=3)
@@ -234,7 +235,8 @@ def main():
            "roofline": roof}
     if not args.no_mixed:
         out["mixed"] = mixed_report(w, args, r64)
-    out["exchange"] = exchange_timing(w)
+    if not args.no_exchange:
+        out["exchange"] = exchange_timing(w)
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(w, args.cpu_fraction, args.cpu_steps, threads)
