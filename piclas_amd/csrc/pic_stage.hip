@@ -994,16 +994,17 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE_MX))
 
 // The same gather for targets whose Gauss points form an axis-aligned tensor
 // grid (Cartesian elements): with u = x/r_sf, S = 1 - (u_i - pu)^2 -
-// (v_j - pv)^2 - (w_k - pw)^2 splits into a per-(i,j) part, formed once per
-// lane and group of 4 particles, and the per-particle (w_k - pw)^2, formed
-// once at staging; a pair then costs a subtraction, the clamp and the square.
-// Lane and accumulator layout as k_sf_gather_mfma, with quarter Q = k and
-// lane%16 = i + 4j.
+// (v_j - pv)^2 - (w_k - pw)^2 = [K_ij + 2 u_i pu + 2 v_j pv] - C_k with the
+// lane constant K_ij = 1 - u_i^2 - v_j^2 and the per-particle
+// C_k = (w_k - pw)^2 + pu^2 + pv^2, formed once at staging.  The bracket costs
+// two fused multiply-adds per lane and group of 4 particles; a pair then costs
+// a subtraction, the clamp and the square.  Lane and accumulator layout as
+// k_sf_gather_mfma, with quarter Q = k and lane%16 = i + 4j.
 template <int ALPHA>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE_MX))) void k_sf_gather_cart(DevMesh M, const int32_t *start, SFArgs A, double *PS) {
 #pragma clang fp contract(fast)
   constexpr int nd = 64;
-  // ring of 128 staged records: pu, pv, (w_k - pw)^2 for k = 0..3, f vx, f vy, f vz, f
+  // ring of 128 staged records: pu, pv, C_k for k = 0..3, f vx, f vy, f vz, f
   constexpr int kRing = 128, kRec = 10;
   __shared__ double lds[kBlock / kWave][kRing][kRec];
   const int wib = threadIdx.x / kWave;
@@ -1027,14 +1028,14 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE_MX))
   const double wlo = fmin(fmin(w[0], w[1]), fmin(w[2], w[3])), whi = fmax(fmax(w[0], w[1]), fmax(w[2], w[3]));
   const double ui = (xg[(lane & 3) * 3 + 0] - bx) * Ri;
   const double vj = (xg[4 * ((lane >> 2) & 3) * 3 + 1] - by) * Ri;
+  const double Kij = 1.0 - ui * ui - vj * vj, tu = 2.0 * ui, tv = 2.0 * vj;
   const double box2 = 1.0 + 1e-12;
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
   double (*L)[kRec] = lds[wib];
   const int kp = lane >> 4, jc = lane & 3;
   auto group = [&](const double (*Lh)[kRec], int g) {
     const double *rec = Lh[4 * g + kp];
-    const double a = ui - rec[0], b = vj - rec[1];
-    const double W = fma(-b, b, fma(-a, a, 1.0));
+    const double W = fma(tv, rec[1], fma(tu, rec[0], Kij));
     const double bval = rec[6 + jc];
 #pragma unroll
     for (int Q = 0; Q < 4; ++Q) {
@@ -1077,7 +1078,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE_MX))
         if (pass) {
           double *r = L[(tail + __popcll(m & ((1ull << lane) - 1ull))) & (kRing - 1)];
           const double c0 = w[0] - pw, c1 = w[1] - pw, c2 = w[2] - pw, c3 = w[3] - pw;
-          r[0] = pu; r[1] = pv; r[2] = c0 * c0; r[3] = c1 * c1; r[4] = c2 * c2; r[5] = c3 * c3;
+          const double P = fma(pv, pv, pu * pu);
+          r[0] = pu; r[1] = pv;
+          r[2] = fma(c0, c0, P); r[3] = fma(c1, c1, P); r[4] = fma(c2, c2, P); r[5] = fma(c3, c3, P);
           r[6] = rb.x; r[7] = rb.y; r[8] = rb.z; r[9] = rb.w;
         }
         tail += __popcll(m);
@@ -1085,6 +1088,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE_MX))
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
         for (int g = 0; g < kWave / 4; ++g) group(L + (head & (kRing - 1)), g);
         head += kWave;
         __builtin_amdgcn_wave_barrier();
