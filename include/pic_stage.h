@@ -163,6 +163,27 @@ typedef struct {
 } pic_emission;
 int  pic_emit_particles(pic_handle *h, int32_t n_inits, const pic_emission *inits, uint64_t seed, int64_t *n_emitted);
 
+/* Particle state in the reference's state-file layout (element-sorted):
+ *   PartInt  [nElems][2]  first and last offset of each element's particles
+ *                         (the element holds rows first .. last-1)
+ *   PartData [n][PIC_PART_DATA_SIZE]  x y z vx vy vz species (as a real)
+ * pic_get_particle_data replaces FillParticleData
+ * (src/io_hdf5/hdf5_output_particle.f90:1514-1620): particles inside, elements
+ * ascending, caller order within an element.  *n_out receives their number
+ * (also when cap is too small: the call then fails with PIC_ECAPACITY).
+ * pic_restart_particles replaces ParticleRestart's particle part
+ * (src/particles/restart/particle_restart.f90:134-440, after ParticleReadin,
+ * particle_readin.f90:503-637): the stored element is checked
+ * (ParticleInsideQuad; RefMapping: |xi| <= ElemEpsOneCell), particles that
+ * fail it are located again (SinglePointToElement) and counted in
+ * *n_missing; those no element holds are set as not inside and counted in
+ * *n_lost.  A species ID <= 0 is rejected (the reference aborts).  The HDF5
+ * file is read and written by the host (piclas_amd/state_file.py). */
+#define PIC_PART_DATA_SIZE 7
+int  pic_get_particle_data(pic_handle *h, int64_t cap, double *PartData, int64_t *PartInt, int64_t *n_out);
+int  pic_restart_particles(pic_handle *h, int64_t n, const double *PartData, const int64_t *PartInt,
+                           int64_t *n_missing, int64_t *n_lost);
+
 /* The four stage routines, by reference name. */
 int  pic_deposition(pic_handle *h);                          /* Deposition()                 */
 int  pic_interpolate_field_to_particle(pic_handle *h);       /* InterpolateFieldToParticle() */

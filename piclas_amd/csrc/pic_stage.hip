@@ -2412,3 +2412,4 @@ int pic_synchronize(pic_handle *h) {
 
 #include "pic_adjoint_impl.h"
 #include "pic_emission.h"
+#include "pic_state_io.h"

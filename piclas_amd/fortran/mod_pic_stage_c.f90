@@ -145,6 +145,23 @@ INTERFACE
     INTEGER(C_INT64_T),VALUE          :: seed
     INTEGER(C_INT64_T),INTENT(OUT)    :: n_emitted
   END FUNCTION pic_emit_particles
+  INTEGER(C_INT) FUNCTION pic_get_particle_data(h, cap, PartData, PartInt, n_out) BIND(C, NAME='pic_get_particle_data')
+    IMPORT :: C_INT, C_PTR, C_DOUBLE, C_INT64_T
+    TYPE(C_PTR),VALUE               :: h
+    INTEGER(C_INT64_T),VALUE        :: cap
+    REAL(C_DOUBLE),INTENT(OUT)      :: PartData(*)
+    INTEGER(C_INT64_T),INTENT(OUT)  :: PartInt(*)
+    INTEGER(C_INT64_T),INTENT(OUT)  :: n_out
+  END FUNCTION pic_get_particle_data
+  INTEGER(C_INT) FUNCTION pic_restart_particles(h, n, PartData, PartInt, n_missing, n_lost) &
+      BIND(C, NAME='pic_restart_particles')
+    IMPORT :: C_INT, C_PTR, C_DOUBLE, C_INT64_T
+    TYPE(C_PTR),VALUE               :: h
+    INTEGER(C_INT64_T),VALUE        :: n
+    REAL(C_DOUBLE),INTENT(IN)       :: PartData(*)
+    INTEGER(C_INT64_T),INTENT(IN)   :: PartInt(*)
+    INTEGER(C_INT64_T),INTENT(OUT)  :: n_missing, n_lost
+  END FUNCTION pic_restart_particles
   INTEGER(C_SIZE_T) FUNCTION c_strlen(s) BIND(C, NAME='strlen')
     IMPORT :: C_SIZE_T, C_PTR
     TYPE(C_PTR),VALUE :: s
@@ -157,6 +174,7 @@ PUBLIC :: InitPICStage, FinalizePICStage, PICStage_SetField, PICStage_SetParticl
 PUBLIC :: Deposition, InterpolateFieldToParticle, PushBorisLeapfrog, PerformTracking, TimeStepPoissonByBorisLeapfrog
 PUBLIC :: PICStage_GetPartSource, PICStage_GetFieldAtParticle, PICStage_GetPartPosRef
 PUBLIC :: CalcDepositedCharge, CalcPotentialEnergy, PICStage_EmitParticles
+PUBLIC :: PICStage_FillParticleData, PICStage_ParticleRestart
 
 CONTAINS
 
@@ -213,6 +231,26 @@ INTEGER(C_INT64_T),INTENT(IN)   :: seed
 INTEGER(C_INT64_T),INTENT(OUT)  :: nEmitted
 CALL Check(pic_emit_particles(stage, INT(SIZE(inits),C_INT32_T), inits, seed, nEmitted), 'pic_emit_particles')
 END SUBROUTINE PICStage_EmitParticles
+
+!> FillParticleData: the particles inside, element-sorted as the state file holds them;
+!> PartData(1:7,1:locnPart) = position, velocity, species; PartInt(1:2,1:nElems) = first and last offset
+SUBROUTINE PICStage_FillParticleData(PartData, PartInt, locnPart)
+REAL(C_DOUBLE),INTENT(OUT)      :: PartData(:,:)
+INTEGER(C_INT64_T),INTENT(OUT)  :: PartInt(:,:)
+INTEGER(C_INT64_T),INTENT(OUT)  :: locnPart
+CALL Check(pic_get_particle_data(stage, INT(SIZE(PartData,2),C_INT64_T), PartData, PartInt, locnPart), &
+           'pic_get_particle_data')
+END SUBROUTINE PICStage_FillParticleData
+
+!> ParticleRestart from the state file's PartData(1:7,:) and PartInt(1:2,1:nElems): stored elements
+!> checked on the device, particles that fail the check located again
+SUBROUTINE PICStage_ParticleRestart(PartData, PartInt, NbrOfMissingParticles, NbrOfLostParticles)
+REAL(C_DOUBLE),INTENT(IN)       :: PartData(:,:)
+INTEGER(C_INT64_T),INTENT(IN)   :: PartInt(:,:)
+INTEGER(C_INT64_T),INTENT(OUT)  :: NbrOfMissingParticles, NbrOfLostParticles
+CALL Check(pic_restart_particles(stage, INT(SIZE(PartData,2),C_INT64_T), PartData, PartInt, &
+                                 NbrOfMissingParticles, NbrOfLostParticles), 'pic_restart_particles')
+END SUBROUTINE PICStage_ParticleRestart
 
 !> PartState(1:6,1:n), PEM%GlobalElemID (0: locate), PartSpecies, PDM%IsNewPart
 SUBROUTINE PICStage_SetParticles(PartState, GlobalElemID, PartSpecies, IsNewPart)

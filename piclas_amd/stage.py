@@ -33,6 +33,7 @@ EPS0 = 8.8541878176e-12               # globals_vars.f90:84
 _i32p = ctypes.POINTER(ctypes.c_int32)
 _f64p = ctypes.POINTER(ctypes.c_double)
 _u8p = ctypes.POINTER(ctypes.c_uint8)
+_i64p = ctypes.POINTER(ctypes.c_int64)
 
 
 class PicMeshDesc(ctypes.Structure):
@@ -105,9 +106,14 @@ EXPORTS = {
     "pic_get_stats": (ctypes.c_int, [_H, ctypes.POINTER(PicStats)]),
     "pic_emit_particles": (ctypes.c_int, [_H, ctypes.c_int32, ctypes.POINTER(PicEmission), ctypes.c_uint64,
                                           ctypes.POINTER(ctypes.c_int64)]),
+    "pic_get_particle_data": (ctypes.c_int, [_H, ctypes.c_int64, _f64p, _i64p, ctypes.POINTER(ctypes.c_int64)]),
+    "pic_restart_particles": (ctypes.c_int, [_H, ctypes.c_int64, _f64p, _i64p, ctypes.POINTER(ctypes.c_int64),
+                                             ctypes.POINTER(ctypes.c_int64)]),
     "pic_reset_stats": (ctypes.c_int, [_H]),
     "pic_synchronize": (ctypes.c_int, [_H]),
 }
+
+PART_DATA_SIZE = 7   # PIC_PART_DATA_SIZE: x y z vx vy vz species
 
 _lib = None
 
@@ -325,6 +331,37 @@ class PICStage:
                                               None if sp is None else _p(sp, _i32p),
                                               None if nw is None else _p(nw, _u8p)), "pic_set_particles")
         self.nPart = n
+
+    def restart_particles(self, PartData, PartInt):
+        """ParticleRestart (particles/restart/particle_restart.f90:134-440) from
+        the state file's element-sorted arrays: PartData (n, 7) = x y z vx vy
+        vz species, PartInt (nElems, 2) = first and last offset per element.
+        The stored element is checked on the device and particles that fail
+        the check are located again.  Returns (n_missing, n_lost); the
+        particles are numbered in file order."""
+        pd = np.ascontiguousarray(PartData, np.float64)
+        if pd.ndim != 2 or pd.shape[1] < PART_DATA_SIZE:
+            raise ValueError("PartData must have shape (n, >= %d)" % PART_DATA_SIZE)
+        pd = np.ascontiguousarray(pd[:, :PART_DATA_SIZE])
+        pi = np.ascontiguousarray(PartInt, np.int64)
+        if pi.shape != (self.nElems, 2):
+            raise ValueError("PartInt must have shape (nElems, 2)")
+        miss, lost = ctypes.c_int64(0), ctypes.c_int64(0)
+        _chk(load_library().pic_restart_particles(self.h, pd.shape[0], _p(pd, _f64p), _p(pi, _i64p),
+                                                  ctypes.byref(miss), ctypes.byref(lost)), "pic_restart_particles")
+        self.nPart = pd.shape[0]
+        return miss.value, lost.value
+
+    def particle_data(self):
+        """FillParticleData (io_hdf5/hdf5_output_particle.f90:1514-1620): the
+        particles inside as PartData (n, 7) sorted by element (caller order
+        within an element) and PartInt (nElems, 2)."""
+        pi = np.zeros((self.nElems, 2), np.int64)
+        n = ctypes.c_int64(0)
+        pd = np.zeros((max(self.nPart, 1), PART_DATA_SIZE))
+        _chk(load_library().pic_get_particle_data(self.h, pd.shape[0], _p(pd, _f64p), _p(pi, _i64p), ctypes.byref(n)),
+             "pic_get_particle_data")
+        return pd[:n.value].copy(), pi
 
     # ---------------------------------------------------------------- the stage
     def deposition(self):

@@ -7,6 +7,11 @@ next to itself.  It runs only where the reference tree is present; the
 fixtures it writes are committed, so tests never read the reference itself.
 
 Fixtures written:
+  2Dplasma_test_State_000.00000005000000000.h5
+      regressioncheck/NIG_PIC_poisson_Leapfrog/2D_innerBC_dielectric_surface_charge/
+      copied as is: a state file the reference wrote (File_Version 3.21,
+      PartInt (nElems, 2), PartData, VarNamesParticles, 5-D DG_Source), the
+      layout check of piclas_amd/state_file.py.
   cvwm_plasma_ball.npz
       regressioncheck/NIG_PIC_Deposition/Plasma_Ball_cell_volweight_mean/
         Box_mesh.h5  (HOPR mesh, 10x10x10 periodic box [-1,1]^3, N=1)
@@ -103,14 +108,27 @@ def cvwm_plasma_ball(lib):
     out["N"] = np.array(1)
     # PartAnalyze_ref.csv: deposited charge and its absolute error
     out["TotalChargeRef"] = np.array([1.06801087489800e1, 1.49213974509621e-13])
+    # File_Version of the state file: 1.5, so PartInt is stored as PartInt(nElems,2)
+    sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+    from piclas_amd import state_file
+    with state_file._File(state) as f:
+        out["File_Version"] = np.array(float(f.attr("File_Version")))
     np.savez_compressed(os.path.join(HERE, "cvwm_plasma_ball.npz"), **out)
     return out
+
+
+def copy_state_file():
+    src = os.path.join(REF, "regressioncheck/NIG_PIC_poisson_Leapfrog/2D_innerBC_dielectric_surface_charge",
+                       "2Dplasma_test_State_000.00000005000000000.h5")
+    with open(src, "rb") as f, open(os.path.join(HERE, os.path.basename(src)), "wb") as g:
+        g.write(f.read())
 
 
 if __name__ == "__main__":
     if not os.path.isdir(REF):
         sys.exit("reference tree not found at %s" % REF)
     lib = _lib()
+    copy_state_file()
     o = cvwm_plasma_ball(lib)
     for k, v in o.items():
         print(k, getattr(v, "shape", None), getattr(v, "dtype", None))
