@@ -40,6 +40,11 @@ int fail(int code, const std::string &msg) {
     if (e_ != hipSuccess) return fail(PIC_EHIP, std::string(#x) + ": " + hipGetErrorString(e_));         \
   } while (0)
 
+// hipMemcpy that treats an empty copy as done (no particles is a valid state)
+inline hipError_t copy_bytes(void *dst, const void *src, size_t bytes, hipMemcpyKind kind) {
+  return bytes == 0 ? hipSuccess : hipMemcpy(dst, src, bytes, kind);
+}
+
 constexpr int kBlock = 256;
 constexpr int kWave = 64;
 
@@ -1385,6 +1390,7 @@ struct pic_handle {
   int64_t cap = 0;      // particle capacity
   int64_t n = 0;        // particles in the sorted arrays
   int64_t nTotal = 0;   // caller's particle count (ids 0..nTotal-1)
+  bool particlesSet = false;   // pic_set_particles (or an emission / restart) was called, possibly with none
   // device buffers
   std::vector<void *> allocs;
   DevParts P{}, Q{};    // current and scratch particle arrays
@@ -1432,7 +1438,7 @@ int dalloc(pic_handle *h, T **p, size_t count) {
 template <typename T>
 int upload(pic_handle *h, T **p, const std::vector<T> &v) {
   if (int rc = dalloc(h, p, v.size())) return rc;
-  if (!v.empty()) HIPCHK(hipMemcpy(*p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  if (!v.empty()) HIPCHK(copy_bytes(*p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
   return 0;
 }
 
@@ -1583,7 +1589,7 @@ int run_deposition(pic_handle *h, bool timed) {
 // old segment table holds on the device.  Pointer swaps happen on the host.
 int enqueue_sort(pic_handle *h, int64_t nGrid, bool withStaged) {
   const int ne = h->hm.nElems;
-  const int64_t n = nGrid;
+  const int64_t n = std::max<int64_t>(nGrid, 1);   // the kernels bound themselves by the device count
   const int32_t *nOld = h->start + ne;
   hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(ne)), dim3(kBlock), 0, h->stream, h->arrCnt, int64_t(ne), 0);
   hipLaunchKernelGGL(k_elem_of, dim3(grid_for(int64_t(ne) * kWave)), dim3(kBlock), 0, h->stream, ne, h->start, h->elemOf);
@@ -1638,7 +1644,7 @@ int run_sort(pic_handle *h, bool withStaged) {
 }
 
 int require_particles(pic_handle *h) {
-  if (h->nTotal <= 0) return fail(PIC_ESTATE, "no particles set");
+  if (!h->particlesSet) return fail(PIC_ESTATE, "no particles set");
   return 0;
 }
 
@@ -1956,14 +1962,14 @@ int set_particles_impl(pic_handle *h, int64_t n, const double *PartState, const 
     std::vector<int32_t> pe(np, -1);
     for (int64_t k = 0; k < np; ++k)
       for (int c = 0; c < 3; ++c) xyz[k * 3 + c] = PartState[pending[k] * 6 + c];
-    HIPCHK(hipMemcpy(h->mv.last, xyz.data(), np * 3 * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(h->newElem, pe.data(), np * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIPCHK(copy_bytes(h->mv.last, xyz.data(), np * 3 * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(copy_bytes(h->newElem, pe.data(), np * sizeof(int32_t), hipMemcpyHostToDevice));
     if (refmap) { LAUNCH_NG(k_rm_locate, h->hm.N, h->hm.NGeo, dim3(grid_for(np)), h->M, np, h->mv.last, h->newElem); }
     else hipLaunchKernelGGL(k_tria_locate, dim3(grid_for(np)), dim3(kBlock), 0, h->stream, h->M, np, h->mv.last, h->newElem);
     HIPCHK(hipGetLastError());
     // the stage's stream does not synchronise with the null stream's copies
     HIPCHK(hipStreamSynchronize(h->stream));
-    HIPCHK(hipMemcpy(pe.data(), h->newElem, np * sizeof(int32_t), hipMemcpyDeviceToHost));
+    HIPCHK(copy_bytes(pe.data(), h->newElem, np * sizeof(int32_t), hipMemcpyDeviceToHost));
     for (int64_t k = 0; k < np; ++k) el[pending[k]] = pe[k];
   }
   for (int64_t i = 0; i < n; ++i)
@@ -1986,20 +1992,20 @@ int set_particles_impl(pic_handle *h, int64_t n, const double *PartState, const 
   double *dst[6] = {h->P.x, h->P.y, h->P.z, h->P.vx, h->P.vy, h->P.vz};
   for (int c = 0; c < 6; ++c) {
     for (int64_t j = 0; j < nin; ++j) buf[j] = PartState[order[j] * 6 + c];
-    HIPCHK(hipMemcpy(dst[c], buf.data(), nin * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(copy_bytes(dst[c], buf.data(), nin * sizeof(double), hipMemcpyHostToDevice));
   }
   std::vector<int32_t> ib(nin);
   std::vector<uint8_t> ub(nin);
   for (int64_t j = 0; j < nin; ++j) ib[j] = el[order[j]];
-  HIPCHK(hipMemcpy(h->P.elem, ib.data(), nin * sizeof(int32_t), hipMemcpyHostToDevice));
+  HIPCHK(copy_bytes(h->P.elem, ib.data(), nin * sizeof(int32_t), hipMemcpyHostToDevice));
   for (int64_t j = 0; j < nin; ++j) ib[j] = int32_t(order[j]);
-  HIPCHK(hipMemcpy(h->P.id, ib.data(), nin * sizeof(int32_t), hipMemcpyHostToDevice));
+  HIPCHK(copy_bytes(h->P.id, ib.data(), nin * sizeof(int32_t), hipMemcpyHostToDevice));
   for (int64_t j = 0; j < nin; ++j) ub[j] = uint8_t(sp[order[j]]);
-  HIPCHK(hipMemcpy(h->P.spec, ub.data(), nin, hipMemcpyHostToDevice));
+  HIPCHK(copy_bytes(h->P.spec, ub.data(), nin, hipMemcpyHostToDevice));
   for (int64_t j = 0; j < nin; ++j) ub[j] = is_new ? is_new[order[j]] : 0;
-  HIPCHK(hipMemcpy(h->P.isNew, ub.data(), nin, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(h->start, start.data(), (ne + 1) * sizeof(int32_t), hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(h->insideById, ins.data(), n, hipMemcpyHostToDevice));
+  HIPCHK(copy_bytes(h->P.isNew, ub.data(), nin, hipMemcpyHostToDevice));
+  HIPCHK(copy_bytes(h->start, start.data(), (ne + 1) * sizeof(int32_t), hipMemcpyHostToDevice));
+  HIPCHK(copy_bytes(h->insideById, ins.data(), n, hipMemcpyHostToDevice));
   HIPCHK(hipMemset(h->lostCount, 0, sizeof(unsigned long long)));
   // particles no element holds keep their state for pic_get_particles
   h->unplaced.clear();
@@ -2007,6 +2013,7 @@ int set_particles_impl(pic_handle *h, int64_t n, const double *PartState, const 
     if (!ins[i]) h->unplaced.emplace_back(i, std::vector<double>(PartState + i * 6, PartState + i * 6 + 6));
   h->n = nin;
   h->nTotal = n;
+  h->particlesSet = true;
   h->stagedValid = false;
   drop_graph(h);   // recorded for the previous particle arrays
   if (refmap && nin > 0) {
@@ -2236,20 +2243,20 @@ int pic_get_particles(pic_handle *h, double *PartState, int32_t *elem, uint8_t *
   std::vector<double> a(n);
   std::vector<int32_t> id(n), el(n);
   std::vector<uint8_t> ins(nt);
-  HIPCHK(hipMemcpy(id.data(), h->P.id, n * sizeof(int32_t), hipMemcpyDeviceToHost));
-  HIPCHK(hipMemcpy(el.data(), h->P.elem, n * sizeof(int32_t), hipMemcpyDeviceToHost));
-  HIPCHK(hipMemcpy(ins.data(), h->insideById, nt, hipMemcpyDeviceToHost));
+  HIPCHK(copy_bytes(id.data(), h->P.id, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+  HIPCHK(copy_bytes(el.data(), h->P.elem, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+  HIPCHK(copy_bytes(ins.data(), h->insideById, nt, hipMemcpyDeviceToHost));
   if (PartState) {
     double *src[6] = {h->P.x, h->P.y, h->P.z, h->P.vx, h->P.vy, h->P.vz};
     for (int c = 0; c < 6; ++c) {
-      HIPCHK(hipMemcpy(a.data(), src[c], n * sizeof(double), hipMemcpyDeviceToHost));
+      HIPCHK(copy_bytes(a.data(), src[c], n * sizeof(double), hipMemcpyDeviceToHost));
       for (int64_t j = 0; j < n; ++j) PartState[int64_t(id[j]) * 6 + c] = a[j];
     }
     int64_t nlost = 0;
     for (int64_t i = 0; i < nt; ++i) nlost += ins[i] ? 0 : 1;
     if (nlost > 0) {
       std::vector<double> st(size_t(nt) * 6);
-      HIPCHK(hipMemcpy(st.data(), h->stateById, size_t(nt) * 6 * sizeof(double), hipMemcpyDeviceToHost));
+      HIPCHK(copy_bytes(st.data(), h->stateById, size_t(nt) * 6 * sizeof(double), hipMemcpyDeviceToHost));
       for (int64_t i = 0; i < nt; ++i)
         if (!ins[i])
           for (int c = 0; c < 6; ++c) PartState[i * 6 + c] = st[i * 6 + c];
@@ -2273,10 +2280,10 @@ int pic_get_last_part_pos(pic_handle *h, double *L) {
   const int64_t n = h->n;
   std::vector<int32_t> id(n);
   std::vector<double> a(n);
-  HIPCHK(hipMemcpy(id.data(), h->P.id, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+  HIPCHK(copy_bytes(id.data(), h->P.id, n * sizeof(int32_t), hipMemcpyDeviceToHost));
   double *src[3] = {h->P.lx, h->P.ly, h->P.lz};
   for (int c = 0; c < 3; ++c) {
-    HIPCHK(hipMemcpy(a.data(), src[c], n * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(copy_bytes(a.data(), src[c], n * sizeof(double), hipMemcpyDeviceToHost));
     for (int64_t j = 0; j < n; ++j) L[int64_t(id[j]) * 3 + c] = a[j];
   }
   return PIC_OK;
@@ -2290,10 +2297,10 @@ int pic_get_part_pos_ref(pic_handle *h, double *ref) {
   const int64_t n = h->n;
   std::vector<int32_t> id(n);
   std::vector<double> a(n);
-  HIPCHK(hipMemcpy(id.data(), h->P.id, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+  HIPCHK(copy_bytes(id.data(), h->P.id, n * sizeof(int32_t), hipMemcpyDeviceToHost));
   double *src[3] = {h->P.rx, h->P.ry, h->P.rz};
   for (int c = 0; c < 3; ++c) {
-    HIPCHK(hipMemcpy(a.data(), src[c], n * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(copy_bytes(a.data(), src[c], n * sizeof(double), hipMemcpyDeviceToHost));
     for (int64_t j = 0; j < n; ++j) ref[int64_t(id[j]) * 3 + c] = a[j];
   }
   return PIC_OK;
@@ -2306,8 +2313,8 @@ int pic_get_field_at_particle(pic_handle *h, double *F) {
   const int64_t n = h->n;
   std::vector<int32_t> id(n);
   std::vector<double> a(size_t(n) * 6);
-  HIPCHK(hipMemcpy(id.data(), h->P.id, n * sizeof(int32_t), hipMemcpyDeviceToHost));
-  HIPCHK(hipMemcpy(a.data(), h->P.F, size_t(n) * 6 * sizeof(double), hipMemcpyDeviceToHost));
+  HIPCHK(copy_bytes(id.data(), h->P.id, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+  HIPCHK(copy_bytes(a.data(), h->P.F, size_t(n) * 6 * sizeof(double), hipMemcpyDeviceToHost));
   for (int64_t j = 0; j < n; ++j)
     for (int c = 0; c < 6; ++c) F[int64_t(id[j]) * 6 + c] = a[j * 6 + c];
   return PIC_OK;
@@ -2317,7 +2324,7 @@ int pic_get_part_source(pic_handle *h, double *PS) {
   if (!h || !PS) return fail(PIC_EINVAL, "pic_get_part_source: null argument");
   if (set_device(h)) return PIC_EHIP;
   HIPCHK(hipStreamSynchronize(h->stream));
-  HIPCHK(hipMemcpy(PS, h->PS, size_t(h->hm.nElems) * h->hm.nDof * 4 * sizeof(double), hipMemcpyDeviceToHost));
+  HIPCHK(copy_bytes(PS, h->PS, size_t(h->hm.nElems) * h->hm.nDof * 4 * sizeof(double), hipMemcpyDeviceToHost));
   return PIC_OK;
 }
 
@@ -2380,7 +2387,7 @@ int pic_get_stats(pic_handle *h, pic_stats *st) {
   if (!h || !st) return fail(PIC_EINVAL, "pic_get_stats: null argument");
   if (set_device(h)) return PIC_EHIP;
   unsigned long long lost = 0;
-  HIPCHK(hipMemcpy(&lost, h->lostCount, sizeof(lost), hipMemcpyDeviceToHost));
+  HIPCHK(copy_bytes(&lost, h->lostCount, sizeof(lost), hipMemcpyDeviceToHost));
   st->n_particles = h->n;
   st->n_lost = int64_t(lost);
   st->n_steps = h->nSteps;

@@ -85,8 +85,8 @@ int pic_restart_particles(pic_handle *h, int64_t n, const double *PartData, cons
     std::vector<double> xyz(size_t(n) * 3);
     for (int64_t p = 0; p < n; ++p)
       for (int c = 0; c < 3; ++c) xyz[p * 3 + c] = state[p * 6 + c];
-    HIPCHK(hipMemcpy(h->mv.last, xyz.data(), size_t(n) * 3 * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(h->newElem, el.data(), size_t(n) * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIPCHK(copy_bytes(h->mv.last, xyz.data(), size_t(n) * 3 * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(copy_bytes(h->newElem, el.data(), size_t(n) * sizeof(int32_t), hipMemcpyHostToDevice));
     const bool refmap = h->par.tracking == PIC_TRACK_REFMAPPING;
     if (refmap) { LAUNCH_NG(k_rm_restart_check, h->hm.N, h->hm.NGeo, dim3(grid_for(n)), h->M, n, h->mv.last, h->newElem); }
     else hipLaunchKernelGGL(k_tria_restart_check, dim3(grid_for(n)), dim3(kBlock), 0, h->stream, h->M, n, h->mv.last,
@@ -94,7 +94,7 @@ int pic_restart_particles(pic_handle *h, int64_t n, const double *PartData, cons
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(h->stream));
     std::vector<int32_t> chk(n);
-    HIPCHK(hipMemcpy(chk.data(), h->newElem, size_t(n) * sizeof(int32_t), hipMemcpyDeviceToHost));
+    HIPCHK(copy_bytes(chk.data(), h->newElem, size_t(n) * sizeof(int32_t), hipMemcpyDeviceToHost));
     for (int64_t p = 0; p < n; ++p) missing += chk[p] < 0;
     if (missing > 0) {
       if (refmap) { LAUNCH_NG(k_rm_locate, h->hm.N, h->hm.NGeo, dim3(grid_for(n)), h->M, n, h->mv.last, h->newElem); }
@@ -102,7 +102,7 @@ int pic_restart_particles(pic_handle *h, int64_t n, const double *PartData, cons
                               h->newElem);
       HIPCHK(hipGetLastError());
       HIPCHK(hipStreamSynchronize(h->stream));
-      HIPCHK(hipMemcpy(chk.data(), h->newElem, size_t(n) * sizeof(int32_t), hipMemcpyDeviceToHost));
+      HIPCHK(copy_bytes(chk.data(), h->newElem, size_t(n) * sizeof(int32_t), hipMemcpyDeviceToHost));
     }
     // 1-based elements for set_particles; -1 marks a lost particle (not inside)
     for (int64_t p = 0; p < n; ++p) {
@@ -124,9 +124,9 @@ int pic_get_particle_data(pic_handle *h, int64_t cap, double *PartData, int64_t 
   const int ne = h->hm.nElems;
   std::vector<int32_t> id(n), el(n);
   std::vector<uint8_t> spec(n);
-  HIPCHK(hipMemcpy(id.data(), h->P.id, n * sizeof(int32_t), hipMemcpyDeviceToHost));
-  HIPCHK(hipMemcpy(el.data(), h->P.elem, n * sizeof(int32_t), hipMemcpyDeviceToHost));
-  HIPCHK(hipMemcpy(spec.data(), h->P.spec, n, hipMemcpyDeviceToHost));
+  HIPCHK(copy_bytes(id.data(), h->P.id, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+  HIPCHK(copy_bytes(el.data(), h->P.elem, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+  HIPCHK(copy_bytes(spec.data(), h->P.spec, n, hipMemcpyDeviceToHost));
   // particles inside, by element; within an element in caller order (the
   // reference's PEM%pNext chain runs in particle index order,
   // particles/particle_tools.f90:99-133)
@@ -157,7 +157,7 @@ int pic_get_particle_data(pic_handle *h, int64_t cap, double *PartData, int64_t 
   std::vector<double> a(n);
   double *src[6] = {h->P.x, h->P.y, h->P.z, h->P.vx, h->P.vy, h->P.vz};
   for (int c = 0; c < 6; ++c) {
-    HIPCHK(hipMemcpy(a.data(), src[c], n * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(copy_bytes(a.data(), src[c], n * sizeof(double), hipMemcpyDeviceToHost));
     for (int64_t j = 0; j < n; ++j)
       if (row[j] >= 0) PartData[row[j] * kPartDataSize + c] = a[j];
   }
