@@ -289,6 +289,103 @@ __device__ inline void eval_field(const DevMesh &M, int e, const double xi[3], d
   }
 }
 
+// ---- wave-uniform element: table reads by scalar loads ---------------------
+// When every lane of a wave maps in the same element, the element's tables
+// are read once per wave into scalar registers (s_load, loads only) instead
+// of once per lane through the vector L1, where the replicated reads of the
+// 27-point map bound the Newton.  The arithmetic is that of newton_map.
+typedef uint32_t sdw16 __attribute__((ext_vector_type(16)));
+typedef uint32_t sdw8 __attribute__((ext_vector_type(8)));
+typedef uint32_t sdw4 __attribute__((ext_vector_type(4)));
+typedef uint32_t sdw2 __attribute__((ext_vector_type(2)));
+
+__device__ inline double dw_to_d(uint32_t lo, uint32_t hi) {
+  return __builtin_bit_cast(double, (uint64_t(hi) << 32) | uint64_t(lo));
+}
+
+// values [16c, min(16c + 16, NV)) of the table at p (p wave-uniform; exact
+// size, nothing past the table is read)
+template <int NV>
+__device__ inline void sload_chunk(const double *p, int c, double buf[16]) {
+  const double *q = p + 16 * c;
+  const int m = NV - 16 * c;
+  if (m >= 16) {
+    // two loads in flight, one wait
+    sdw16 r0, r1;
+    asm volatile("s_load_dwordx16 %0, %2, 0x0\n\ts_load_dwordx16 %1, %2, 0x40\n\ts_waitcnt lgkmcnt(0)"
+                 : "=&s"(r0), "=&s"(r1) : "s"(q));
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      buf[k] = dw_to_d(r0[2 * k], r0[2 * k + 1]);
+      buf[8 + k] = dw_to_d(r1[2 * k], r1[2 * k + 1]);
+    }
+    return;
+  }
+  int k = 0;
+  if (m & 8) {
+    sdw16 r;
+    asm volatile("s_load_dwordx16 %0, %1, 0x0\n\ts_waitcnt lgkmcnt(0)" : "=s"(r) : "s"(q));
+#pragma unroll
+    for (int t = 0; t < 8; ++t) buf[t] = dw_to_d(r[2 * t], r[2 * t + 1]);
+    k = 8;
+  }
+  if (m & 4) {
+    sdw8 r;
+    asm volatile("s_load_dwordx8 %0, %1, 0x0\n\ts_waitcnt lgkmcnt(0)" : "=s"(r) : "s"(q + k));
+#pragma unroll
+    for (int t = 0; t < 4; ++t) buf[k + t] = dw_to_d(r[2 * t], r[2 * t + 1]);
+    k += 4;
+  }
+  if (m & 2) {
+    sdw4 r;
+    asm volatile("s_load_dwordx4 %0, %1, 0x0\n\ts_waitcnt lgkmcnt(0)" : "=s"(r) : "s"(q + k));
+    buf[k] = dw_to_d(r[0], r[1]);
+    buf[k + 1] = dw_to_d(r[2], r[3]);
+    k += 2;
+  }
+  if (m & 1) {
+    sdw2 r;
+    asm volatile("s_load_dwordx2 %0, %1, 0x0\n\ts_waitcnt lgkmcnt(0)" : "=s"(r) : "s"(q + k));
+    buf[k] = dw_to_d(r[0], r[1]);
+  }
+}
+
+// eval_field for a wave whose lanes are all in element e (e wave-uniform):
+// the element's E is read one k-plane at a time by scalar loads; same
+// arithmetic as eval_field.
+template <int N>
+__device__ inline void eval_field_s(const DevMesh &M, int e, const double xi[3], double U[3]) {
+#pragma clang fp contract(fast)
+  constexpr int n1 = N + 1, NP = n1 * n1 * 3;
+  double L[3][n1];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) lagrange<n1>(xi[d], M.xGP, M.wBaryGP, L[d]);
+  const double *Ue = M.E + (size_t)e * n1 * n1 * n1 * 3;
+  U[0] = U[1] = U[2] = 0.0;
+#pragma unroll 1
+  for (int k = 0; k < n1; ++k) {
+    double s[3] = {0.0, 0.0, 0.0}, t[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int ch = 0; ch < (NP + 15) / 16; ++ch) {
+      double buf[16];
+      sload_chunk<NP>(Ue + k * NP, ch, buf);
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const int v = 16 * ch + q;
+        if (v < NP) {
+          const int c = v % 3, i = (v / 3) % n1, j = v / (3 * n1);
+          if (i == 0) t[c] = 0.0;
+          t[c] += buf[q] * L[0][i];
+          if (i == n1 - 1) s[c] += t[c] * L[1][j];
+        }
+      }
+    }
+    U[0] += s[0] * L[2][k];
+    U[1] += s[1] * L[2][k];
+    U[2] += s[2] * L[2][k];
+  }
+}
+
 // GetEMFieldDW (E only), particles/pic/interpolation/pic_interpolation_tools.f90:458-572.
 // The reference's loop nest is DO k; DO l; DO m, and the EXIT taken when the
 // particle sits on a Gauss point leaves only the m-loop: weights of points
@@ -397,7 +494,8 @@ __device__ inline void affine_xi(const DevMesh &M, int e, const double x[3], dou
   xi[2] = a[9] * d0 + a[10] * d1 + a[11] * d2;
 }
 
-template <int N, bool MIXED = false, bool AFFINE = false>
+// UNI: e is wave-uniform (every lane of the wave in element e), E by scalar loads
+template <int N, bool MIXED = false, bool AFFINE = false, bool UNI = false>
 __device__ inline void field_at_particle(const DevMesh &M, int e, const double x[3], double F[6]) {
 #pragma unroll
   for (int c = 0; c < 6; ++c) F[c] = M.ext[c];
@@ -412,6 +510,7 @@ __device__ inline void field_at_particle(const DevMesh &M, int e, const double x
   double U[3];
   if (!ok && M.deposition == 1) em_field_dw<N>(M, e, x, U);
   else if (MIXED) eval_field_f32<N>(M, e, xi, U);
+  else if (UNI) eval_field_s<N>(M, e, xi, U);
   else eval_field<N>(M, e, xi, U);
   F[0] = F[0] + U[0];
   F[1] = F[1] + U[1];

@@ -102,6 +102,104 @@ __device__ inline void newton_map(const double *X, const double *dX, const doubl
   }
 }
 
+// ---- wave-uniform element: the map read by scalar loads (sload_chunk) ----
+// F = -xin + sum_r X_r (L_i L_j L_k), the association of the first evaluation
+// in newton_map ((X L_i) buff) or of the Armijo loop (X (L_i buff))
+template <int G, bool FIRST>
+__device__ inline void map_s(const double *X, const double L[3][G], const double xin[3], double F[3]) {
+  constexpr int NV = G * G * G * 3;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) F[c] = -xin[c];
+  double buff = 0.0, li = 0.0;
+#pragma unroll
+  for (int ch = 0; ch < (NV + 15) / 16; ++ch) {
+    double buf[16];
+    sload_chunk<NV>(X, ch, buf);
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int v = 16 * ch + q;
+      if (v < NV) {
+        const int r = v / 3, cc = v % 3, i = r % G, j = (r / G) % G, k = r / (G * G);
+        if (cc == 0) {
+          buff = L[1][j] * L[2][k];
+          li = FIRST ? L[0][i] : L[0][i] * buff;
+        }
+        if (FIRST) F[cc] = F[cc] + buf[q] * li * buff;
+        else F[cc] = F[cc] + buf[q] * li;
+      }
+    }
+  }
+}
+
+template <int G>
+__device__ inline void newton_map_s(const double *X, const double *dX, const double *xiCL, const double *wb,
+                                    double refEps, const double xin[3], double xi[3]) {
+  constexpr int NJ = G * G * G * 9;
+  double L[3][G], F[3], xo[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) lagrange<G>(xi[d], xiCL, wb, L[d]);
+  map_s<G, true>(X, L, xin, F);
+  double deltaXi2 = (fabs(F[0]) < kEpsMach && fabs(F[1]) < kEpsMach && fabs(F[2]) < kEpsMach) ? 0.0 : 1.0;
+  double NormF = F[0] * F[0] + F[1] * F[1] + F[2] * F[2];
+  double NormF_old;
+  int it = 0;
+  while (deltaXi2 > refEps && it < 100) {
+    ++it;
+    double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    double b2 = 0.0;
+#pragma unroll
+    for (int ch = 0; ch < (NJ + 15) / 16; ++ch) {
+      double buf[16];
+      sload_chunk<NJ>(dX, ch, buf);
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const int v = 16 * ch + q;
+        if (v < NJ) {
+          const int r = v / 9, ab = v % 9, i = r % G, j = (r / G) % G, k = r / (G * G);
+          if (ab == 0) {
+            const double buff = L[1][j] * L[2][k];
+            b2 = L[0][i] * buff;
+          }
+          J[ab / 3][ab % 3] = J[ab / 3][ab % 3] + buf[q] * b2;
+        }
+      }
+    }
+    double sdet = get_det(J);
+    if (sdet > 0.) sdet = 1. / sdet;
+    double S[3][3];
+    S[0][0] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]) * sdet;
+    S[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * sdet;
+    S[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * sdet;
+    S[1][0] = (J[1][2] * J[2][0] - J[1][0] * J[2][2]) * sdet;
+    S[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * sdet;
+    S[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * sdet;
+    S[2][0] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) * sdet;
+    S[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * sdet;
+    S[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * sdet;
+    double dxi[3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) dxi[r] = S[r][0] * F[0] + S[r][1] * F[1] + S[r][2] * F[2];
+    deltaXi2 = dxi[0] * dxi[0] + dxi[1] * dxi[1] + dxi[2] * dxi[2];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) xo[d] = xi[d];
+    NormF_old = NormF;
+    NormF = NormF * 2.;
+    double lambda = 1.0;
+    int iArmijo = 1;
+    while (NormF > NormF_old * (1. - 0.0001 * lambda) && iArmijo <= 8) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) xi[d] = xo[d] - lambda * dxi[d];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) lagrange<G>(xi[d], xiCL, wb, L[d]);
+      map_s<G, false>(X, L, xin, F);
+      lambda = 0.2 * lambda;
+      iArmijo = iArmijo + 1;
+      NormF = F[0] * F[0] + F[1] * F[1] + F[2] * F[2];
+    }
+    if (fabs(xi[0]) > 1.5 || fabs(xi[1]) > 1.5 || fabs(xi[2]) > 1.5) break;
+  }
+}
+
 // GetRefNewtonStartValue guess 3: the closest CL point of XCL_NGeo
 // (interpolation/eval_xyz.f90:580-598); guesses 1, 2 and 4 as ref_newton_start
 template <int N, int G>
@@ -214,6 +312,27 @@ __device__ inline bool rm_track_head(const DevMesh &M, int e0, double x[3], doub
     for (int g = 0; g < 4; ++g)
       if (!periodic_movement(M, x, last)) break;
   rm_position<N, G>(M, e0, x, xi);
+  return max_abs3(xi) < 1.0;
+}
+
+// rm_position / rm_track_head for a wave whose lanes all map in element e
+// (e wave-uniform): the Newton reads the map by scalar loads
+template <int N, int G>
+__device__ inline void rm_position_s(const DevMesh &M, int e, const double x[3], double xi[3]) {
+  rm_start<N, G>(M, e, x, xi);
+  if (G == 2 || M.curved[e])
+    newton_map_s<G>(M.XCL + (size_t)e * G * G * G * 3, M.dXCLg + (size_t)e * G * G * G * 9, M.xiCLg, M.wBaryCLg,
+                    M.refEps, x, xi);
+  else
+    newton_map_s<2>(M.corner + (size_t)e * 24, M.dXCL + (size_t)e * 72, M.xiCL, M.wBaryCL, M.refEps, x, xi);
+}
+
+template <int N, int G>
+__device__ inline bool rm_track_head_s(const DevMesh &M, int e0, double x[3], double last[3], double xi[3]) {
+  if (M.nPV > 0)
+    for (int g = 0; g < 4; ++g)
+      if (!periodic_movement(M, x, last)) break;
+  rm_position_s<N, G>(M, e0, x, xi);
   return max_abs3(xi) < 1.0;
 }
 

@@ -178,8 +178,13 @@ __global__ __launch_bounds__(kBlock) void k_push_inside(DevMesh M, DevParts P, i
   if (nDev) n = *nDev;   // particle count on the device (graph replay)
   bool mover = false;
   double last[3] = {0, 0, 0};
+  // whether all particles of the wave sit in one element (sorted order: most
+  // waves); its field is then read by scalar loads
+  const int eAll = i < n ? P.elem[i] : -1;
+  const int eu = __builtin_amdgcn_readfirstlane(eAll);
+  const bool uni = eu >= 0 && __ballot(i < n && eAll != eu) == 0;
   if (i < n) {
-    const int e0 = P.elem[i];
+    const int e0 = eAll;
     if (e0 < 0) {
       newElem[i] = -1;
     } else {
@@ -188,7 +193,10 @@ __global__ __launch_bounds__(kBlock) void k_push_inside(DevMesh M, DevParts P, i
       last[0] = x[0]; last[1] = x[1]; last[2] = x[2];
       const int sp = P.spec[i];
       double F[6] = {0, 0, 0, 0, 0, 0};
-      if (fabs(M.charge[sp]) > 0.0) field_at_particle<N, MIXED, AFFINE>(M, e0, x, F);
+      if (fabs(M.charge[sp]) > 0.0) {
+        if (uni) field_at_particle<N, MIXED, AFFINE, true>(M, eu, x, F);
+        else field_at_particle<N, MIXED, AFFINE>(M, e0, x, F);
+      }
       boris_push(M.charge[sp], M.mass[sp], P.isNew[i] != 0, F, dt, x, v);
       P.x[i] = x[0]; P.y[i] = x[1]; P.z[i] = x[2];
       P.vx[i] = v[0]; P.vy[i] = v[1]; P.vz[i] = v[2];
@@ -241,12 +249,13 @@ __global__ __launch_bounds__(kBlock) void k_track_movers(DevMesh M, DevParts P, 
 // ---- RefMapping (ParticleRefTracking) kernels -------------------------------
 // G = NGeo + 1 geometry points per direction.  The field is evaluated at the
 // stored PartPosRef (GetInterpolatedFieldPartPos, pic_interpolation_tools.f90:242-244).
-template <int N, bool MIXED = false>
+template <int N, bool MIXED = false, bool UNI = false>
 __device__ inline void field_at_ref(const DevMesh &M, int e, const double xi[3], double F[6]) {
 #pragma unroll
   for (int c = 0; c < 6; ++c) F[c] = M.ext[c];
   double U[3];
   if (MIXED) eval_field_f32<N>(M, e, xi, U);
+  else if (UNI) eval_field_s<N>(M, e, xi, U);
   else eval_field<N>(M, e, xi, U);
   F[0] = F[0] + U[0];
   F[1] = F[1] + U[1];
@@ -299,8 +308,13 @@ __global__ __launch_bounds__(kBlock) void k_rm_push_inside(DevMesh M, DevParts P
   const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
   if (nDev) n = *nDev;
   bool mover = false;
+  // whether all particles of the wave sit in one element (sorted order: most
+  // waves); its map is then read by scalar loads
+  const int eAll = i < n ? P.elem[i] : -1;
+  const int eu = __builtin_amdgcn_readfirstlane(eAll);
+  const bool uni = eu >= 0 && __ballot(i < n && eAll != eu) == 0;
   if (i < n) {
-    const int e0 = P.elem[i];
+    const int e0 = eAll;
     if (e0 < 0) {
       newElem[i] = -1;
     } else {
@@ -310,9 +324,12 @@ __global__ __launch_bounds__(kBlock) void k_rm_push_inside(DevMesh M, DevParts P
       double xi[3] = {P.rx[i], P.ry[i], P.rz[i]};
       const int sp = P.spec[i];
       double F[6] = {0, 0, 0, 0, 0, 0};
-      if (fabs(M.charge[sp]) > 0.0) field_at_ref<N, MIXED>(M, e0, xi, F);
+      if (fabs(M.charge[sp]) > 0.0) {
+        if (uni) field_at_ref<N, MIXED, true>(M, eu, xi, F);
+        else field_at_ref<N, MIXED>(M, e0, xi, F);
+      }
       boris_push(M.charge[sp], M.mass[sp], P.isNew[i] != 0, F, dt, x, v);
-      const bool in = rm_track_head<N, G>(M, e0, x, last, xi);
+      const bool in = uni ? rm_track_head_s<N, G>(M, eu, x, last, xi) : rm_track_head<N, G>(M, e0, x, last, xi);
       P.x[i] = x[0]; P.y[i] = x[1]; P.z[i] = x[2];
       P.vx[i] = v[0]; P.vy[i] = v[1]; P.vz[i] = v[2];
       P.rx[i] = xi[0]; P.ry[i] = xi[1]; P.rz[i] = xi[2];
