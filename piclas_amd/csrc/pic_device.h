@@ -486,8 +486,14 @@ __device__ inline void eval_field_f32(const DevMesh &M, int e, const double xi[3
 // On affine elements (parallelepipeds) the corner map is X(xi) = c + A xi and
 // RefElemNewton converges to xi = A^-1 (x - c), which is formed directly
 // (the two agree to round-off).  Used when every element is affine.
+template <bool UNI = false>
 __device__ inline void affine_xi(const DevMesh &M, int e, const double x[3], double xi[3]) {
+  double buf[16];
   const double *a = M.affine + (size_t)e * 12;
+  if (UNI) {
+    sload_chunk<12>(a, 0, buf);
+    a = buf;
+  }
   const double d0 = x[0] - a[0], d1 = x[1] - a[1], d2 = x[2] - a[2];
   xi[0] = a[3] * d0 + a[4] * d1 + a[5] * d2;
   xi[1] = a[6] * d0 + a[7] * d1 + a[8] * d2;
@@ -502,7 +508,7 @@ __device__ inline void field_at_particle(const DevMesh &M, int e, const double x
   double xi[3];
   bool ok = true;
   if (AFFINE) {
-    affine_xi(M, e, x, xi);
+    affine_xi<UNI>(M, e, x, xi);
   } else {
     ref_newton_start<N>(M, e, x, xi);
     ok = ref_elem_newton(M, e, x, xi, true);
@@ -559,6 +565,8 @@ __device__ inline void boris_push(double q, double m, bool isNew, const double F
 }
 
 // ParticleInsideQuad3D, particles/particle_mesh/particle_mesh_tools.f90:78-222
+// UNI: e is wave-uniform, the side corners come by scalar loads
+template <bool UNI = false>
 __device__ inline bool inside_quad(const DevMesh &M, int e, const double x[3], double det[6][2]) {
   const uint8_t cc = M.concave[e];
   bool in = true;
@@ -566,10 +574,19 @@ __device__ inline bool inside_quad(const DevMesh &M, int e, const double x[3], d
   for (int l = 0; l < 6; ++l) {
     const double *sx = M.sideXYZ + ((size_t)e * 6 + l) * 12;
     double A[3][4];
+    if (UNI) {
+      double buf[16];
+      sload_chunk<12>(sx, 0, buf);
 #pragma unroll
-    for (int q = 0; q < 4; ++q)
+      for (int q = 0; q < 4; ++q)
 #pragma unroll
-      for (int c = 0; c < 3; ++c) A[c][q] = sx[q * 3 + c] - x[c];
+        for (int c = 0; c < 3; ++c) A[c][q] = buf[q * 3 + c] - x[c];
+    } else {
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) A[c][q] = sx[q * 3 + c] - x[c];
+    }
     const double cp0 = A[1][0] * A[2][2] - A[2][0] * A[1][2];
     const double cp1 = A[2][0] * A[0][2] - A[0][0] * A[2][2];
     const double cp2 = A[0][0] * A[1][2] - A[1][0] * A[0][2];

@@ -202,7 +202,8 @@ __global__ __launch_bounds__(kBlock) void k_push_inside(DevMesh M, DevParts P, i
       P.vx[i] = v[0]; P.vy[i] = v[1]; P.vz[i] = v[2];
       P.isNew[i] = 0;
       double det[6][2];
-      if (inside_quad(M, e0, x, det)) newElem[i] = e0;
+      const bool in = uni ? inside_quad<true>(M, eu, x, det) : inside_quad(M, e0, x, det);
+      if (in) newElem[i] = e0;
       else mover = true;
     }
   }
