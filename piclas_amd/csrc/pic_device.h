@@ -210,7 +210,21 @@ __device__ inline bool ref_elem_newton(const DevMesh &M, int e, const double xin
 }
 
 // GetRefNewtonStartValue, interpolation/eval_xyz.f90:498-612 (guesses 1, 2, 4)
-template <int N>
+// the 3 coordinates of one point by scalar loads (p wave-uniform; exactly 24 bytes read)
+__device__ inline void sload_xyz(const double *p, double out[3]) {
+  typedef uint32_t dw4 __attribute__((ext_vector_type(4)));
+  typedef uint32_t dw2 __attribute__((ext_vector_type(2)));
+  dw4 a;
+  dw2 b;
+  asm volatile("s_load_dwordx4 %0, %2, 0x0\n\ts_load_dwordx2 %1, %2, 0x10\n\ts_waitcnt lgkmcnt(0)"
+               : "=&s"(a), "=&s"(b) : "s"(p));
+  out[0] = __builtin_bit_cast(double, (uint64_t(a[1]) << 32) | uint64_t(a[0]));
+  out[1] = __builtin_bit_cast(double, (uint64_t(a[3]) << 32) | uint64_t(a[2]));
+  out[2] = __builtin_bit_cast(double, (uint64_t(b[1]) << 32) | uint64_t(b[0]));
+}
+
+// UNI: e is wave-uniform; guess 2 then reads the Gauss points by scalar loads
+template <int N, bool UNI = false>
 __device__ inline void ref_newton_start(const DevMesh &M, int e, const double x[3], double xi[3]) {
   if (M.refGuess == 1) {
     const double *b0 = M.bary + (size_t)e * 3;
@@ -230,7 +244,10 @@ __device__ inline void ref_newton_start(const DevMesh &M, int e, const double x[
   } else if (M.refGuess == 2) {
     constexpr int n1 = N + 1;
     const double *xg = M.xGPe + (size_t)e * n1 * n1 * n1 * 3;
-    const double d0[3] = {x[0] - xg[0], x[1] - xg[1], x[2] - xg[2]};
+    double g[3];
+    if (UNI) sload_xyz(xg, g);
+    else { g[0] = xg[0]; g[1] = xg[1]; g[2] = xg[2]; }
+    const double d0[3] = {x[0] - g[0], x[1] - g[1], x[2] - g[2]};
     double win = sqrt(d0[0] * d0[0] + d0[1] * d0[1] + d0[2] * d0[2]);
     xi[0] = xi[1] = xi[2] = M.xGP[0];
 #pragma unroll 1
@@ -240,12 +257,23 @@ __device__ inline void ref_newton_start(const DevMesh &M, int e, const double x[
 #pragma unroll 1
         for (int k = 0; k < n1; ++k) {
           const int r = (k * n1 + j) * n1 + i;
-          const double dX = fabs(x[0] - xg[r * 3]);
-          if (dX > win) continue;
-          const double dY = fabs(x[1] - xg[r * 3 + 1]);
-          if (dY > win) continue;
-          const double dZ = fabs(x[2] - xg[r * 3 + 2]);
-          if (dZ > win) continue;
+          double dX, dY, dZ;
+          if (UNI) {
+            sload_xyz(xg + r * 3, g);
+            dX = fabs(x[0] - g[0]);
+            if (dX > win) continue;
+            dY = fabs(x[1] - g[1]);
+            if (dY > win) continue;
+            dZ = fabs(x[2] - g[2]);
+            if (dZ > win) continue;
+          } else {
+            dX = fabs(x[0] - xg[r * 3]);
+            if (dX > win) continue;
+            dY = fabs(x[1] - xg[r * 3 + 1]);
+            if (dY > win) continue;
+            dZ = fabs(x[2] - xg[r * 3 + 2]);
+            if (dZ > win) continue;
+          }
           const double dist = sqrt(dX * dX + dY * dY + dZ * dZ);
           if (dist < win) { win = dist; xi[0] = M.xGP[i]; xi[1] = M.xGP[j]; xi[2] = M.xGP[k]; }
         }

@@ -202,10 +202,10 @@ __device__ inline void newton_map_s(const double *X, const double *dX, const dou
 
 // GetRefNewtonStartValue guess 3: the closest CL point of XCL_NGeo
 // (interpolation/eval_xyz.f90:580-598); guesses 1, 2 and 4 as ref_newton_start
-template <int N, int G>
+template <int N, int G, bool UNI = false>
 __device__ inline void rm_start(const DevMesh &M, int e, const double x[3], double xi[3]) {
   if (M.refGuess != 3) {
-    ref_newton_start<N>(M, e, x, xi);
+    ref_newton_start<N, UNI>(M, e, x, xi);
     return;
   }
   const double *X = M.XCL + (size_t)e * G * G * G * 3;
@@ -319,7 +319,7 @@ __device__ inline bool rm_track_head(const DevMesh &M, int e0, double x[3], doub
 // (e wave-uniform): the Newton reads the map by scalar loads
 template <int N, int G>
 __device__ inline void rm_position_s(const DevMesh &M, int e, const double x[3], double xi[3]) {
-  rm_start<N, G>(M, e, x, xi);
+  rm_start<N, G, true>(M, e, x, xi);
   if (G == 2 || M.curved[e])
     newton_map_s<G>(M.XCL + (size_t)e * G * G * G * 3, M.dXCLg + (size_t)e * G * G * G * 9, M.xiCLg, M.wBaryCLg,
                     M.refEps, x, xi);
