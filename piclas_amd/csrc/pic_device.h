@@ -508,6 +508,53 @@ __device__ inline void eval_field_f32(const DevMesh &M, int e, const double xi[3
   U[0] = u0; U[1] = u1; U[2] = u2;
 }
 
+// eval_field_f32 for a wave whose lanes are all in element e (e wave-uniform):
+// the FP32 field read one k-plane at a time by scalar loads; same arithmetic
+template <int N>
+__device__ inline void eval_field_f32_s(const DevMesh &M, int e, const double xi[3], double U[3]) {
+  constexpr int n1 = N + 1, NP = n1 * n1 * 3;
+  float L[3][n1];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) lagrange_f32<n1>(float(xi[d]), M.xGP32, M.wBaryGP32, L[d]);
+  if constexpr (NP % 2 != 0) {   // planes not 8-byte aligned: the vector loads (mixed runs at N = 3)
+    eval_field_f32<N>(M, e, xi, U);
+    return;
+  }
+  const float *Ue = M.E32 + (size_t)e * n1 * n1 * n1 * 3;
+  float u[3] = {0.f, 0.f, 0.f};
+#pragma unroll 1
+  for (int k = 0; k < n1; ++k) {
+    float Lez = 0.f, w = 0.f;
+    // the plane's NP floats as NP/2 8-byte words (plane offsets are 8-byte aligned)
+    const double *Pk = reinterpret_cast<const double *>(Ue + k * NP);
+#pragma unroll
+    for (int ch = 0; ch < (NP / 2 + 15) / 16; ++ch) {
+      double dbuf[16];
+      sload_chunk<NP / 2>(Pk, ch, dbuf);
+      float buf[32];
+#pragma unroll
+      for (int t = 0; t < 16; ++t) {
+        const uint64_t bits = __builtin_bit_cast(uint64_t, dbuf[t]);
+        buf[2 * t] = __builtin_bit_cast(float, uint32_t(bits));
+        buf[2 * t + 1] = __builtin_bit_cast(float, uint32_t(bits >> 32));
+      }
+#pragma unroll
+      for (int q = 0; q < 32; ++q) {
+        const int v = 32 * ch + q;
+        if (v < NP) {
+          const int c = v % 3, i = (v / 3) % n1, j = v / (3 * n1);
+          if (c == 0) {
+            if (i == 0) Lez = L[1][j] * L[2][k];
+            w = L[0][i] * Lez;
+          }
+          u[c] = u[c] + buf[q] * w;
+        }
+      }
+    }
+  }
+  U[0] = u[0]; U[1] = u[1]; U[2] = u[2];
+}
+
 // InterpolateFieldToSingleParticle for the 508 stepper
 // (particles/pic/interpolation/pic_interpolation.f90:325-369):
 // external field + E(xi) with xi from GetPositionInRefElem.
@@ -543,6 +590,7 @@ __device__ inline void field_at_particle(const DevMesh &M, int e, const double x
   }
   double U[3];
   if (!ok && M.deposition == 1) em_field_dw<N>(M, e, x, U);
+  else if (MIXED && UNI) eval_field_f32_s<N>(M, e, xi, U);
   else if (MIXED) eval_field_f32<N>(M, e, xi, U);
   else if (UNI) eval_field_s<N>(M, e, xi, U);
   else eval_field<N>(M, e, xi, U);

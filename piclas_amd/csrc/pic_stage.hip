@@ -255,7 +255,8 @@ __device__ inline void field_at_ref(const DevMesh &M, int e, const double xi[3],
 #pragma unroll
   for (int c = 0; c < 6; ++c) F[c] = M.ext[c];
   double U[3];
-  if (MIXED) eval_field_f32<N>(M, e, xi, U);
+  if (MIXED && UNI) eval_field_f32_s<N>(M, e, xi, U);
+  else if (MIXED) eval_field_f32<N>(M, e, xi, U);
   else if (UNI) eval_field_s<N>(M, e, xi, U);
   else eval_field<N>(M, e, xi, U);
   F[0] = F[0] + U[0];
