@@ -1050,8 +1050,15 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE_MX))
   const double ulo = fmin(fmin(u[0], u[1]), fmin(u[2], u[3])), uhi = fmax(fmax(u[0], u[1]), fmax(u[2], u[3]));
   const double vlo = fmin(fmin(v[0], v[1]), fmin(v[2], v[3])), vhi = fmax(fmax(v[0], v[1]), fmax(v[2], v[3]));
   const double wlo = fmin(fmin(w[0], w[1]), fmin(w[2], w[3])), whi = fmax(fmax(w[0], w[1]), fmax(w[2], w[3]));
-  const double ui = (xg[(lane & 3) * 3 + 0] - bx) * Ri;
-  const double vj = (xg[4 * ((lane >> 2) & 3) * 3 + 1] - by) * Ri;
+  // Staged coordinates are taken relative to the centre of the DOF box, so
+  // that the cull is |p| - half width per axis (the absolute value is a free
+  // operand modifier); the DOF constants move to the same frame.
+  const double uc = 0.5 * (ulo + uhi), vc = 0.5 * (vlo + vhi), wc = 0.5 * (wlo + whi);
+  const double uh = 0.5 * (uhi - ulo), vh = 0.5 * (vhi - vlo), wh = 0.5 * (whi - wlo);
+#pragma unroll
+  for (int a = 0; a < 4; ++a) w[a] -= wc;
+  const double ui = (xg[(lane & 3) * 3 + 0] - bx) * Ri - uc;
+  const double vj = (xg[4 * ((lane >> 2) & 3) * 3 + 1] - by) * Ri - vc;
   const double Kij = 1.0 - ui * ui - vj * vj, tu = 2.0 * ui, tv = 2.0 * vj;
   const double box2 = 1.0 + 1e-12;
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
@@ -1086,16 +1093,20 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE_MX))
       const int s0 = __builtin_amdgcn_readlane(ms0, j), s1 = __builtin_amdgcn_readlane(ms1, j);
       const int si = __builtin_amdgcn_readlane(msi, j);
       const double ox = A.sh[si][0] - bx, oy = A.sh[si][1] - by, oz = A.sh[si][2] - bz;
-      for (int base = s0; base < s1; base += kWave) {
-        const int i = base + lane;
-        const bool valid = i < s1;
-        const size_t ic = size_t(valid ? i : s1 - 1);
-        const double4 ra = A.rec[2 * ic], rb = A.rec[2 * ic + 1];
-        // shifted position (GetPartPosShifted) relative to the barycentre, scaled
-        const double pu = (ra.x + ox) * Ri, pv = (ra.y + oy) * Ri, pw = (ra.z + oz) * Ri;
-        const double gx = fmax(0.0, fmax(ulo - pu, pu - uhi));
-        const double gy = fmax(0.0, fmax(vlo - pv, pv - vhi));
-        const double gz = fmax(0.0, fmax(wlo - pw, pw - whi));
+      // the run's records from a wave-uniform base, addressed by 32-bit offsets
+      const char *rr = reinterpret_cast<const char *>(A.rec + 2 * (size_t)s0);
+      const int last = s1 - 1 - s0;
+      for (int base = 0; base <= last; base += kWave) {
+        const int k = base + lane;
+        const bool valid = k <= last;
+        const unsigned o = unsigned(valid ? k : last) * 64u;
+        const double4 ra = *reinterpret_cast<const double4 *>(rr + o);
+        const double4 rb = *reinterpret_cast<const double4 *>(rr + o + 32u);
+        // shifted position (GetPartPosShifted) relative to the DOF-box centre, scaled
+        const double pu = fma(ra.x + ox, Ri, -uc), pv = fma(ra.y + oy, Ri, -vc), pw = fma(ra.z + oz, Ri, -wc);
+        const double gx = fmax(fabs(pu) - uh, 0.0);
+        const double gy = fmax(fabs(pv) - vh, 0.0);
+        const double gz = fmax(fabs(pw) - wh, 0.0);
         const bool pass = valid && (rb.w != 0.0) && (gx * gx + gy * gy + gz * gz <= box2);
         const unsigned long long m = __ballot(pass);
         if (m == 0) continue;
