@@ -85,6 +85,13 @@ def lib():
         L.om_get_nparticles.argtypes = [ctypes.c_void_p]
         L.om_get_lost_count.restype = ctypes.c_int64
         L.om_get_lost_count.argtypes = [ctypes.c_void_p]
+        L.om_get_multi_crossings.restype = ctypes.c_int64
+        L.om_get_multi_crossings.argtypes = [ctypes.c_void_p]
+        L.om_plant_mistake.restype = None
+        L.om_plant_mistake.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.om_set_tiebreak_log.argtypes = [ctypes.c_void_p, ctypes.c_int64]
+        L.om_get_tiebreak_log.restype = ctypes.c_int64
+        L.om_get_tiebreak_log.argtypes = [ctypes.c_void_p, _f64p]
         L.om_get_n_unique_nodes.restype = ctypes.c_int32
         L.om_get_n_unique_nodes.argtypes = [ctypes.c_void_p]
         for name in ("om_deposited_charge", "om_particle_charge", "om_field_energy"):
@@ -201,6 +208,24 @@ class Oracle:
 
     def lost(self):
         return lib().om_get_lost_count(self.h)
+
+    def multi_crossings(self):
+        return lib().om_get_multi_crossings(self.h)
+
+    def tiebreak_log(self, cap=None):
+        """With cap: start recording up to cap multi-crossing events of the
+        TriaTracking walk.  Without: the records so far, shape (n, 29)."""
+        if cap is not None:
+            lib().om_set_tiebreak_log(self.h, int(cap))
+            return None
+        n = lib().om_get_tiebreak_log(self.h, None)
+        a = np.zeros((max(n, 1), 29))
+        lib().om_get_tiebreak_log(self.h, _p(a, _f64p))
+        return a[:n]
+
+    def plant_mistake(self, which):
+        """Test only: 1 makes the TriaTracking tie-break take the side crossed last."""
+        lib().om_plant_mistake(self.h, which)
 
     def elem_xgp(self):
         a = np.zeros((self.nElems, self.nDof, 3))

@@ -200,6 +200,10 @@ struct om_state {
   int32_t *elem, *lastElem, *species; /* 0-based elem; -1 unknown */
   uint8_t *inside, *isNew;
   int64_t lost;
+  int64_t multiCross; /* walk iterations that took the multi-crossing branch */
+  double *tbLog;      /* tie-break events (om_set_tiebreak_log), OM_TB_REC doubles each */
+  int64_t tbCap, tbN;
+  int plant;          /* planted mistakes (om_plant_mistake), 0 normally */
   int nthreads;
 };
 
@@ -1074,6 +1078,7 @@ void om_destroy(om_state *s) {
                   s->LastPartPos, s->Field, s->elem, s->lastElem, s->species, s->inside, s->isNew,
                   s->curved, s->dXCL1, s->epsOneCell, s->openElem, s->PartPosRef, s->Gop, s->rhoBar, s->Enode};
   for (size_t i = 0; i < sizeof ptrs / sizeof ptrs[0]; i++) free(ptrs[i]);
+  free(s->tbLog);
   free(s);
 }
 
@@ -1118,6 +1123,21 @@ int om_set_particles(om_state *s, int32_t n, const double *PartState, const int3
 
 int32_t om_get_nparticles(const om_state *s) { return s->n; }
 int64_t om_get_lost_count(const om_state *s) { return s->lost; }
+int64_t om_get_multi_crossings(const om_state *s) { return s->multiCross; }
+int om_set_tiebreak_log(om_state *s, int64_t cap) {
+  free(s->tbLog);
+  s->tbLog = NULL;
+  s->tbN = 0;
+  s->tbCap = cap > 0 ? cap : 0;
+  if (s->tbCap) s->tbLog = xcalloc((size_t)s->tbCap * OM_TB_REC, sizeof(double));
+  return 0;
+}
+int64_t om_get_tiebreak_log(const om_state *s, double *out) {
+  const int64_t n = s->tbN < s->tbCap ? s->tbN : s->tbCap;
+  if (out && n) memcpy(out, s->tbLog, (size_t)n * OM_TB_REC * sizeof(double));
+  return n;
+}
+void om_plant_mistake(om_state *s, int which) { s->plant = which; }
 
 int om_get_particles(const om_state *s, double *PartState, int32_t *elem, uint8_t *inside) {
   if (PartState) memcpy(PartState, s->PartState, (size_t)s->n * 6 * sizeof(double));
@@ -1334,6 +1354,9 @@ static int tria_track_one(om_state *s, int i) {
       } else {
         int second = 0;
         double minRatio = 0;
+#pragma omp atomic
+        s->multiCross++;
+        int tbFlag[6] = {0};
         for (int i2 = 0; i2 < nThrough; i2++) {
           int doCheck = 1;
           for (int is = 2; is <= 6; is++)
@@ -1341,8 +1364,10 @@ static int tria_track_one(om_state *s, int i) {
                 DoneLastElem[is - 1][2] == TriNumTemp[i2])
               doCheck = 0;
           if (!doCheck) continue;
+          tbFlag[i2] = 1;
           double detM;
           if (through_side_last_pos(s, last, LocSidesTemp[i2], ElemID, TriNumTemp[i2], &detM)) {
+            tbFlag[i2] = 3;
             double dPart = det[LocSidesTemp[i2] - 1][TriNumTemp[i2] - 1];
             if (detM == 0 && dPart == 0) continue;
             if (detM == 0 && minRatio == 0) {
@@ -1353,13 +1378,28 @@ static int tria_track_one(om_state *s, int i) {
             } else {
               if (detM == 0) continue;
               double ratio = dPart / detM;
-              if (ratio < minRatio) {
+              /* planted mistake 1 (tests only): take the side crossed last */
+              const int take = (s->plant == 1) ? (second == 0 || ratio > minRatio) : (ratio < minRatio);
+              if (take) {
                 minRatio = ratio;
                 second++;
                 SideID = GlobSideTemp[i2] - 1;
                 LocalSide = LocSidesTemp[i2];
                 TriNum = TriNumTemp[i2];
               }
+            }
+          }
+        }
+        if (s->tbCap) {
+          int64_t slot;
+#pragma omp atomic capture
+          slot = s->tbN++;
+          if (slot < s->tbCap) {
+            double *r = s->tbLog + slot * OM_TB_REC;
+            r[0] = i; r[1] = ElemID + 1; r[2] = nThrough; r[3] = second ? LocalSide : 0; r[4] = second ? TriNum : 0;
+            for (int c = 0; c < 3; c++) { r[5 + c] = last[c]; r[8 + c] = x[c]; }
+            for (int i2 = 0; i2 < nThrough; i2++) {
+              r[11 + 3 * i2] = LocSidesTemp[i2]; r[12 + 3 * i2] = TriNumTemp[i2]; r[13 + 3 * i2] = tbFlag[i2];
             }
           }
         }

@@ -87,6 +87,23 @@ int  om_get_last_part_pos(const om_state *s, double *LastPartPos);
 int  om_get_field_at_particle(const om_state *s, double *FieldAtParticle);
 int  om_get_part_source(const om_state *s, double *PS);
 int64_t om_get_lost_count(const om_state *s);
+/* How often the TriaTracking walk took the multi-crossing branch
+ * (particle_triatracking.f90:309-405), summed over all calls. */
+int64_t om_get_multi_crossings(const om_state *s);
+/* Test only: plant a known mistake so that a test can show its checks catch
+ * it.  1: the multi-crossing tie-break takes the side crossed last instead of
+ * first.  0 restores the restatement. */
+void om_plant_mistake(om_state *s, int which);
+/* Test only: record up to cap multi-crossing events of the TriaTracking walk
+ * (0 switches the log off).  One record is OM_TB_REC doubles: particle index,
+ * 1-based element, number of candidate sides, the chosen local side and
+ * triangle (0, 0 when the particle was lost), LastPartPos(3), PartState(3),
+ * then per candidate: local side, triangle, flags (bit 0: not excluded by
+ * DoneLastElem, bit 1: the last position lies on the element's side of the
+ * triangle, i.e. ParticleThroughSideLastPosCheck held). */
+#define OM_TB_REC 29
+int  om_set_tiebreak_log(om_state *s, int64_t cap);
+int64_t om_get_tiebreak_log(const om_state *s, double *out);  /* returns the number of records */
 
 /* Derived tables, for checking the product's own mesh producers. */
 int  om_get_elem_xgp(const om_state *s, double *xgp);        /* [nElems][(N+1)^3][3] */

@@ -391,6 +391,24 @@ def test_deformed_box_parity(oracle_mod, depo):
     s.close()
 
 
+def test_tiebreak_deformed_box_parity(oracle_mod):
+    """TriaTracking's multi-crossing branch (several sides register a
+    crossing; the side crossed first is kept) on a strongly deformed periodic
+    box with moves of about 0.75 element widths per step and no field: 3
+    steps against the oracle, ownership bit-exact.  The oracle's own check
+    that the kept side is the first crossed is in the closed-form tests."""
+    w = deformed_box(W.thermal_cube(nx=5, ppc=40, deposition=stage.DEPO_CVWM, steps=3), amp=0.3)
+    rng = np.random.default_rng(3)
+    w.PartState[:, 3:] = rng.normal(size=(len(w.PartState), 3)) * 1.5e5
+    w.E[:] = 0.0
+    w.dt = 1e-6
+    s, o = run_pair(oracle_mod, w, 3, sf_radius=0.0, external_field=np.zeros(6))
+    print("multi-crossing walk steps in the oracle: %d" % o.multi_crossings())
+    assert o.multi_crossings() >= 20
+    assert compare(s, o, 1.0) == 0
+    s.close()
+
+
 def test_device_resident_field_and_source():
     """pic_set_field_device / pic_part_source_device: the field handed over
     from device memory gives the same steps as the host hand-over

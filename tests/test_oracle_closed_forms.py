@@ -6,7 +6,9 @@ weights against numpy's Gauss-Legendre rule, Lagrange cardinality and
 polynomial exactness, Newton round trips on affine and twisted hexahedra,
 Boris invariants (|u| under pure B, the rotation angle, the exact momentum
 update under pure E, second-order convergence), ownership against the
-closed-form Cartesian owner, charge conservation of cell_volweight_mean, a
+closed-form Cartesian owner, the TriaTracking tie-break keeping the side
+crossed first (with a planted mistake that the check must catch), charge
+conservation of cell_volweight_mean, a
 uniform lattice giving uniform density, and the shape-function deposit
 against a brute-force sum over all DOFs and periodic images.
 """
@@ -203,6 +205,89 @@ def test_tracking_open_boundary_removes(oracle_mod):
     assert np.array_equal(~ins, exits)
     assert o.lost() == 0          # removed at an open boundary, not lost
     assert np.array_equal(el[ins], m.element_of(ps[ins, :3]))
+
+
+def _deformed_box(shape, amp):
+    """Generated periodic unit box with its nodes displaced by a smooth
+    periodic field (bilinear, partly concave faces)."""
+    m = M.cartesian_box(shape)
+    X = m.NodeCoords
+    x, y, z = X[:, 0].copy(), X[:, 1].copy(), X[:, 2].copy()
+    a = amp / shape[0]
+    s, c = (lambda u: np.sin(2 * np.pi * u)), (lambda u: np.cos(2 * np.pi * u))
+    X[:, 0] = x + a * s(x) * c(y) * c(z)
+    X[:, 1] = y + a * c(x) * s(y) * c(z)
+    X[:, 2] = z + a * c(x) * c(y) * s(z)
+    return m
+
+
+def _first_crossing_violations(o, m, log, tol=1e-12):
+    """For each recorded multi-crossing event of the TriaTracking walk, the
+    parameter s in (0, 1] at which the segment LastPartPos -> PartState
+    crosses the plane of each candidate triangle, computed here from the node
+    coordinates.  The reference keeps the candidate with the smallest
+    det(PartState)/det(LastPartPos) (particle_triatracking.f90:309-405); as
+    det is proportional to the signed distance from the triangle's plane,
+    that ratio is r = -d_new/d_last and s = 1/(1 - r), so the kept side must
+    be the one crossed first.  Returns (events with two or more crossed
+    candidates whose s differ by more than 1e-9, events whose kept side is
+    not the first crossed)."""
+    sid = o.elem_side_node_id()
+    X = m.NodeCoords
+    distinct = bad = 0
+    for r in log:
+        if r[3] == 0:
+            continue
+        e, n, last, x = int(r[1]) - 1, int(r[2]), r[5:8], r[8:11]
+        cand = []
+        for k in range(n):
+            ls, tn, flags = int(r[11 + 3 * k]), int(r[12 + 3 * k]), int(r[13 + 3 * k])
+            if flags != 3:
+                continue
+            ids = sid[e, ls - 1]
+            p1, p2, p3 = X[ids[0]], X[ids[tn]], X[ids[tn + 1]]
+            nrm = np.cross(p2 - p1, p3 - p1)
+            dl, dx = np.dot(last - p1, nrm), np.dot(x - p1, nrm)
+            if dl != dx:
+                cand.append((dl / (dl - dx), ls, tn))
+        if len(cand) < 2:
+            continue
+        s_all = [c[0] for c in cand]
+        s_kept = [c[0] for c in cand if (c[1], c[2]) == (int(r[3]), int(r[4]))][0]
+        distinct += max(s_all) - min(s_all) > 1e-9
+        bad += s_kept > min(s_all) + tol
+    return distinct, bad
+
+
+def test_tracking_tiebreak_first_crossing(oracle_mod):
+    """The multi-crossing branch of TriaTracking keeps the side that the
+    trajectory crosses first.  On a strongly deformed periodic box (faces
+    split into two triangles, partly concave), moves of about 0.75 element
+    widths produce walk steps where several sides register a crossing at
+    different points of the trajectory; every kept side must be the first
+    crossed, and a planted mistake (keep the side crossed last) must be
+    caught on every such step."""
+    m = _deformed_box((5, 5, 5), 0.3)
+    rng = np.random.default_rng(3)
+    n = 30000
+    x = rng.uniform(0, 1, (n, 3))
+    v = rng.normal(size=(n, 3)) * 1.5e5
+    res = {}
+    for plant in (0, 1):
+        o = oracle_mod.Oracle(m, 1)
+        o.plant_mistake(plant)
+        o.tiebreak_log(200000)
+        o.set_particles(np.hstack([x, v]), elem=np.zeros(n, np.int32))
+        o.locate()
+        for _ in range(3):
+            o.time_step(1e-6, 1)
+        log = o.tiebreak_log()
+        assert len(log) == o.multi_crossings()
+        res[plant] = _first_crossing_violations(o, m, log) + (o.lost(),)
+    distinct, bad, lost = res[0]
+    assert distinct >= 50, res
+    assert bad == 0 and lost == 0, res
+    assert res[1][1] == res[1][0] >= 50, res      # the planted mistake fails every distinct step
 
 
 def test_cvwm_charge_conservation(oracle_mod):
