@@ -65,7 +65,7 @@ enum {
   PIC_KERNEL_SF_VALU = 2,       /* k_sf_gather: shape function on the vector ALUs */
   PIC_KERNEL_SF_MATRIX = 3,     /* k_sf_gather_mfma: sums on the matrix cores     */
   PIC_KERNEL_SF_CARTESIAN = 4,  /* k_sf_gather_cart: axis-aligned tensor grids    */
-  PIC_KERNEL_SF_MIXED = 5       /* k_sf_gather_mixed: FP32 sums                   */
+  PIC_KERNEL_SF_MIXED = 5       /* k_sf_gather_mixed(_cart): FP32 sums           */
 };
 
 /* Mesh in HOPR layout, as the reference's mesh reader holds it

@@ -1152,23 +1152,20 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE_MX))
   for (int Q = 0; Q < 4; ++Q) PS[((size_t)t * nd + 16 * Q + dofLo) * 4 + jc] = acc[Q];
 }
 
-// Mixed-precision shape-function gather for N = 3: S and the products in
-// FP32, summed in FP32 on the matrix cores (v_mfma_f32_4x4x1, 16 blocks of
-// 4x4: lane 4b+i gives A_b[i] = S^alpha of DOF lane, lane 4b+j gives
-// B_b[j] = f (v,1)_j of the particle, lane 4b+j holds D_b[i][j] in register
-// i), then added into FP64 after every batch of 64 staged particles.
-// Positions relative to the target barycentre and the cull are formed in
-// FP64 from the same records and runs as the FP64 gathers.  Records in LDS
-// (FP32): general elements px, py, pz, |p|^2/R^2, f (v,1); axis-aligned
-// tensor grids (CART) 1 - (u_i - pu)^2 - (v_j - pv)^2 for the 16 (i,j),
-// (w_k - pw)^2 for k = 0..3 and f (v,1), so that a pair costs a
-// subtraction, the clamp and the square.
+// Mixed-precision shape-function gather for N = 3 on general elements: S and
+// the products in FP32, summed in FP32 on the matrix cores
+// (v_mfma_f32_4x4x1, 16 blocks of 4x4: lane 4b+i gives A_b[i] = S^alpha of
+// DOF lane, lane 4b+j gives B_b[j] = f (v,1)_j of the particle, lane 4b+j
+// holds D_b[i][j] in register i), then added into FP64 after every batch of
+// 64 staged particles.  Positions relative to the target barycentre and the
+// cull are formed in FP64 from the same records and runs as the FP64
+// gathers.  Records in LDS (FP32): px, py, pz, |p|^2/R^2, f (v,1).
 typedef float v4f __attribute__((ext_vector_type(4)));
-template <int ALPHA, bool CART>
+template <int ALPHA>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE_MX))) void k_sf_gather_mixed(DevMesh M, const int32_t *start, SFArgs A, double *PS) {
 #pragma clang fp contract(fast)
   constexpr int nd = 64, kRing = 128;
-  constexpr int kRec = CART ? 24 : 8;
+  constexpr int kRec = 8;
   __shared__ float lds[kBlock / kWave][kRing][kRec];
   const int wib = threadIdx.x / kWave;
   const int lane = lane_id();
@@ -1176,41 +1173,24 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE_MX))
   const int t = __builtin_amdgcn_readfirstlane(lb * (kBlock / kWave) + wib);
   if (t >= M.nElems) return;
   const double bx = M.bary[t * 3], by = M.bary[t * 3 + 1], bz = M.bary[t * 3 + 2];
-  const double R2i0 = A.r2_sf_inv, Ri = A.r_inv;
+  const double R2i0 = A.r2_sf_inv;
   const double *xg = M.xGPe + (size_t)t * nd * 3;
-  // general elements: S = cd - qq + m.q for DOF lane
-  float mx = 0.f, my = 0.f, mz = 0.f, cd = 0.f;
-  // tensor grids: scaled DOF coordinates u_i, v_j, w_k
-  double u[4], v[4], w[4];
-  double blo[3], bhi[3];
-  if (CART) {
+  // S = cd - qq + m.q for DOF lane
+  const double dx = xg[lane * 3] - bx, dy = xg[lane * 3 + 1] - by, dz = xg[lane * 3 + 2] - bz;
+  const float mx = float(2.0 * R2i0 * dx), my = float(2.0 * R2i0 * dy), mz = float(2.0 * R2i0 * dz);
+  const float cd = float(1.0 - R2i0 * (dx * dx + dy * dy + dz * dz));
+  double blo[3] = {dx, dy, dz}, bhi[3] = {dx, dy, dz};
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      u[q] = (xg[q * 3 + 0] - bx) * Ri;
-      v[q] = (xg[4 * q * 3 + 1] - by) * Ri;
-      w[q] = (xg[16 * q * 3 + 2] - bz) * Ri;
+  for (int off = 32; off >= 1; off >>= 1)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      blo[c] = fmin(blo[c], __shfl_xor(blo[c], off, kWave));
+      bhi[c] = fmax(bhi[c], __shfl_xor(bhi[c], off, kWave));
     }
-    blo[0] = fmin(fmin(u[0], u[1]), fmin(u[2], u[3])); bhi[0] = fmax(fmax(u[0], u[1]), fmax(u[2], u[3]));
-    blo[1] = fmin(fmin(v[0], v[1]), fmin(v[2], v[3])); bhi[1] = fmax(fmax(v[0], v[1]), fmax(v[2], v[3]));
-    blo[2] = fmin(fmin(w[0], w[1]), fmin(w[2], w[3])); bhi[2] = fmax(fmax(w[0], w[1]), fmax(w[2], w[3]));
-  } else {
-    const double dx = xg[lane * 3] - bx, dy = xg[lane * 3 + 1] - by, dz = xg[lane * 3 + 2] - bz;
-    mx = float(2.0 * R2i0 * dx); my = float(2.0 * R2i0 * dy); mz = float(2.0 * R2i0 * dz);
-    cd = float(1.0 - R2i0 * (dx * dx + dy * dy + dz * dz));
-    blo[0] = bhi[0] = dx; blo[1] = bhi[1] = dy; blo[2] = bhi[2] = dz;
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1)
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        blo[c] = fmin(blo[c], __shfl_xor(blo[c], off, kWave));
-        bhi[c] = fmax(bhi[c], __shfl_xor(bhi[c], off, kWave));
-      }
-  }
   const double thr = A.r_sf + M.radius[t];
-  // box test threshold: in scaled coordinates 1, else R^2
-  const double box2 = (CART ? 1.0 : A.r2_sf) * (1.0 + 1e-12);
+  const double box2 = A.r2_sf * (1.0 + 1e-12);
   float (*L)[kRec] = lds[wib];
-  const int jc = lane & 3, wq = lane & 15, zk = lane >> 4;
+  const int jc = lane & 3;
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
   // the batch of n staged records from ring position h0 (no wrap)
   auto batch = [&](int h0, int n) {
@@ -1218,12 +1198,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE_MX))
     v4f c = {0.f, 0.f, 0.f, 0.f};
     for (int p = 0; p < n; ++p) {
       const float *r = Lh[p];
-      float S;
-      if (CART) {
-        S = clamp01f(r[wq] - r[16 + zk]);
-      } else {
-        S = clamp01f(mx * r[0] + (my * r[1] + (mz * r[2] + (cd - r[3]))));
-      }
+      const float S = clamp01f(mx * r[0] + (my * r[1] + (mz * r[2] + (cd - r[3]))));
       float S1 = S * S;
       if (ALPHA == 0) {
         for (int ex = 3; ex <= A.alpha; ++ex) S1 = S * S1;
@@ -1231,7 +1206,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE_MX))
 #pragma unroll
         for (int ex = 3; ex <= ALPHA; ++ex) S1 = S * S1;
       }
-      c = __builtin_amdgcn_mfma_f32_4x4x1f32(S1, r[(CART ? 20 : 4) + jc], c, 0, 0, 0);
+      c = __builtin_amdgcn_mfma_f32_4x4x1f32(S1, r[4 + jc], c, 0, 0, 0);
     }
 #pragma unroll
     for (int q = 0; q < 4; ++q) acc[q] += double(c[q]);
@@ -1253,36 +1228,19 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE_MX))
         const bool valid = i < s1;
         const size_t ic = size_t(valid ? i : s1 - 1);
         const double4 ra = A.rec[2 * ic], rb = A.rec[2 * ic + 1];
-        double px = ra.x + ox, py = ra.y + oy, pz = ra.z + oz;
-        if (CART) { px *= Ri; py *= Ri; pz *= Ri; }
+        const double px = ra.x + ox, py = ra.y + oy, pz = ra.z + oz;
         const double gx = fmax(0.0, fmax(blo[0] - px, px - bhi[0]));
         const double gy = fmax(0.0, fmax(blo[1] - py, py - bhi[1]));
         const double gz = fmax(0.0, fmax(blo[2] - pz, pz - bhi[2]));
         bool pass = valid && (rb.w != 0.0) && (gx * gx + gy * gy + gz * gz <= box2);
-        if (!CART && !A.linear) pass = pass && !(sqrt(px * px + py * py + pz * pz) > thr);
+        if (!A.linear) pass = pass && !(sqrt(px * px + py * py + pz * pz) > thr);
         const unsigned long long m = __ballot(pass);
         if (m == 0) continue;
         if (pass) {
           float *r = L[(tail + __popcll(m & ((1ull << lane) - 1ull))) & (kRing - 1)];
-          if (CART) {
-            float X[4], Y[4];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-              const float a = float(u[q] - px), b = float(v[q] - py), c = float(w[q] - pz);
-              X[q] = 1.f - a * a;
-              Y[q] = b * b;
-              r[16 + q] = c * c;
-            }
-#pragma unroll
-            for (int jj = 0; jj < 4; ++jj)
-#pragma unroll
-              for (int ii = 0; ii < 4; ++ii) r[ii + 4 * jj] = X[ii] - Y[jj];
-            r[20] = float(rb.x); r[21] = float(rb.y); r[22] = float(rb.z); r[23] = float(rb.w);
-          } else {
-            r[0] = float(px); r[1] = float(py); r[2] = float(pz);
-            r[3] = float(R2i0 * (px * px + py * py + pz * pz));
-            r[4] = float(rb.x); r[5] = float(rb.y); r[6] = float(rb.z); r[7] = float(rb.w);
-          }
+          r[0] = float(px); r[1] = float(py); r[2] = float(pz);
+          r[3] = float(R2i0 * (px * px + py * py + pz * pz));
+          r[4] = float(rb.x); r[5] = float(rb.y); r[6] = float(rb.z); r[7] = float(rb.w);
         }
         tail += __popcll(m);
         if (tail - head < kWave) continue;
@@ -1304,6 +1262,157 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE_MX))
 #pragma unroll
   for (int q = 0; q < 4; ++q) PS[((size_t)t * nd + 4 * (lane >> 2) + q) * 4 + jc] = acc[q];
 }
+
+// Mixed-precision gather on axis-aligned tensor grids (N = 3).  The cull and
+// the ring follow k_sf_gather_cart; the ring keeps each survivor's position
+// (FP64, relative to the DOF-box centre, scaled by 1/r_sf) and f (v,1) in
+// FP32.  Once 64 are pending, one pass with a survivor per lane forms the 16
+// (i,j) parts 1 - (u_i - pu)^2 - (v_j - pv)^2 and the 4 k parts
+// (w_k - pw)^2 of S in FP32.  The products then run 16 particles per
+// sequence of v_mfma_f32_4x4x1 (16 blocks): operand A is f (v,1) of the
+// particle, taken for every block from the lanes 4p..4p+3 that hold
+// particle p (broadcast by the instruction's CBSZ = 4, ABID = p), operand B
+// is S^alpha of DOF lane, and lane l gathers the four source components of
+// DOF l in its four result registers.  A pair costs two LDS reads, the
+// subtraction with the clamp and the square; the FP32 sums of each batch of
+// 64 are added into FP64.
+#define SF_MIXED_STEP(P)                                                         \
+  {                                                                              \
+    const float *o_ = Pr[g + (P)];                                               \
+    const float S_ = clamp01f(o_[wq] - o_[16 + zk]);                             \
+    float S1_ = S_ * S_;                                                         \
+    if (ALPHA == 0) {                                                            \
+      for (int ex = 3; ex <= A.alpha; ++ex) S1_ = S_ * S1_;                      \
+    } else {                                                                     \
+      _Pragma("unroll") for (int ex = 3; ex <= ALPHA; ++ex) S1_ = S_ * S1_;      \
+    }                                                                            \
+    c = __builtin_amdgcn_mfma_f32_4x4x1f32(fa, S1_, c, 4, (P), 0);               \
+  }
+template <int ALPHA>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE_MX))) void k_sf_gather_mixed_cart(DevMesh M, const int32_t *start, SFArgs A, double *PS) {
+#pragma clang fp contract(fast)
+  constexpr int nd = 64, kRing = 128;
+  __shared__ double ringP[kBlock / kWave][kRing][3];
+  __shared__ float ringF[kBlock / kWave][kRing][4];
+  __shared__ float prep[kBlock / kWave][kWave][20];
+  const int wib = threadIdx.x / kWave;
+  const int lane = lane_id();
+  const int lb = xcd_block(blockIdx.x, gridDim.x);
+  const int t = __builtin_amdgcn_readfirstlane(lb * (kBlock / kWave) + wib);
+  if (t >= M.nElems) return;
+  const double bx = M.bary[t * 3], by = M.bary[t * 3 + 1], bz = M.bary[t * 3 + 2];
+  const double Ri = A.r_inv;
+  const double *xg = M.xGPe + (size_t)t * nd * 3;
+  double u[4], v[4], w[4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    u[a] = (xg[a * 3 + 0] - bx) * Ri;
+    v[a] = (xg[4 * a * 3 + 1] - by) * Ri;
+    w[a] = (xg[16 * a * 3 + 2] - bz) * Ri;
+  }
+  const double ulo = fmin(fmin(u[0], u[1]), fmin(u[2], u[3])), uhi = fmax(fmax(u[0], u[1]), fmax(u[2], u[3]));
+  const double vlo = fmin(fmin(v[0], v[1]), fmin(v[2], v[3])), vhi = fmax(fmax(v[0], v[1]), fmax(v[2], v[3]));
+  const double wlo = fmin(fmin(w[0], w[1]), fmin(w[2], w[3])), whi = fmax(fmax(w[0], w[1]), fmax(w[2], w[3]));
+  const double uc = 0.5 * (ulo + uhi), vc = 0.5 * (vlo + vhi), wc = 0.5 * (wlo + whi);
+  const double uh = 0.5 * (uhi - ulo), vh = 0.5 * (vhi - vlo), wh = 0.5 * (whi - wlo);
+#pragma unroll
+  for (int a = 0; a < 4; ++a) { u[a] -= uc; v[a] -= vc; w[a] -= wc; }
+  const double box2 = 1.0 + 1e-12;
+  double (*RP)[3] = ringP[wib];
+  float (*RF)[4] = ringF[wib];
+  float (*Pr)[20] = prep[wib];
+  // rows past the last record of a batch are read with zero weight: keep them finite
+#pragma unroll
+  for (int q = 0; q < 20; ++q) Pr[lane][q] = 0.f;
+  const int wq = lane & 15, zk = lane >> 4;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  // the batch of n <= 64 ring records from position h0 (no wrap)
+  auto batch = [&](int h0, int n) {
+    if (lane < n) {
+      const double *p = RP[(h0 + lane) & (kRing - 1)];
+      float *o = Pr[lane];
+      float X[4], Y[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const float a = float(u[q] - p[0]), b = float(v[q] - p[1]), cz = float(w[q] - p[2]);
+        X[q] = 1.f - a * a;
+        Y[q] = b * b;
+        o[16 + q] = cz * cz;
+      }
+#pragma unroll
+      for (int jj = 0; jj < 4; ++jj)
+#pragma unroll
+        for (int ii = 0; ii < 4; ++ii) o[ii + 4 * jj] = X[ii] - Y[jj];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    v4f c = {0.f, 0.f, 0.f, 0.f};
+    for (int g = 0; g < n; g += 16) {
+      const int pf = g + (lane >> 2);
+      const float fa = pf < n ? RF[(h0 + pf) & (kRing - 1)][lane & 3] : 0.f;
+      SF_MIXED_STEP(0) SF_MIXED_STEP(1) SF_MIXED_STEP(2) SF_MIXED_STEP(3)
+      SF_MIXED_STEP(4) SF_MIXED_STEP(5) SF_MIXED_STEP(6) SF_MIXED_STEP(7)
+      SF_MIXED_STEP(8) SF_MIXED_STEP(9) SF_MIXED_STEP(10) SF_MIXED_STEP(11)
+      SF_MIXED_STEP(12) SF_MIXED_STEP(13) SF_MIXED_STEP(14) SF_MIXED_STEP(15)
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) acc[q] += double(c[q]);
+    __builtin_amdgcn_wave_barrier();
+  };
+  int head = 0, tail = 0;   // ring positions in records (head multiple of 64)
+  const int q0 = A.roff[t], qend = A.roff[t + 1];
+  for (int qb = q0; qb < qend; qb += kWave) {
+    int ms0 = 0, ms1 = 0, msi = 0;
+    if (qb + lane < qend) {
+      ms0 = start[A.rfirst[qb + lane]]; ms1 = start[A.rend[qb + lane]]; msi = A.rshift[qb + lane];
+    }
+    const int jn = min(kWave, qend - qb);
+    for (int j = 0; j < jn; ++j) {
+      const int s0 = __builtin_amdgcn_readlane(ms0, j), s1 = __builtin_amdgcn_readlane(ms1, j);
+      const int si = __builtin_amdgcn_readlane(msi, j);
+      const double ox = A.sh[si][0] - bx, oy = A.sh[si][1] - by, oz = A.sh[si][2] - bz;
+      const char *rr = reinterpret_cast<const char *>(A.rec + 2 * (size_t)s0);
+      const int last = s1 - 1 - s0;
+      for (int base = 0; base <= last; base += kWave) {
+        const int k = base + lane;
+        const bool valid = k <= last;
+        const unsigned o = unsigned(valid ? k : last) * 64u;
+        const double4 ra = *reinterpret_cast<const double4 *>(rr + o);
+        const double4 rb = *reinterpret_cast<const double4 *>(rr + o + 32u);
+        const double pu = fma(ra.x + ox, Ri, -uc), pv = fma(ra.y + oy, Ri, -vc), pw = fma(ra.z + oz, Ri, -wc);
+        const double gx = fmax(fabs(pu) - uh, 0.0);
+        const double gy = fmax(fabs(pv) - vh, 0.0);
+        const double gz = fmax(fabs(pw) - wh, 0.0);
+        const bool pass = valid && (rb.w != 0.0) && (gx * gx + gy * gy + gz * gz <= box2);
+        const unsigned long long m = __ballot(pass);
+        if (m == 0) continue;
+        if (pass) {
+          const int slot = (tail + __popcll(m & ((1ull << lane) - 1ull))) & (kRing - 1);
+          RP[slot][0] = pu; RP[slot][1] = pv; RP[slot][2] = pw;
+          RF[slot][0] = float(rb.x); RF[slot][1] = float(rb.y); RF[slot][2] = float(rb.z); RF[slot][3] = float(rb.w);
+        }
+        tail += __popcll(m);
+        if (tail - head < kWave) continue;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        batch(head, kWave);
+        head += kWave;
+      }
+    }
+  }
+  if (tail > head) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    batch(head, tail - head);
+  }
+  double *out = PS + ((size_t)t * nd + lane) * 4;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) out[q] = acc[q];
+}
+#undef SF_MIXED_STEP
 
 // ---- analysis integrals ------------------------------------------------------
 // Per element, in the reference's i-fastest DOF order:
@@ -1580,13 +1689,13 @@ int run_deposition(pic_handle *h, bool timed) {
     if (N == 3 && mixed) {
       const bool cart = h->sf.cartesian != 0;
       if (h->sf.alpha == 2 && cart)
-        hipLaunchKernelGGL((k_sf_gather_mixed<2, true>), dim3(nblk), dim3(kBlock), 0, h->stream, h->M, h->start, h->sf, h->PS);
+        hipLaunchKernelGGL((k_sf_gather_mixed_cart<2>), dim3(nblk), dim3(kBlock), 0, h->stream, h->M, h->start, h->sf, h->PS);
       else if (h->sf.alpha == 2)
-        hipLaunchKernelGGL((k_sf_gather_mixed<2, false>), dim3(nblk), dim3(kBlock), 0, h->stream, h->M, h->start, h->sf, h->PS);
+        hipLaunchKernelGGL((k_sf_gather_mixed<2>), dim3(nblk), dim3(kBlock), 0, h->stream, h->M, h->start, h->sf, h->PS);
       else if (cart)
-        hipLaunchKernelGGL((k_sf_gather_mixed<0, true>), dim3(nblk), dim3(kBlock), 0, h->stream, h->M, h->start, h->sf, h->PS);
+        hipLaunchKernelGGL((k_sf_gather_mixed_cart<0>), dim3(nblk), dim3(kBlock), 0, h->stream, h->M, h->start, h->sf, h->PS);
       else
-        hipLaunchKernelGGL((k_sf_gather_mixed<0, false>), dim3(nblk), dim3(kBlock), 0, h->stream, h->M, h->start, h->sf, h->PS);
+        hipLaunchKernelGGL((k_sf_gather_mixed<0>), dim3(nblk), dim3(kBlock), 0, h->stream, h->M, h->start, h->sf, h->PS);
     } else if (N == 3 && h->sfMatrix) {
       const bool lin = h->sf.linear != 0;
       if (h->sf.cartesian && h->sf.alpha == 2)

@@ -266,17 +266,23 @@ def test_refmapping_linear_box_matches_triatracking(oracle_mod):
     s.close()
 
 
-def test_mixed_precision_error_bounds(oracle_mod):
+@pytest.mark.parametrize("mesh_kind", ["cube", "deformed"])
+def test_mixed_precision_error_bounds(oracle_mod, mesh_kind):
     """Mixed precision (FP32 Lagrange basis, field interpolation and shape
     weights with FP32 partial sums, FP64 positions, Newton and face tests)
     against the FP64 oracle: deposited source within 1e-6 relative (L2 and
-    max), positions within 1e-9 of the domain after 10 steps."""
+    max), positions within 1e-9 of the domain after 10 steps.  The Cartesian
+    cube runs the tensor-grid gather, the deformed box the general one."""
     w = W.thermal_cube(nx=8, ppc=40, steps=10)
+    if mesh_kind == "deformed":
+        w = deformed_box(w)
+    print("case mixed precision on %s" % mesh_kind, flush=True)
     kw = W.stage_kwargs(w)
     with stage.PICStage(w.mesh, w.N, max_particles=len(w.PartState), precision=stage.PREC_MIXED, **kw) as s:
         s.set_field(w.E)
         s.set_particles(w.PartState, elem=w.elem, species=w.species)
         s.deposition()
+        assert s.stats()["deposit_kernel"] == stage.KERNEL_SF_MIXED
         PS0 = s.part_source()
         s.time_step(w.dt, 10)
         s.deposition()
@@ -286,6 +292,8 @@ def test_mixed_precision_error_bounds(oracle_mod):
                           external_field=kw["external_field"], charge=kw["charge"], mass=kw["mass"], mpf=kw["mpf"])
     o.set_field(w.E)
     o.set_particles(w.PartState, elem=w.elem, species=w.species)
+    if mesh_kind == "deformed":
+        o.locate()
     o.deposition()
     oPS0 = o.part_source()
     o.time_step(w.dt, 10)
@@ -303,7 +311,9 @@ def test_mixed_precision_error_bounds(oracle_mod):
     assert dx <= 1e-9
     # owners agree except for particles within 1e-9 of a face
     diff = el != oel
-    if diff.any():
+    if mesh_kind == "deformed":
+        assert diff.sum() <= 2
+    elif diff.any():
         h = 1.0 / 8
         r = np.mod(ops[diff, :3], h) / h
         assert (np.minimum(r, 1 - r).min(1) <= 1e-9).all()
