@@ -784,7 +784,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
       blo[a] = fmin(blo[a], __shfl_xor(blo[a], off, kWave));
       bhi[a] = fmax(bhi[a], __shfl_xor(bhi[a], off, kWave));
     }
-  const double thr = A.r_sf + M.radius[t];
+  // the reference's element cull |x - bary| > r_sf + ElemRadius, compared squared
+  const double thr = A.r_sf + M.radius[t], thr2 = thr * thr;
   const double R2 = A.r2_sf;
   const double R2box = R2 * (1.0 + 1e-12);
   double (*L)[8] = lds[wib];
@@ -810,7 +811,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
         // On linear elements the cull is implied by the box test (every DOF
         // lies within ElemRadius of the barycentre), so only the box test runs.
         pass = (fabs(q0) > 0.0) && (gx * gx + gy * gy + gz * gz <= R2box) &&
-               (A.linear || !(sqrt(px * px + py * py + pz * pz) > thr));
+               (A.linear || !(px * px + py * py + pz * pz > thr2));
         f = q0 * M.mpf[sp] * A.w_sf;
         vx = P.vx[i]; vy = P.vy[i]; vz = P.vz[i];
       }
@@ -925,7 +926,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE_MX))
       blo[a] = fmin(blo[a], __shfl_xor(blo[a], off, kWave));
       bhi[a] = fmax(bhi[a], __shfl_xor(bhi[a], off, kWave));
     }
-  const double thr = A.r_sf + M.radius[t];
+  // the reference's element cull |x - bary| > r_sf + ElemRadius, compared squared
+  const double thr = A.r_sf + M.radius[t], thr2 = thr * thr;
   const double R2box = A.r2_sf * (1.0 + 1e-12);
   double (*L)[8] = lds[wib];
   const int kp = lane >> 4, jc = lane & 3;
@@ -963,11 +965,15 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE_MX))
       const int si = __builtin_amdgcn_readlane(msi, j);
       // shifted position (GetPartPosShifted) relative to the target barycentre
       const double ox = A.sh[si][0] - bx, oy = A.sh[si][1] - by, oz = A.sh[si][2] - bz;
-      for (int base = s0; base < s1; base += kWave) {
-        const int i = base + lane;
-        const bool valid = i < s1;
-        const size_t ic = size_t(valid ? i : s1 - 1);
-        const double4 ra = A.rec[2 * ic], rb = A.rec[2 * ic + 1];
+      // the run's records from a wave-uniform base, addressed by 32-bit offsets
+      const char *rr = reinterpret_cast<const char *>(A.rec + 2 * (size_t)s0);
+      const int last = s1 - 1 - s0;
+      for (int base = 0; base <= last; base += kWave) {
+        const int k = base + lane;
+        const bool valid = k <= last;
+        const unsigned o = unsigned(valid ? k : last) * 64u;
+        const double4 ra = *reinterpret_cast<const double4 *>(rr + o);
+        const double4 rb = *reinterpret_cast<const double4 *>(rr + o + 32u);
         const double px = ra.x + ox, py = ra.y + oy, pz = ra.z + oz;
         // the exact test that some DOF of the target lies within r_sf, and on
         // curved elements the reference's element cull |x - bary| > r_sf + ElemRadius
@@ -977,7 +983,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE_MX))
         bool pass = valid && (rb.w != 0.0) && (gx * gx + gy * gy + gz * gz <= R2box);
         // (on linear elements every DOF lies within ElemRadius of the
         // barycentre, so the box test implies the cull)
-        if (!LINEAR) pass = pass && !(sqrt(px * px + py * py + pz * pz) > thr);
+        if (!LINEAR) pass = pass && !(px * px + py * py + pz * pz > thr2);
         const unsigned long long m = __ballot(pass);
         if (m == 0) continue;
         if (pass) {
@@ -1187,7 +1193,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE_MX))
       blo[c] = fmin(blo[c], __shfl_xor(blo[c], off, kWave));
       bhi[c] = fmax(bhi[c], __shfl_xor(bhi[c], off, kWave));
     }
-  const double thr = A.r_sf + M.radius[t];
+  // the reference's element cull |x - bary| > r_sf + ElemRadius, compared squared
+  const double thr = A.r_sf + M.radius[t], thr2 = thr * thr;
   const double box2 = A.r2_sf * (1.0 + 1e-12);
   float (*L)[kRec] = lds[wib];
   const int jc = lane & 3;
@@ -1223,17 +1230,21 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE_MX))
       const int s0 = __builtin_amdgcn_readlane(ms0, j), s1 = __builtin_amdgcn_readlane(ms1, j);
       const int si = __builtin_amdgcn_readlane(msi, j);
       const double ox = A.sh[si][0] - bx, oy = A.sh[si][1] - by, oz = A.sh[si][2] - bz;
-      for (int base = s0; base < s1; base += kWave) {
-        const int i = base + lane;
-        const bool valid = i < s1;
-        const size_t ic = size_t(valid ? i : s1 - 1);
-        const double4 ra = A.rec[2 * ic], rb = A.rec[2 * ic + 1];
+      // the run's records from a wave-uniform base, addressed by 32-bit offsets
+      const char *rr = reinterpret_cast<const char *>(A.rec + 2 * (size_t)s0);
+      const int last = s1 - 1 - s0;
+      for (int base = 0; base <= last; base += kWave) {
+        const int k = base + lane;
+        const bool valid = k <= last;
+        const unsigned o = unsigned(valid ? k : last) * 64u;
+        const double4 ra = *reinterpret_cast<const double4 *>(rr + o);
+        const double4 rb = *reinterpret_cast<const double4 *>(rr + o + 32u);
         const double px = ra.x + ox, py = ra.y + oy, pz = ra.z + oz;
         const double gx = fmax(0.0, fmax(blo[0] - px, px - bhi[0]));
         const double gy = fmax(0.0, fmax(blo[1] - py, py - bhi[1]));
         const double gz = fmax(0.0, fmax(blo[2] - pz, pz - bhi[2]));
         bool pass = valid && (rb.w != 0.0) && (gx * gx + gy * gy + gz * gz <= box2);
-        if (!A.linear) pass = pass && !(sqrt(px * px + py * py + pz * pz) > thr);
+        if (!A.linear) pass = pass && !(px * px + py * py + pz * pz > thr2);
         const unsigned long long m = __ballot(pass);
         if (m == 0) continue;
         if (pass) {
