@@ -1159,14 +1159,28 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE_MX))
 }
 
 // Mixed-precision shape-function gather for N = 3 on general elements: S and
-// the products in FP32, summed in FP32 on the matrix cores
-// (v_mfma_f32_4x4x1, 16 blocks of 4x4: lane 4b+i gives A_b[i] = S^alpha of
-// DOF lane, lane 4b+j gives B_b[j] = f (v,1)_j of the particle, lane 4b+j
-// holds D_b[i][j] in register i), then added into FP64 after every batch of
-// 64 staged particles.  Positions relative to the target barycentre and the
-// cull are formed in FP64 from the same records and runs as the FP64
-// gathers.  Records in LDS (FP32): px, py, pz, |p|^2/R^2, f (v,1).
+// the products in FP32, summed in FP32 on the matrix cores, then added into
+// FP64 after every batch of 64 staged particles.  The products run 16
+// particles per sequence of v_mfma_f32_4x4x1 (16 blocks of 4x4): operand A
+// is f (v,1) of particle p, taken for every block from the lanes 4p..4p+3
+// (broadcast by CBSZ = 4, ABID = p), operand B is S^alpha of DOF lane, and
+// lane l gathers the four source components of DOF l.  Positions relative to
+// the target barycentre and the cull are formed in FP64 from the same
+// records and runs as the FP64 gathers.  Records in LDS (FP32): px, py, pz,
+// |p|^2/R^2, f (v,1).
 typedef float v4f __attribute__((ext_vector_type(4)));
+#define SF_MIXED_GSTEP(P)                                                        \
+  {                                                                              \
+    const float *r_ = Lh[g + (P)];                                               \
+    const float S_ = clamp01f(mx * r_[0] + (my * r_[1] + (mz * r_[2] + (cd - r_[3])))); \
+    float S1_ = S_ * S_;                                                         \
+    if (ALPHA == 0) {                                                            \
+      for (int ex = 3; ex <= A.alpha; ++ex) S1_ = S_ * S1_;                      \
+    } else {                                                                     \
+      _Pragma("unroll") for (int ex = 3; ex <= ALPHA; ++ex) S1_ = S_ * S1_;      \
+    }                                                                            \
+    c = __builtin_amdgcn_mfma_f32_4x4x1f32(fa, S1_, c, 4, (P), 0);               \
+  }
 template <int ALPHA>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE_MX))) void k_sf_gather_mixed(DevMesh M, const int32_t *start, SFArgs A, double *PS) {
 #pragma clang fp contract(fast)
@@ -1197,23 +1211,21 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE_MX))
   const double thr = A.r_sf + M.radius[t], thr2 = thr * thr;
   const double box2 = A.r2_sf * (1.0 + 1e-12);
   float (*L)[kRec] = lds[wib];
-  const int jc = lane & 3;
+  // ring rows past the last record of a batch are read with zero weight: keep them finite
+#pragma unroll
+  for (int q = 0; q < 2 * kRec; ++q) L[2 * lane + q / kRec][q % kRec] = 0.f;
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
   // the batch of n staged records from ring position h0 (no wrap)
   auto batch = [&](int h0, int n) {
     const float (*Lh)[kRec] = L + (h0 & (kRing - 1));
     v4f c = {0.f, 0.f, 0.f, 0.f};
-    for (int p = 0; p < n; ++p) {
-      const float *r = Lh[p];
-      const float S = clamp01f(mx * r[0] + (my * r[1] + (mz * r[2] + (cd - r[3]))));
-      float S1 = S * S;
-      if (ALPHA == 0) {
-        for (int ex = 3; ex <= A.alpha; ++ex) S1 = S * S1;
-      } else {
-#pragma unroll
-        for (int ex = 3; ex <= ALPHA; ++ex) S1 = S * S1;
-      }
-      c = __builtin_amdgcn_mfma_f32_4x4x1f32(S1, r[4 + jc], c, 0, 0, 0);
+    for (int g = 0; g < n; g += 16) {
+      const int pf = g + (lane >> 2);
+      const float fa = pf < n ? Lh[pf][4 + (lane & 3)] : 0.f;
+      SF_MIXED_GSTEP(0) SF_MIXED_GSTEP(1) SF_MIXED_GSTEP(2) SF_MIXED_GSTEP(3)
+      SF_MIXED_GSTEP(4) SF_MIXED_GSTEP(5) SF_MIXED_GSTEP(6) SF_MIXED_GSTEP(7)
+      SF_MIXED_GSTEP(8) SF_MIXED_GSTEP(9) SF_MIXED_GSTEP(10) SF_MIXED_GSTEP(11)
+      SF_MIXED_GSTEP(12) SF_MIXED_GSTEP(13) SF_MIXED_GSTEP(14) SF_MIXED_GSTEP(15)
     }
 #pragma unroll
     for (int q = 0; q < 4; ++q) acc[q] += double(c[q]);
@@ -1270,9 +1282,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE_MX))
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     batch(head, tail - head);
   }
+  double *out = PS + ((size_t)t * nd + lane) * 4;
 #pragma unroll
-  for (int q = 0; q < 4; ++q) PS[((size_t)t * nd + 4 * (lane >> 2) + q) * 4 + jc] = acc[q];
+  for (int q = 0; q < 4; ++q) out[q] = acc[q];
 }
+#undef SF_MIXED_GSTEP
 
 // Mixed-precision gather on axis-aligned tensor grids (N = 3).  The cull and
 // the ring follow k_sf_gather_cart; the ring keeps each survivor's position
