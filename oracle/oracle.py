@@ -1,7 +1,7 @@
 """ctypes binding of the CPU oracle (libpic_oracle.so).
 
-TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() as
-the checker, and by bench.py's cpu_baseline leg.  The product never imports
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() to
+check its result, and by bench.py's cpu_baseline leg.  The product never imports
 this module.
 """
 import ctypes

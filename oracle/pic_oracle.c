@@ -4,8 +4,9 @@
  * Plain C99, FP64, compiled with -ffp-contract=off so that every product and
  * sum is rounded exactly as written.  Each routine restates one routine of
  * the reference (Fortran, PICLas v4.1.0) and cites it as file:line, paths
- * relative to the reference's src/ directory.  It is the checker for the HIP
- * path and the CPU baseline timed beside it; the product never calls it.
+ * relative to the reference's src/ directory.  The tests compare the HIP path
+ * against it, and it is the CPU baseline timed beside it; the product never
+ * calls it.
  *
  * Scope: 3D hexahedral meshes without mortars; TriaTracking (periodic and
  * open particle boundaries); cell_volweight_mean and 3D shape_function

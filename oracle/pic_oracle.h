@@ -5,9 +5,9 @@
  * restatement of the reference's particle stage (field interpolation, the
  * Boris-Leapfrog push, TriaTracking and RefMapping tracking on linear and
  * curved hexahedra, cell_volweight_mean and shape_function
- * deposition, and the mesh tables those routines read).  It is the checker
- * for the HIP implementation in piclas_amd/ and the CPU baseline timed
- * beside it.  Only tests/, __graft_entry__.smoke() and bench.py's
+ * deposition, and the mesh tables those routines read).  The tests compare
+ * the HIP implementation in piclas_amd/ against it, and it is the CPU
+ * baseline timed beside it.  Only tests/, __graft_entry__.smoke() and bench.py's
  * cpu_baseline leg may load it; the product never links or calls it.
  *
  * Conventions follow the reference's Fortran arrays:

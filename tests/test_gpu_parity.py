@@ -1,7 +1,7 @@
 """Parity of the HIP particle stage with the CPU oracle (and with the
 reference's own golden deposition), through the C ABI.
 
-Bars (from the north-star parity definition):
+Bars:
   element ownership   bit-exact, except particles within 1e-12 of a face
   positions           |dx| <= 1e-12 * domain size          (FP64)
   velocities          |dv| <= 1e-12 * max|v|               (FP64)
