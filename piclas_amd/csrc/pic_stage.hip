@@ -1035,6 +1035,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE_MX))
 #pragma clang fp contract(fast)
   constexpr int nd = 64;
   // ring of 128 staged records: pu, pv, C_k for k = 0..3, f vx, f vy, f vz, f
+  // (staging writes pw in the place of C_0; a pass with one record per lane
+  // forms the four C_k before each batch of 64 is consumed)
   constexpr int kRing = 128, kRec = 10;
   __shared__ double lds[kBlock / kWave][kRing][kRec];
   const int wib = threadIdx.x / kWave;
@@ -1087,6 +1089,15 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE_MX))
       acc[Q] = __builtin_amdgcn_mfma_f64_4x4x4f64(S1, bval, acc[Q], 0, 0, 0);
     }
   };
+  // C_k = (w_k - pw)^2 + pu^2 + pv^2 of the 64 records from Lh, one per lane
+  // (rows past the staged ones are overwritten harmlessly and never read)
+  auto form_ck = [&](double (*Lh)[kRec]) {
+    double *r = Lh[lane];
+    const double pu = r[0], pv = r[1], pw = r[2];
+    const double c0 = w[0] - pw, c1 = w[1] - pw, c2 = w[2] - pw, c3 = w[3] - pw;
+    const double P = fma(pv, pv, pu * pu);
+    r[2] = fma(c0, c0, P); r[3] = fma(c1, c1, P); r[4] = fma(c2, c2, P); r[5] = fma(c3, c3, P);
+  };
   int head = 0, tail = 0;   // ring positions in records (head multiple of 64)
   const int q0 = A.roff[t], qend = A.roff[t + 1];
   for (int qb = q0; qb < qend; qb += kWave) {
@@ -1118,14 +1129,15 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE_MX))
         if (m == 0) continue;
         if (pass) {
           double *r = L[(tail + __popcll(m & ((1ull << lane) - 1ull))) & (kRing - 1)];
-          const double c0 = w[0] - pw, c1 = w[1] - pw, c2 = w[2] - pw, c3 = w[3] - pw;
-          const double P = fma(pv, pv, pu * pu);
-          r[0] = pu; r[1] = pv;
-          r[2] = fma(c0, c0, P); r[3] = fma(c1, c1, P); r[4] = fma(c2, c2, P); r[5] = fma(c3, c3, P);
+          r[0] = pu; r[1] = pv; r[2] = pw;
           r[6] = rb.x; r[7] = rb.y; r[8] = rb.z; r[9] = rb.w;
         }
         tail += __popcll(m);
         if (tail - head < kWave) continue;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        form_ck(L + (head & (kRing - 1)));
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -1142,12 +1154,14 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE_MX))
     const int ng = (rest + 3) >> 2;
     if (lane >= rest && lane < 4 * ng) {
       double *r = L[(head + lane) & (kRing - 1)];
-      r[0] = 0.0; r[1] = 0.0;
-#pragma unroll
-      for (int c = 2; c < 6; ++c) r[c] = 4.0;
+      r[0] = 0.0; r[1] = 0.0; r[2] = 1e3;
 #pragma unroll
       for (int c = 6; c < kRec; ++c) r[c] = 0.0;
     }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    form_ck(L + (head & (kRing - 1)));
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
