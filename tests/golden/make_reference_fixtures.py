@@ -9,7 +9,8 @@ fixtures it writes are committed, so tests never read the reference itself.
 Fixtures written:
   2Dplasma_test_State_000.00000005000000000.h5
       regressioncheck/NIG_PIC_poisson_Leapfrog/2D_innerBC_dielectric_surface_charge/
-      copied as is: a state file the reference wrote (File_Version 3.21,
+      a state file the reference wrote, its userblock (parameter file and
+      build notes, not data) zeroed: File_Version 3.21,
       PartInt (nElems, 2), PartData, VarNamesParticles, 5-D DG_Source), the
       layout check of piclas_amd/state_file.py.
   cvwm_plasma_ball.npz
@@ -117,11 +118,33 @@ def cvwm_plasma_ball(lib):
     return out
 
 
+HDF5_SIGNATURE = b"\x89HDF\r\n\x1a\n"
+
+
+def blank_userblock(data):
+    """Zero the HDF5 userblock (everything before the superblock).
+
+    The reference writes its parameter file, build configuration and a source
+    diff into the userblock of every state file.  That is not data: only the
+    datasets and attributes after the superblock are kept.  The HDF5 library
+    ignores the userblock's content, so the file still opens unchanged.  The
+    superblock sits at offset 0, 512, 1024, 2048, ... (HDF5 file format)."""
+    data = bytearray(data)
+    off = 0
+    while data[off:off + 8] != HDF5_SIGNATURE:
+        off = 512 if off == 0 else 2 * off
+        if off >= len(data):
+            raise ValueError("no HDF5 superblock found")
+    data[:off] = bytes(off)
+    return bytes(data)
+
+
 def copy_state_file():
+    """The state file's datasets and attributes, with its userblock zeroed."""
     src = os.path.join(REF, "regressioncheck/NIG_PIC_poisson_Leapfrog/2D_innerBC_dielectric_surface_charge",
                        "2Dplasma_test_State_000.00000005000000000.h5")
     with open(src, "rb") as f, open(os.path.join(HERE, os.path.basename(src)), "wb") as g:
-        g.write(f.read())
+        g.write(blank_userblock(f.read()))
 
 
 if __name__ == "__main__":
