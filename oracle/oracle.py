@@ -77,12 +77,17 @@ def lib():
         L.om_get_refmapping_tables.argtypes = [ctypes.c_void_p, _u8p, _f64p]
         L.om_position_in_ref_elem.argtypes = [ctypes.c_void_p, ctypes.c_int32, _f64p, _i32p, _f64p, _u8p]
         L.om_eval_field_at_ref_pos.argtypes = [ctypes.c_void_p, ctypes.c_int32, _f64p, _i32p, _f64p]
+        L.om_em_field_dw.argtypes = [ctypes.c_void_p, ctypes.c_int32, _f64p, _i32p, _f64p]
+        L.om_set_part_source.argtypes = [ctypes.c_void_p, _f64p]
+        L.om_get_field.argtypes = [ctypes.c_void_p, _f64p]
         L.om_legendre_gauss_nodes.argtypes = [ctypes.c_int32, _f64p, _f64p]
         L.om_cheby_gauss_lobatto_nodes.argtypes = [ctypes.c_int32, _f64p]
         L.om_barycentric_weights.argtypes = [ctypes.c_int32, _f64p, _f64p]
         L.om_lagrange_polys.argtypes = [ctypes.c_double, ctypes.c_int32, _f64p, _f64p, _f64p]
         L.om_get_nparticles.restype = ctypes.c_int32
         L.om_get_nparticles.argtypes = [ctypes.c_void_p]
+        L.om_get_private_reduce_seconds.restype = ctypes.c_double
+        L.om_get_private_reduce_seconds.argtypes = [ctypes.c_void_p]
         L.om_get_lost_count.restype = ctypes.c_int64
         L.om_get_lost_count.argtypes = [ctypes.c_void_p]
         L.om_get_multi_crossings.restype = ctypes.c_int64
@@ -206,6 +211,11 @@ class Oracle:
         lib().om_get_part_source(self.h, _p(a, _f64p))
         return a
 
+    def private_reduce_seconds(self):
+        """Threaded shape-function deposit: seconds spent zeroing and summing
+        the per-thread source copies (summed over calls)."""
+        return lib().om_get_private_reduce_seconds(self.h)
+
     def lost(self):
         return lib().om_get_lost_count(self.h)
 
@@ -268,6 +278,18 @@ class Oracle:
         self._G = np.ascontiguousarray(G, np.float64)
         self._check(lib().om_sc_set_operator(self.h, _p(self._G, _f64p)), "sc_set_operator")
 
+    def set_part_source(self, PS):
+        """Test only: overwrite PartSource [nElems][(N+1)^3][4]."""
+        PS = np.ascontiguousarray(PS, np.float64)
+        assert PS.size == self.nElems * self.nDof * 4
+        self._check(lib().om_set_part_source(self.h, _p(PS, _f64p)), "set_part_source")
+
+    def field(self):
+        """E at the Gauss points [nElems][(N+1)^3][3] as the oracle holds it."""
+        E = np.zeros((self.nElems, self.nDof, 3))
+        lib().om_get_field(self.h, _p(E, _f64p))
+        return E
+
     def sc_field_from_source(self):
         self._check(lib().om_sc_field_from_source(self.h), "sc_field_from_source")
 
@@ -305,6 +327,15 @@ class Oracle:
         n = xi.shape[0]
         U = np.zeros((n, 3))
         lib().om_eval_field_at_ref_pos(self.h, n, _p(xi, _f64p), _p(elem, _i32p), _p(U, _f64p))
+        return U
+
+    def em_field_dw(self, x, elem):
+        """GetEMFieldDW (E part) at physical positions x in elements elem (1-based)."""
+        x = np.ascontiguousarray(x, np.float64)
+        elem = np.ascontiguousarray(elem, np.int32)
+        n = x.shape[0]
+        U = np.zeros((n, 3))
+        lib().om_em_field_dw(self.h, n, _p(x, _f64p), _p(elem, _i32p), _p(U, _f64p))
         return U
 
     def deposited_charge(self):

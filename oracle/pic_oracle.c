@@ -206,6 +206,11 @@ struct om_state {
   int64_t tbCap, tbN;
   int plant;          /* planted mistakes (om_plant_mistake), 0 normally */
   int nthreads;
+  /* threaded shape-function deposit: per-thread PartSource copies, kept
+   * between calls, and the wall time spent zeroing and summing them */
+  double **priv;
+  int nPriv;
+  double tPrivReduce;
 };
 
 #define ELEMINFO(e, c) (s->ElemInfo[(size_t)(e) * 6 + (c)])
@@ -503,7 +508,9 @@ static int build_mesh(om_state *s, const om_mesh_desc *m) {
     int bc = SIDEINFO(sd, SIDE_BCID);
     if (bc > 0 && s->partBC[bc - 1] == 1) s->openElem[s->sideElem[sd]] = 1;
   }
-  /* GEO%x/y/zminglob and maxglob over the mesh nodes; DirPeriodicVectors */
+  /* GEO%x/y/zminglob and maxglob over the mesh nodes (TriaTracking,
+   * particle_mesh_tools.f90:775-783; RefMapping replaces them below);
+   * DirPeriodicVectors */
   for (int c = 0; c < 3; c++) { s->gmin[c] = 1e300; s->gmax[c] = -1e300; }
   for (int i = 0; i < s->nNodes; i++)
     for (int c = 0; c < 3; c++) {
@@ -560,6 +567,7 @@ static int build_mesh(om_state *s, const om_mesh_desc *m) {
       }
       for (int r = 0; r < g1; r++) for (int c = 0; c < g1; c++) sVb[r * g1 + c] = A[r][g1 + c];
     }
+    double bzmin[3] = {1e300, 1e300, 1e300}, bzmax[3] = {-1e300, -1e300, -1e300};
     for (int e = 0; e < ne; e++) {
       const double *X = &s->XCL[(size_t)e * nGP * 3];
       for (int d = 0; d < 6; d++) {
@@ -599,10 +607,17 @@ static int build_mesh(om_state *s, const om_mesh_desc *m) {
           double d0 = Bz[a][bq][0] - s->bary[e * 3], d1 = Bz[a][bq][1] - s->bary[e * 3 + 1], d2 = Bz[a][bq][2] - s->bary[e * 3 + 2];
           double v = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
           if (v > r) r = v;
+          for (int c = 0; c < 3; c++) {
+            if (Bz[a][bq][c] < bzmin[c]) bzmin[c] = Bz[a][bq][c];
+            if (Bz[a][bq][c] > bzmax[c]) bzmax[c] = Bz[a][bq][c];
+          }
         }
       }
       s->radius[e] = r;
     }
+    /* with RefMapping GEO%x/y/zminglob, maxglob bound the sides' Bezier
+     * control points (GetMeshMinMax, particle_mesh_tools.f90:735-747) */
+    for (int c = 0; c < 3; c++) { s->gmin[c] = bzmin[c]; s->gmax[c] = bzmax[c]; }
     /* ElemEpsOneCell = 1 + sqrt(3 scaleJ RefMappingEps), scaleJ = max sJ / min sJ
      * (BuildEpsOneCell, particle_mesh_build.f90:447-642) */
     s->epsOneCell = xcalloc(ne, sizeof(double));
@@ -1079,6 +1094,8 @@ void om_destroy(om_state *s) {
                   s->LastPartPos, s->Field, s->elem, s->lastElem, s->species, s->inside, s->isNew,
                   s->curved, s->dXCL1, s->epsOneCell, s->openElem, s->PartPosRef, s->Gop, s->rhoBar, s->Enode};
   for (size_t i = 0; i < sizeof ptrs / sizeof ptrs[0]; i++) free(ptrs[i]);
+  for (int t = 0; t < s->nPriv; t++) free(s->priv[t]);
+  free(s->priv);
   free(s->tbLog);
   free(s);
 }
@@ -1123,6 +1140,7 @@ int om_set_particles(om_state *s, int32_t n, const double *PartState, const int3
 }
 
 int32_t om_get_nparticles(const om_state *s) { return s->n; }
+double om_get_private_reduce_seconds(const om_state *s) { return s->tPrivReduce; }
 int64_t om_get_lost_count(const om_state *s) { return s->lost; }
 int64_t om_get_multi_crossings(const om_state *s) { return s->multiCross; }
 int om_set_tiebreak_log(om_state *s, int64_t cap) {
@@ -1151,6 +1169,8 @@ int om_get_particles(const om_state *s, double *PartState, int32_t *elem, uint8_
 int om_get_last_part_pos(const om_state *s, double *L) { memcpy(L, s->LastPartPos, (size_t)s->n * 3 * sizeof(double)); return 0; }
 int om_get_field_at_particle(const om_state *s, double *F) { memcpy(F, s->Field, (size_t)s->n * 6 * sizeof(double)); return 0; }
 int om_get_part_source(const om_state *s, double *PS) { memcpy(PS, s->PS, (size_t)s->nElems * s->nDof * 4 * sizeof(double)); return 0; }
+int om_set_part_source(om_state *s, const double *PS) { memcpy(s->PS, PS, (size_t)s->nElems * s->nDof * 4 * sizeof(double)); return 0; }
+int om_get_field(const om_state *s, double *E) { memcpy(E, s->E, (size_t)s->nElems * s->nDof * 3 * sizeof(double)); return 0; }
 int om_get_elem_xgp(const om_state *s, double *x) { memcpy(x, s->xGPe, (size_t)s->nElems * s->nDof * 3 * sizeof(double)); return 0; }
 int om_get_sj(const om_state *s, double *sJ) { memcpy(sJ, s->sJ, (size_t)s->nElems * s->nDof * sizeof(double)); return 0; }
 int om_get_elem_side_node_id(const om_state *s, int32_t *id) { memcpy(id, s->ElemSideNodeID, (size_t)s->nElems * 24 * sizeof(int32_t)); return 0; }
@@ -1186,6 +1206,10 @@ int om_position_in_ref_elem(om_state *s, int32_t n, const double *x, const int32
 }
 int om_eval_field_at_ref_pos(om_state *s, int32_t n, const double *xi, const int32_t *elem, double *U) {
   for (int i = 0; i < n; i++) eval_field_at_ref_pos(s, &xi[i * 3], elem[i] - 1, &U[i * 3]);
+  return 0;
+}
+int om_em_field_dw(om_state *s, int32_t n, const double *x, const int32_t *elem, double *U) {
+  for (int i = 0; i < n; i++) em_field_dw(s, elem[i] - 1, &x[i * 3], &U[i * 3]);
   return 0;
 }
 
@@ -1893,13 +1917,26 @@ static int deposit_sf(om_state *s) {
     return 0;
   }
 #ifdef _OPENMP
-  /* timing build: private PS per thread, summed afterwards */
+  /* timing build: a private PS per thread (kept between calls), summed
+   * afterwards; zeroing and summing the copies is timed on its own
+   * (tPrivReduce), since a domain-decomposed code has no such copies */
   int nt = s->nthreads;
-  double **priv = xcalloc(nt, sizeof(double *));
+  if (s->nPriv != nt) {
+    for (int t = 0; t < s->nPriv; t++) free(s->priv[t]);
+    free(s->priv);
+    s->priv = xcalloc(nt, sizeof(double *));
+    s->nPriv = nt;
+  }
+  double **priv = s->priv;
+  double tz = 0.0, tr = 0.0;
 #pragma omp parallel num_threads(nt)
   {
     int t = omp_get_thread_num();
-    priv[t] = xcalloc(nps, sizeof(double));
+    double t0 = omp_get_wtime();
+    if (!priv[t]) priv[t] = xcalloc(nps, sizeof(double));
+    else memset(priv[t], 0, nps * sizeof(double));
+#pragma omp barrier
+    if (t == 0) tz = omp_get_wtime() - t0;
     uint8_t *done = xcalloc(s->nElems, 1);
 #pragma omp for schedule(static)
     for (int i = 0; i < s->n; i++) {
@@ -1907,15 +1944,16 @@ static int deposit_sf(om_state *s) {
       sf_source_one(s, i, priv[t], done);
     }
     free(done);
+    double t1 = omp_get_wtime();
 #pragma omp for schedule(static)
     for (size_t q = 0; q < nps; q++) {
       double acc = 0.0;
       for (int u = 0; u < nt; u++) acc += priv[u][q];
       s->PS[q] = acc;
     }
+    if (t == 0) tr = omp_get_wtime() - t1;
   }
-  for (int t = 0; t < nt; t++) free(priv[t]);
-  free(priv);
+  s->tPrivReduce += tz + tr;
 #endif
   return 0;
 }

@@ -87,6 +87,9 @@ int  om_get_last_part_pos(const om_state *s, double *LastPartPos);
 int  om_get_field_at_particle(const om_state *s, double *FieldAtParticle);
 int  om_get_part_source(const om_state *s, double *PS);
 int64_t om_get_lost_count(const om_state *s);
+/* Threaded shape-function deposit only: wall seconds spent zeroing and
+ * summing the per-thread PartSource copies, over all calls. */
+double om_get_private_reduce_seconds(const om_state *s);
 /* How often the TriaTracking walk took the multi-crossing branch
  * (particle_triatracking.f90:309-405), summed over all calls. */
 int64_t om_get_multi_crossings(const om_state *s);
@@ -123,6 +126,9 @@ int  om_position_in_ref_elem(om_state *s, int32_t n, const double *x, const int3
 /* Field evaluation at reference positions (EvaluateFieldAtRefPos) */
 int  om_eval_field_at_ref_pos(om_state *s, int32_t n, const double *xi, const int32_t *elem,
                               double *U);
+/* GetEMFieldDW (inverse-distance weights to the Gauss points) at physical
+ * positions, whatever RefElemNewton would say: for pinning the loop exit. */
+int  om_em_field_dw(om_state *s, int32_t n, const double *x, const int32_t *elem, double *U);
 
 /* Analysis: integral of rho (CalcDepositedCharge) and electric field energy (CalcPotentialEnergy) */
 double om_deposited_charge(const om_state *s);
@@ -134,6 +140,10 @@ double om_field_energy(const om_state *s);
  * nodes; E at the Gauss points is their trilinear interpolation. */
 int  om_sc_set_operator(om_state *s, const double *G);
 int  om_sc_field_from_source(om_state *s);
+/* Test only: overwrite PartSource (as a deposition would leave it) and read
+ * back the field at the Gauss points (as om_sc_field_from_source leaves it). */
+int  om_set_part_source(om_state *s, const double *PS);
+int  om_get_field(const om_state *s, double *E);
 int  om_sc_forward(om_state *s, double dt, int32_t K, double *loss);
 
 /* 1D node / basis helpers (basis.f90), exposed for closed-form checks */

@@ -584,6 +584,10 @@ __device__ inline void field_at_particle(const DevMesh &M, int e, const double x
   bool ok = true;
   if (AFFINE) {
     affine_xi<UNI>(M, e, x, xi);
+    // RefElemNewton's first step lands on this xi and gives up when a
+    // component exceeds 1.5: the particle is then far
+    // outside its element and isSuccessful is .FALSE. (eval_xyz.f90:424-441)
+    ok = !(fabs(xi[0]) > 1.5 || fabs(xi[1]) > 1.5 || fabs(xi[2]) > 1.5);
   } else {
     ref_newton_start<N>(M, e, x, xi);
     ok = ref_elem_newton(M, e, x, xi, true);

@@ -137,13 +137,9 @@ int pic_emit_particles(pic_handle *h, int32_t n_inits, const pic_emission *inits
   EmitArgs A{};
   A.nInits = n_inits;
   A.seed = seed;
-  // GEO%x/y/z min/max glob: the bounding box of the element corner nodes
-  for (int c = 0; c < 3; ++c) { A.gmin[c] = 1e300; A.gmax[c] = -1e300; }
-  for (size_t t = 0; t < h->hm.corner.size() / 3; ++t)
-    for (int c = 0; c < 3; ++c) {
-      A.gmin[c] = std::min(A.gmin[c], h->hm.corner[t * 3 + c]);
-      A.gmax[c] = std::max(A.gmax[c], h->hm.corner[t * 3 + c]);
-    }
+  // GEO%x/y/z min/max glob (GetMeshMinMax): all mesh nodes with
+  // TriaTracking, the sides' Bezier control points with RefMapping
+  for (int c = 0; c < 3; ++c) { A.gmin[c] = h->hm.gmin[c]; A.gmax[c] = h->hm.gmax[c]; }
   int64_t n = 0;
   for (int a = 0; a < n_inits; ++a) {
     const pic_emission &E = inits[a];

@@ -356,6 +356,8 @@ std::string build_host_mesh(const pic_mesh_desc &m, const pic_params &par, HostM
   for (int e = 0; e < ne; ++e)
     for (int l = 0; l < 6; ++l)
       if (hm.sideBC[e * 6 + l] == PIC_BC_OPEN) hm.openElem[e] = 1;
+  // GEO%x/y/zminglob, maxglob over the mesh nodes (TriaTracking,
+  // particle_mesh_tools.f90:775-783; RefMapping replaces them below)
   for (int c = 0; c < 3; ++c) { hm.gmin[c] = 1e300; hm.gmax[c] = -1e300; }
   for (int i = 0; i < m.nNodes; ++i)
     for (int c = 0; c < 3; ++c) {
@@ -409,6 +411,7 @@ std::string build_host_mesh(const pic_mesh_desc &m, const pic_params &par, HostM
       }
       for (int r = 0; r < g1; ++r) for (int c = 0; c < g1; ++c) sVb[r * g1 + c] = A[r][g1 + c];
     }
+    double bzmin[3] = {1e300, 1e300, 1e300}, bzmax[3] = {-1e300, -1e300, -1e300};
     for (int e = 0; e < ne; ++e) {
       const double *X = &hm.XCL[size_t(e) * nGP * 3];
       double L[3][8];
@@ -444,10 +447,14 @@ std::string build_host_mesh(const pic_mesh_desc &m, const pic_params &par, HostM
           }
           const double d0 = B[0] - hm.bary[e * 3], d1 = B[1] - hm.bary[e * 3 + 1], d2 = B[2] - hm.bary[e * 3 + 2];
           r = std::max(r, std::sqrt(d0 * d0 + d1 * d1 + d2 * d2));
+          for (int c = 0; c < 3; ++c) { bzmin[c] = std::min(bzmin[c], B[c]); bzmax[c] = std::max(bzmax[c], B[c]); }
         }
       }
       hm.radius[e] = r;
     }
+    // with RefMapping GEO%x/y/zminglob, maxglob bound the sides' Bezier
+    // control points (GetMeshMinMax, particle_mesh_tools.f90:735-747)
+    for (int c = 0; c < 3; ++c) { hm.gmin[c] = bzmin[c]; hm.gmax[c] = bzmax[c]; }
     // ElemEpsOneCell = 1 + sqrt(3 scaleJ RefMappingEps), scaleJ = max sJ / min sJ (BuildEpsOneCell)
     hm.epsOneCell.assign(ne, 0.0);
     for (int e = 0; e < ne; ++e) {

@@ -22,6 +22,9 @@
 #include "pic_device.h"
 #include "pic_refmap.h"
 #include "stage_internal.h"
+#ifndef SF_QSKIP
+#define SF_QSKIP 2
+#endif
 
 #pragma clang fp contract(off)
 
@@ -1072,10 +1075,14 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE_MX))
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
   double (*L)[kRec] = lds[wib];
   const int kp = lane >> 4, jc = lane & 3;
-  auto group = [&](const double (*Lh)[kRec], int g) {
+  // live[Q] bit r: staged record r of the batch reaches some DOF of the
+  // plane k = Q; a group of 4 records with no live bit in a plane has S = 0
+  // at all 16 of its DOFs (the clamp), so that plane's products are skipped
+  auto group = [&](const double (*Lh)[kRec], int g, const unsigned long long *live) {
     const double *rec = Lh[4 * g + kp];
     const double W = fma(tv, rec[1], fma(tu, rec[0], Kij));
-    const double bval = rec[6 + jc];
+    double bval = rec[6 + jc];
+#if SF_QSKIP == 0
 #pragma unroll
     for (int Q = 0; Q < 4; ++Q) {
       const double S = clamp01(W - rec[2 + Q]);
@@ -1088,15 +1095,61 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE_MX))
       }
       acc[Q] = __builtin_amdgcn_mfma_f64_4x4x4f64(S1, bval, acc[Q], 0, 0, 0);
     }
+#elif SF_QSKIP == 1
+    double S1[4];
+#pragma unroll
+    for (int Q = 0; Q < 4; ++Q) {
+      const double S = clamp01(W - rec[2 + Q]);
+      S1[Q] = S * S;
+      if (ALPHA == 0) {
+        for (int ex = 3; ex <= A.alpha; ++ex) S1[Q] = S * S1[Q];
+      } else {
+#pragma unroll
+        for (int ex = 3; ex <= ALPHA; ++ex) S1[Q] = S * S1[Q];
+      }
+    }
+    asm volatile("" : "+v"(S1[0]), "+v"(S1[1]), "+v"(S1[2]), "+v"(S1[3]), "+v"(bval));
+#pragma unroll
+    for (int Q = 0; Q < 4; ++Q)
+      if (((live[Q] >> (4 * g)) & 0xFull) != 0) acc[Q] = __builtin_amdgcn_mfma_f64_4x4x4f64(S1[Q], bval, acc[Q], 0, 0, 0);
+#else
+    double C[4] = {rec[2], rec[3], rec[4], rec[5]};
+#if SF_QSKIP == 2
+    asm volatile("" : "+v"(C[0]), "+v"(C[1]), "+v"(C[2]), "+v"(C[3]), "+v"(bval));
+#endif
+#pragma unroll
+    for (int Q = 0; Q < 4; ++Q) {
+      if (((live[Q] >> (4 * g)) & 0xFull) == 0) continue;
+      const double S = clamp01(W - C[Q]);
+      double S1 = S * S;
+      if (ALPHA == 0) {
+        for (int ex = 3; ex <= A.alpha; ++ex) S1 = S * S1;
+      } else {
+#pragma unroll
+        for (int ex = 3; ex <= ALPHA; ++ex) S1 = S * S1;
+      }
+      acc[Q] = __builtin_amdgcn_mfma_f64_4x4x4f64(S1, bval, acc[Q], 0, 0, 0);
+    }
+#endif
   };
   // C_k = (w_k - pw)^2 + pu^2 + pv^2 of the 64 records from Lh, one per lane
-  // (rows past the staged ones are overwritten harmlessly and never read)
-  auto form_ck = [&](double (*Lh)[kRec]) {
+  // (rows past the staged ones are overwritten harmlessly and never read),
+  // and the planes each record reaches: plane k is out of reach when the
+  // distance from (pu, pv) to the box of the (u_i, v_j), squared, plus
+  // (w_k - pw)^2 is not below 1 (a margin of 1e-9 covers the rounding of S)
+  auto form_ck = [&](double (*Lh)[kRec], unsigned long long *live) {
     double *r = Lh[lane];
     const double pu = r[0], pv = r[1], pw = r[2];
     const double c0 = w[0] - pw, c1 = w[1] - pw, c2 = w[2] - pw, c3 = w[3] - pw;
     const double P = fma(pv, pv, pu * pu);
     r[2] = fma(c0, c0, P); r[3] = fma(c1, c1, P); r[4] = fma(c2, c2, P); r[5] = fma(c3, c3, P);
+    const double gu = fmax(fabs(pu) - uh, 0.0), gv = fmax(fabs(pv) - vh, 0.0);
+    const double dxy = fma(gv, gv, gu * gu);
+    constexpr double reach = 1.0 + 1e-9;
+    live[0] = __ballot(fma(c0, c0, dxy) < reach);
+    live[1] = __ballot(fma(c1, c1, dxy) < reach);
+    live[2] = __ballot(fma(c2, c2, dxy) < reach);
+    live[3] = __ballot(fma(c3, c3, dxy) < reach);
   };
   int head = 0, tail = 0;   // ring positions in records (head multiple of 64)
   const int q0 = A.roff[t], qend = A.roff[t + 1];
@@ -1137,12 +1190,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE_MX))
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        form_ck(L + (head & (kRing - 1)));
+        unsigned long long live[4];
+        form_ck(L + (head & (kRing - 1)), live);
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #pragma unroll
-        for (int g = 0; g < kWave / 4; ++g) group(L + (head & (kRing - 1)), g);
+        for (int g = 0; g < kWave / 4; ++g) group(L + (head & (kRing - 1)), g, live);
         head += kWave;
         __builtin_amdgcn_wave_barrier();
       }
@@ -1161,11 +1215,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE_MX))
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    form_ck(L + (head & (kRing - 1)));
+    unsigned long long live[4];
+    form_ck(L + (head & (kRing - 1)), live);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    for (int g = 0; g < ng; ++g) group(L + (head & (kRing - 1)), g);
+    for (int g = 0; g < ng; ++g) group(L + (head & (kRing - 1)), g, live);
   }
   const int dofLo = 4 * ((lane >> 2) & 3) + (lane >> 4);
 #pragma unroll
@@ -2407,9 +2462,14 @@ int pic_time_step(pic_handle *h, double dt, int32_t nsteps) {
 int pic_set_tuning(pic_handle *h, const pic_tuning *t) {
   if (!h || !t) return fail(PIC_EINVAL, "pic_set_tuning: null argument");
   if (t->sf_kernel < 0 || t->sf_kernel > 2) return fail(PIC_EINVAL, "pic_set_tuning: sf_kernel must be 0, 1 or 2");
+  if (set_device(h)) return PIC_EHIP;
+  // a captured graph holds the kernel choice it was recorded with, and the
+  // record refresh before a replay follows the current choice: any change
+  // of the tuning records the graph again at the next replayed step
+  HIPCHK(hipStreamSynchronize(h->stream));
+  drop_graph(h);
   h->useGraph = t->use_graph != 0;
   if (t->sf_kernel != 0) h->sfMatrix = t->sf_kernel == 2;
-  if (!h->useGraph && set_device(h) == 0) drop_graph(h);
   return PIC_OK;
 }
 

@@ -266,17 +266,18 @@ def test_refmapping_linear_box_matches_triatracking(oracle_mod):
     s.close()
 
 
-@pytest.mark.parametrize("mesh_kind", ["cube", "deformed"])
-def test_mixed_precision_error_bounds(oracle_mod, mesh_kind):
+@pytest.mark.parametrize("mesh_kind,alpha", [("cube", 2), ("deformed", 2), ("cube", 3), ("deformed", 3)])
+def test_mixed_precision_error_bounds(oracle_mod, mesh_kind, alpha):
     """Mixed precision (FP32 Lagrange basis, field interpolation and shape
     weights with FP32 partial sums, FP64 positions, Newton and face tests)
     against the FP64 oracle: deposited source within 1e-6 relative (L2 and
     max), positions within 1e-9 of the domain after 10 steps.  The Cartesian
     cube runs the tensor-grid gather, the deformed box the general one."""
     w = W.thermal_cube(nx=8, ppc=40, steps=10)
+    w.sf_alpha = alpha
     if mesh_kind == "deformed":
         w = deformed_box(w)
-    print("case mixed precision on %s" % mesh_kind, flush=True)
+    print("case mixed precision on %s alpha=%d" % (mesh_kind, alpha), flush=True)
     kw = W.stage_kwargs(w)
     with stage.PICStage(w.mesh, w.N, max_particles=len(w.PartState), precision=stage.PREC_MIXED, **kw) as s:
         s.set_field(w.E)
@@ -359,6 +360,71 @@ def test_graph_replay_matches_eager(case):
     se.close()
 
 
+def test_tuning_change_rerecords_graph(oracle_mod):
+    """A graph recorded with one shape-function kernel is not replayed after
+    pic_set_tuning chose another: graph on with the matrix-core gather, then
+    the VALU gather for two single eager steps (the sort then writes no
+    records), then two replayed steps.  The source each step leaves behind
+    matches an eager stage that runs the same kernel sequence, and the oracle."""
+    w = W.thermal_cube(nx=6, ppc=20, steps=6)
+    kw = W.stage_kwargs(w)
+    print("case tuning switch under graph replay n=%d" % len(w.PartState), flush=True)
+    sg = stage.PICStage(w.mesh, w.N, max_particles=len(w.PartState), **kw)
+    se = stage.PICStage(w.mesh, w.N, max_particles=len(w.PartState), **kw)
+    o = oracle_mod.Oracle(w.mesh, w.N, deposition=kw["deposition"], sf_radius=kw["sf_radius"],
+                          sf_alpha=kw["sf_alpha"], external_field=kw["external_field"], charge=kw["charge"],
+                          mass=kw["mass"], mpf=kw["mpf"])
+    for s in (sg, se, o):
+        s.set_field(w.E)
+        s.set_particles(w.PartState, elem=w.elem, species=w.species)
+    sg.set_tuning(use_graph=True, sf_kernel=2)
+    se.set_tuning(use_graph=False, sf_kernel=2)
+    plan = [(2, 2), (1, 1), (1, 1), (1, 2), (2, 2)]   # (sf_kernel, steps)
+    for kern, steps in plan:
+        sg.set_tuning(use_graph=True, sf_kernel=kern)
+        se.set_tuning(use_graph=False, sf_kernel=kern)
+        sg.time_step(w.dt, steps)
+        se.time_step(w.dt, steps)
+        o.time_step(w.dt, steps)
+        a, b = sg.particles(), se.particles()
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+        PSg, PSe, oPS = sg.part_source(), se.part_source(), o.part_source()
+        rs = np.abs(oPS).max()
+        assert np.abs(PSg - PSe).max() <= 1e-12 * rs, (kern, steps)
+        assert np.abs(PSg - oPS).max() <= 1e-12 * rs, (kern, steps)
+    sg.close()
+    se.close()
+
+
+@pytest.mark.parametrize("kern", [1, 2])
+@pytest.mark.parametrize("alpha", [2, 3])
+def test_sf_alpha_and_kernel_parity(oracle_mod, alpha, kern):
+    """Shape-function exponent 2 (squared-weight kernels) and 3 (the
+    run-time exponent kernels) on the VALU gather and the matrix-core
+    tensor-grid gather, 5 steps against the oracle."""
+    w = W.thermal_cube(nx=6, ppc=24, steps=5)
+    w.sf_alpha = alpha
+    kw = W.stage_kwargs(w)
+    print("case sf alpha=%d kernel=%d" % (alpha, kern), flush=True)
+    s = stage.PICStage(w.mesh, w.N, max_particles=len(w.PartState), **kw)
+    s.set_tuning(sf_kernel=kern)
+    s.set_field(w.E)
+    s.set_particles(w.PartState, elem=w.elem, species=w.species)
+    o = oracle_mod.Oracle(w.mesh, w.N, deposition=kw["deposition"], sf_radius=kw["sf_radius"],
+                          sf_alpha=kw["sf_alpha"], external_field=kw["external_field"], charge=kw["charge"],
+                          mass=kw["mass"], mpf=kw["mpf"])
+    o.set_field(w.E)
+    o.set_particles(w.PartState, elem=w.elem, species=w.species)
+    s.time_step(w.dt, 5)
+    o.time_step(w.dt, 5)
+    s.deposition()
+    o.deposition()
+    want = stage.KERNEL_SF_VALU if kern == 1 else stage.KERNEL_SF_CARTESIAN
+    assert s.stats()["deposit_kernel"] == want
+    compare(s, o, 1.0, mesh=w.mesh)
+    s.close()
+
+
 @pytest.mark.parametrize("env", ["PIC_SF_CARTESIAN", "PIC_AFFINE"])
 def test_general_kernels_on_cartesian_mesh(oracle_mod, env, monkeypatch):
     """The general shape-function gather and the Newton interpolation, forced
@@ -388,12 +454,13 @@ def deformed_box(w, amp=0.12):
     return w
 
 
-@pytest.mark.parametrize("depo", [stage.DEPO_SF, stage.DEPO_CVWM])
-def test_deformed_box_parity(oracle_mod, depo):
+@pytest.mark.parametrize("depo,alpha", [(stage.DEPO_SF, 2), (stage.DEPO_SF, 3), (stage.DEPO_CVWM, 2)])
+def test_deformed_box_parity(oracle_mod, depo, alpha):
     """Linear hexahedra with bilinear faces (ConcaveElemSide, Newton on the
     trilinear map, the general gather): device localization and 10 steps
     against the oracle, ownership bit-exact."""
     w = deformed_box(W.thermal_cube(nx=5, ppc=30, deposition=depo, steps=10))
+    w.sf_alpha = alpha
     s, o = run_pair(oracle_mod, w, 10, sf_radius=w.sf_radius if depo == stage.DEPO_SF else 0.0)
     if depo == stage.DEPO_SF:
         assert s.stats()["deposit_kernel"] == stage.KERNEL_SF_MATRIX

@@ -81,3 +81,4 @@ def test_create_rejects_reflective_and_mortars():
     m.SideInfo[0, 0] = 104 + 100
     with pytest.raises(stage.StageError, match="mortar"):
         stage.PICStage(m, 1)
+

@@ -319,34 +319,40 @@ def test_cvwm_uniform_lattice(oracle_mod):
     assert np.allclose(rho, 64.0, rtol=1e-13)
 
 
-def _brute_sf(xp, q, dofs, R, alpha, L):
-    """rho at each DOF: sum over particles and 27 periodic images of w_sf q (1-r^2/R^2)^alpha."""
+def _brute_sf(xp, vp, q, dofs, R, alpha, L):
+    """(J_x, J_y, J_z, rho) at each DOF: sum over particles and 27 periodic
+    images of w_sf q (v, 1) (1-r^2/R^2)^alpha."""
     from math import gamma, pi
     B = gamma(1.5) * gamma(alpha + 1) / gamma(alpha + 2.5)
     w = 1 / (2 * B * pi * R ** 3)
-    rho = np.zeros(len(dofs))
+    qv1 = q * np.hstack([vp, np.ones((len(xp), 1))])            # (n, 4)
+    src = np.zeros((len(dofs), 4))
     for s in np.array(np.meshgrid([-1, 0, 1], [-1, 0, 1], [-1, 0, 1], indexing="ij")).reshape(3, -1).T:
         d = dofs[:, None, :] - (xp[None, :, :] + s * L)
         r2 = (d ** 2).sum(-1)
         S = np.where(r2 <= R * R, (1 - r2 / R ** 2) ** alpha, 0.0)
-        rho += w * (S * q).sum(1)
-    return rho
+        src += w * (S @ qv1)
+    return src
 
 
 @pytest.mark.parametrize("alpha", [2, 3])
 def test_sf_brute_force(oracle_mod, alpha):
+    """Charge and all three current components: a velocity component taken
+    from the wrong column, or a dropped factor, fails one of them."""
     m = M.cartesian_box((4, 4, 4))
     R = 1.5 / 4
     o = oracle_mod.Oracle(m, 3, deposition=oracle_mod.DEPO_SF, sf_radius=R, sf_alpha=alpha, charge=(2.0,),
                           mass=(1.0,), mpf=(1.5,))
     rng = np.random.default_rng(9)
     x = rng.uniform(0, 1, (60, 3))
-    o.set_particles(np.hstack([x, rng.normal(size=(60, 3))]), elem=m.element_of(x))
+    v = rng.normal(size=(60, 3)) * np.array([1.0, 3.0, -7.0])
+    o.set_particles(np.hstack([x, v]), elem=m.element_of(x))
     o.deposition()
-    rho = o.part_source()[..., 3].reshape(-1)
+    src = o.part_source().reshape(-1, 4)
     dofs = o.elem_xgp().reshape(-1, 3)
-    ref = _brute_sf(x, 3.0, dofs, R, alpha, np.ones(3))
-    assert np.abs(rho - ref).max() < 1e-12 * np.abs(ref).max()
+    ref = _brute_sf(x, v, 3.0, dofs, R, alpha, np.ones(3))
+    for c in range(4):
+        assert np.abs(src[:, c] - ref[:, c]).max() < 1e-12 * np.abs(ref[:, c]).max(), c
 
 
 def test_sf_weight_normalises_kernel():
