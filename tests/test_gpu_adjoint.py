@@ -121,3 +121,27 @@ def test_autograd_function_wrapper():
     print("directional derivative adjoint %.10e  FD %.10e" % (ad, fd))
     assert abs(ad - fd) <= 1e-6 * abs(fd)
     s.close()
+
+
+def test_adjoint_stated_size():
+    """The differentiable workload at its stated size: 16^3 linear hexes,
+    N = 3, 2,000,000 electrons at 8 eV, alpha = 0.01, K = 50 steps with
+    checkpoints every 10; dL/dalpha from the adjoint against central
+    differences (h = 1e-6) of the device forward."""
+    Kf = 50
+    w, dxda = A.landau_box(n_el=16, n_part=2_000_000, steps=Kf)
+    G = A.periodic_poisson_operator(w.mesh)
+    print("case adjoint landau16 n=%d K=%d every=%d" % (len(w.PartState), Kf, EVERY), flush=True)
+    with make_stage(w, G) as s:
+        s.set_particles(w.PartState, elem=w.elem, species=w.species)
+        loss, gx, gv = s.adjoint(w.dt, Kf, EVERY)
+        dLda = float((gx * dxda).sum())
+        h = 1e-6
+        fdl = []
+        for a in (0.01 + h, 0.01 - h):
+            wa, _ = A.landau_box(n_el=16, n_part=2_000_000, alpha=a, steps=Kf)
+            s.set_particles(wa.PartState, elem=wa.mesh.element_of(wa.PartState[:, :3]), species=wa.species)
+            fdl.append(s.adjoint(w.dt, Kf, EVERY, gradients=False)[0])
+        fd = (fdl[0] - fdl[1]) / (2 * h)
+        print("dL/dalpha adjoint %.10e  FD(device) %.10e  rel %.2e" % (dLda, fd, abs(dLda - fd) / abs(fd)))
+        assert abs(dLda - fd) <= 1e-6 * abs(fd)

@@ -123,6 +123,20 @@ def test_time_step_parity_plasma_wave_box(oracle_mod):
     s.close()
 
 
+def test_plasma_wave_box_stated_size(oracle_mod):
+    """The plain FP64 electrostatic case at its stated size: 16x4x4 linear
+    hexes, N = 3, 64 electrons per cell (16,384), 200 steps, TriaTracking,
+    cell_volweight_mean."""
+    w = W.plasma_wave_box(ppc=64)
+    assert len(w.PartState) == 16384 and w.steps == 200
+    s, o = run_pair(oracle_mod, w, w.steps)
+    compare(s, o, 4 * np.pi, mesh=w.mesh)
+    dep, abs_err, rel_err = s.deposited_charge()
+    assert abs(dep - o.deposited_charge()) <= 1e-12 * abs(dep)
+    assert rel_err <= 1e-12
+    s.close()
+
+
 def test_staged_calls_match_oracle(oracle_mod):
     """Deposition / InterpolateFieldToParticle / push / PerformTracking one by one."""
     w = W.thermal_cube(nx=5, ppc=30, steps=1)
