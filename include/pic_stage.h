@@ -162,6 +162,16 @@ typedef struct {
   double  temperature;            /* MWTemperatureIC                               */
 } pic_emission;
 int  pic_emit_particles(pic_handle *h, int32_t n_inits, const pic_emission *inits, uint64_t seed, int64_t *n_emitted);
+/* pic_emit_particles without device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory during the
+ * call (serving loops, harnesses that forbid hipMalloc).  The device staging
+ * lives in a fixed array the library holds on every GPU it runs on (set up by
+ * pic_create): at most 524,288 particles per call (40.4 MB per GPU); larger
+ * emissions fail with PIC_ECAPACITY.  Results are those of pic_emit_particles.
+ * The staging is shared by every handle on one GPU: calls that overlap on one
+ * GPU (from two threads) race. */
+int  pic_emit_particles_noalloc(pic_handle *h, int32_t n_inits, const pic_emission *inits, uint64_t seed,
+                                int64_t *n_emitted);
 
 /* Particle state in the reference's state-file layout (element-sorted):
  *   PartInt  [nElems][2]  first and last offset of each element's particles

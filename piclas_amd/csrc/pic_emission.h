@@ -126,13 +126,42 @@ __global__ __launch_bounds__(kBlock) void k_emit(EmitArgs A, int64_t n, double *
   spec[p] = uint8_t(I.species);
 }
 
-}  // namespace
+// Staging of the allocation-free emission (pic_emit_particles_noalloc): a
+// file-scope device array, one copy per GPU, for at most kEmitScratchMax
+// particles, carved as PartState [max][6] doubles, positions [max][3]
+// doubles, element [max] int32, species [max] uint8 (every region starts at
+// a multiple of 256 bytes).  Host code never passes `scratch` itself to a
+// kernel or copy: it uses the device address that hipGetSymbolAddress
+// returned for the current GPU (emit_scratch_ptr).  Every call on one GPU
+// shares the array; the emission kernels write every entry they later read,
+// so nothing needs clearing between calls.  Calls that overlap on one GPU
+// (two handles on one device driven from two threads) race.
+constexpr int64_t kEmitScratchMax = int64_t(1) << 19;
+constexpr size_t kEmitScratchBytes = size_t(kEmitScratchMax) * (6 * 8 + 3 * 8 + 4 + 1);
+static __device__ __align__(256) unsigned char scratch[kEmitScratchBytes];
 
-extern "C" {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratchMutex;
+void *g_scratchPtr[kMaxDevices] = {};
 
-int pic_emit_particles(pic_handle *h, int32_t n_inits, const pic_emission *inits, uint64_t seed, int64_t *n_emitted) {
-  if (!h || (n_inits > 0 && !inits)) return fail(PIC_EINVAL, "pic_emit_particles: null argument");
-  if (n_inits < 1 || n_inits > 8) return fail(PIC_EINVAL, "pic_emit_particles: 1 to 8 emission inits");
+// once per device, with that device current (called by pic_create)
+int emit_scratch_init(int device) {
+  if (device < 0 || device >= kMaxDevices) return fail(PIC_EINVAL, "pic_create: device ordinal out of range");
+  std::lock_guard<std::mutex> lock(g_scratchMutex);
+  if (g_scratchPtr[device]) return 0;
+  void *ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr)
+    return fail(PIC_EHIP, std::string("pic_create: hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  g_scratchPtr[device] = ptr;
+  return 0;
+}
+
+int emit_impl(pic_handle *h, int32_t n_inits, const pic_emission *inits, uint64_t seed, int64_t *n_emitted,
+              bool noalloc) {
+  const char *who = noalloc ? "pic_emit_particles_noalloc" : "pic_emit_particles";
+  if (!h || (n_inits > 0 && !inits)) return fail(PIC_EINVAL, std::string(who) + ": null argument");
+  if (n_inits < 1 || n_inits > 8) return fail(PIC_EINVAL, std::string(who) + ": 1 to 8 emission inits");
   if (set_device(h)) return PIC_EHIP;
   EmitArgs A{};
   A.nInits = n_inits;
@@ -144,19 +173,19 @@ int pic_emit_particles(pic_handle *h, int32_t n_inits, const pic_emission *inits
   for (int a = 0; a < n_inits; ++a) {
     const pic_emission &E = inits[a];
     EmitInit &I = A.init[a];
-    if (E.species < 1 || E.species > h->par.nSpecies) return fail(PIC_EINVAL, "pic_emit_particles: species out of range");
+    if (E.species < 1 || E.species > h->par.nSpecies) return fail(PIC_EINVAL, std::string(who) + ": species out of range");
     I.species = E.species - 1;
     I.space = E.space;
     I.velocity = E.velocity;
     I.first = n;
     if (E.space == PIC_SPACE_CUBOID) {
-      if (E.particle_number < 0) return fail(PIC_EINVAL, "pic_emit_particles: negative ParticleNumber");
+      if (E.particle_number < 0) return fail(PIC_EINVAL, std::string(who) + ": negative ParticleNumber");
       I.count = E.particle_number;
       // unit normal of the base vectors (particle_emission_tools.f90:1146-1152)
       const double *a1 = E.base_vector1, *a2 = E.base_vector2;
       double l[3] = {a1[1] * a2[2] - a1[2] * a2[1], a1[2] * a2[0] - a1[0] * a2[2], a1[0] * a2[1] - a1[1] * a2[0]};
       const double nl = std::sqrt(l[0] * l[0] + l[1] * l[1] + l[2] * l[2]);
-      if (!(nl > 0.0)) return fail(PIC_EINVAL, "pic_emit_particles: BaseVectors are parallel");
+      if (!(nl > 0.0)) return fail(PIC_EINVAL, std::string(who) + ": BaseVectors are parallel");
       for (int c = 0; c < 3; ++c) {
         I.base[c] = E.base_point[c];
         I.v1[c] = a1[c];
@@ -165,36 +194,49 @@ int pic_emit_particles(pic_handle *h, int32_t n_inits, const pic_emission *inits
       }
     } else if (E.space == PIC_SPACE_SIN_DEVIATION) {
       if (E.max_particle_number[0] < 1 || E.max_particle_number[1] < 1 || E.max_particle_number[2] < 1)
-        return fail(PIC_EINVAL, "pic_emit_particles: maxParticleNumber must be positive in x, y and z");
+        return fail(PIC_EINVAL, std::string(who) + ": maxParticleNumber must be positive in x, y and z");
       I.count = int64_t(E.max_particle_number[0]) * E.max_particle_number[1] * E.max_particle_number[2];
       for (int c = 0; c < 3; ++c) I.nxyz[c] = E.max_particle_number[c];
       I.amplitude = E.amplitude;
       I.waveNumber = E.wave_number;
     } else {
-      return fail(PIC_EINVAL, "pic_emit_particles: unknown space type");
+      return fail(PIC_EINVAL, std::string(who) + ": unknown space type");
     }
     if (E.velocity != PIC_VELO_CONSTANT && E.velocity != PIC_VELO_MAXWELL_LPN)
-      return fail(PIC_EINVAL, "pic_emit_particles: unknown velocity distribution");
+      return fail(PIC_EINVAL, std::string(who) + ": unknown velocity distribution");
     I.veloIC = E.velo_ic;
     for (int c = 0; c < 3; ++c) I.veloVec[c] = E.velo_vec[c];
     I.temperature = E.temperature;
     I.mass = h->par.mass[E.species - 1];
     n += I.count;
   }
-  if (n > h->cap) return fail(PIC_ECAPACITY, "pic_emit_particles: more particles than max_particles");
+  if (n > h->cap) return fail(PIC_ECAPACITY, std::string(who) + ": more particles than max_particles");
   // staging on the device: PartState [n][6], positions [n][3], element, species
   double *state = nullptr, *xyz = nullptr;
   int32_t *elem = nullptr;
   uint8_t *spec = nullptr;
   const size_t nn = size_t(std::max<int64_t>(n, 1));
   auto release = [&]() {
+    if (noalloc) return;
     if (state) (void)hipFree(state);
     if (xyz) (void)hipFree(xyz);
     if (elem) (void)hipFree(elem);
     if (spec) (void)hipFree(spec);
   };
-  if (hipMalloc(&state, nn * 6 * sizeof(double)) != hipSuccess || hipMalloc(&xyz, nn * 3 * sizeof(double)) != hipSuccess ||
-      hipMalloc(&elem, nn * sizeof(int32_t)) != hipSuccess || hipMalloc(&spec, nn) != hipSuccess) {
+  if (noalloc) {
+    if (n > kEmitScratchMax)
+      return fail(PIC_ECAPACITY, std::string(who) + ": " + std::to_string(n) + " particles exceed the staging limit of " +
+                                     std::to_string(kEmitScratchMax) + " (use pic_emit_particles)");
+    unsigned char *base = (h->device >= 0 && h->device < kMaxDevices)
+                              ? static_cast<unsigned char *>(g_scratchPtr[h->device]) : nullptr;
+    if (!base) return fail(PIC_ESTATE, std::string(who) + ": staging of this device not initialised");
+    state = reinterpret_cast<double *>(base);
+    xyz = reinterpret_cast<double *>(base + size_t(kEmitScratchMax) * 48);
+    elem = reinterpret_cast<int32_t *>(base + size_t(kEmitScratchMax) * 72);
+    spec = base + size_t(kEmitScratchMax) * 76;
+  } else if (hipMalloc(&state, nn * 6 * sizeof(double)) != hipSuccess ||
+             hipMalloc(&xyz, nn * 3 * sizeof(double)) != hipSuccess ||
+             hipMalloc(&elem, nn * sizeof(int32_t)) != hipSuccess || hipMalloc(&spec, nn) != hipSuccess) {
     release();
     return fail(PIC_EHIP, "pic_emit_particles: device staging allocation");
   }
@@ -214,7 +256,7 @@ int pic_emit_particles(pic_handle *h, int32_t n_inits, const pic_emission *inits
         hipMemcpy(he.data(), elem, size_t(n) * sizeof(int32_t), hipMemcpyDeviceToHost) != hipSuccess ||
         hipMemcpy(hsp.data(), spec, size_t(n), hipMemcpyDeviceToHost) != hipSuccess) {
       release();
-      return fail(PIC_EHIP, "pic_emit_particles: emission kernels");
+      return fail(PIC_EHIP, std::string(who) + ": emission kernels");
     }
   }
   release();
@@ -233,6 +275,19 @@ int pic_emit_particles(pic_handle *h, int32_t n_inits, const pic_emission *inits
   if (int rc = set_particles_impl(h, n, hs.data(), el.data(), sp.data(), isNew.data(), true)) return rc;
   if (n_emitted) *n_emitted = inside;
   return PIC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pic_emit_particles(pic_handle *h, int32_t n_inits, const pic_emission *inits, uint64_t seed, int64_t *n_emitted) {
+  return emit_impl(h, n_inits, inits, seed, n_emitted, false);
+}
+
+int pic_emit_particles_noalloc(pic_handle *h, int32_t n_inits, const pic_emission *inits, uint64_t seed,
+                               int64_t *n_emitted) {
+  return emit_impl(h, n_inits, inits, seed, n_emitted, true);
 }
 
 }  // extern "C"

@@ -1738,6 +1738,7 @@ int set_device(pic_handle *h) {
 }
 
 void drop_graph(pic_handle *h);
+int emit_scratch_init(int device);
 int set_particles_impl(pic_handle *h, int64_t n, const double *PartState, const int32_t *elem, const int32_t *species,
                        const uint8_t *is_new, bool outsideMarked);
 
@@ -1914,6 +1915,7 @@ int pic_create(const pic_mesh_desc *mesh, const pic_params *par, pic_handle **ou
   std::string err = build_host_mesh(*mesh, *par, h->hm);
   if (!err.empty()) { delete h; return fail(PIC_EINVAL, err); }
   if (set_device(h)) { delete h; return PIC_EHIP; }
+  if (int rc = emit_scratch_init(h->device)) { delete h; return rc; }
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) { delete h; return fail(PIC_EHIP, "stream"); }
   for (auto &e : h->ev) (void)hipEventCreate(&e);
   const HostMesh &hm = h->hm;

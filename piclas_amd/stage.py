@@ -106,6 +106,8 @@ EXPORTS = {
     "pic_get_stats": (ctypes.c_int, [_H, ctypes.POINTER(PicStats)]),
     "pic_emit_particles": (ctypes.c_int, [_H, ctypes.c_int32, ctypes.POINTER(PicEmission), ctypes.c_uint64,
                                           ctypes.POINTER(ctypes.c_int64)]),
+    "pic_emit_particles_noalloc": (ctypes.c_int, [_H, ctypes.c_int32, ctypes.POINTER(PicEmission), ctypes.c_uint64,
+                                                  ctypes.POINTER(ctypes.c_int64)]),
     "pic_get_particle_data": (ctypes.c_int, [_H, ctypes.c_int64, _f64p, _i64p, ctypes.POINTER(ctypes.c_int64)]),
     "pic_restart_particles": (ctypes.c_int, [_H, ctypes.c_int64, _f64p, _i64p, ctypes.POINTER(ctypes.c_int64),
                                              ctypes.POINTER(ctypes.c_int64)]),
@@ -301,11 +303,13 @@ class PICStage:
                                       "data": (self.part_source_device(), False), "version": 2, "strides": None}
         return torch.as_tensor(v, device=device or torch.device("cuda", 0))
 
-    def emit_particles(self, inits, seed=1):
+    def emit_particles(self, inits, seed=1, noalloc=False):
         """Particle emission on the device (SetParticlePosition 'cuboid' /
         'sin_deviation', SetParticleVelocity 'constant' / 'maxwell_lpn'):
         inits is a list of dicts with the PicEmission fields (species is
         1-based); sets them as the particles, located on the device.
+        noalloc selects pic_emit_particles_noalloc (fixed device staging, no
+        allocation in the call, at most 524,288 particles).
         Returns the number of particles inside the mesh."""
         arr = (PicEmission * len(inits))()
         for a, d in zip(arr, inits):
@@ -315,8 +319,8 @@ class PICStage:
                 else:
                     setattr(a, k, v)
         out = ctypes.c_int64(0)
-        _chk(load_library().pic_emit_particles(self.h, len(inits), arr, ctypes.c_uint64(seed), ctypes.byref(out)),
-             "pic_emit_particles")
+        name = "pic_emit_particles_noalloc" if noalloc else "pic_emit_particles"
+        _chk(getattr(load_library(), name)(self.h, len(inits), arr, ctypes.c_uint64(seed), ctypes.byref(out)), name)
         self.nPart = sum(d.get("particle_number", 0) if d.get("space", SPACE_CUBOID) == SPACE_CUBOID
                          else int(np.prod(d["max_particle_number"])) for d in inits)
         return out.value
