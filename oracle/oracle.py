@@ -78,6 +78,13 @@ def lib():
         L.om_position_in_ref_elem.argtypes = [ctypes.c_void_p, ctypes.c_int32, _f64p, _i32p, _f64p, _u8p]
         L.om_eval_field_at_ref_pos.argtypes = [ctypes.c_void_p, ctypes.c_int32, _f64p, _i32p, _f64p]
         L.om_em_field_dw.argtypes = [ctypes.c_void_p, ctypes.c_int32, _f64p, _i32p, _f64p]
+        L.om_get_n_bc_sides.restype = ctypes.c_int32
+        L.om_get_n_bc_sides.argtypes = [ctypes.c_void_p]
+        L.om_get_bc_sides.argtypes = [ctypes.c_void_p, _i32p, _u8p, _f64p, _f64p, _f64p]
+        L.om_get_bc_list_size.restype = ctypes.c_int32
+        L.om_get_bc_list_size.argtypes = [ctypes.c_void_p]
+        L.om_get_bc_elem_lists.argtypes = [ctypes.c_void_p, _i32p, _i32p, _f64p]
+        L.om_bc_side_intersection.argtypes = [ctypes.c_void_p, ctypes.c_int32, _f64p, _f64p, _f64p]
         L.om_set_part_source.argtypes = [ctypes.c_void_p, _f64p]
         L.om_get_field.argtypes = [ctypes.c_void_p, _f64p]
         L.om_legendre_gauss_nodes.argtypes = [ctypes.c_int32, _f64p, _f64p]
@@ -139,6 +146,7 @@ class Oracle:
         if not self.h:
             raise RuntimeError("om_create: " + L.om_last_error().decode())
         self.nElems, self.N = ei.shape[0], N
+        self.NGeo = mesh.NGeo
         self.nDof = (N + 1) ** 3
 
     def __del__(self):
@@ -346,6 +354,37 @@ class Oracle:
 
     def field_energy(self):
         return lib().om_field_energy(self.h)
+
+    # --- RefMapping boundary tracking tables (meshes with non-periodic boundaries)
+    def bc_sides(self):
+        """(side rows, SideType, Bezier control points [nBC][g1][g1][3], origin, radius)"""
+        n = lib().om_get_n_bc_sides(self.h)
+        g1 = self.NGeo + 1
+        side = np.zeros(n, np.int32)
+        typ = np.zeros(n, np.uint8)
+        bez = np.zeros((n, g1, g1, 3))
+        org = np.zeros((n, 3))
+        rad = np.zeros(n)
+        lib().om_get_bc_sides(self.h, _p(side, _i32p), _p(typ, _u8p), _p(bez, _f64p), _p(org, _f64p), _p(rad, _f64p))
+        return side, typ, bez, org, rad
+
+    def bc_elem_lists(self):
+        m = lib().om_get_bc_list_size(self.h)
+        off = np.zeros(self.nElems + 1, np.int32)
+        bs = np.zeros(m, np.int32)
+        dist = np.zeros(m)
+        lib().om_get_bc_elem_lists(self.h, _p(off, _i32p), _p(bs, _i32p), _p(dist, _f64p))
+        return off, bs, dist
+
+    def bc_side_intersection(self, b, last, x):
+        """(alpha, xi, eta) of the segment last -> x with boundary side b; alpha = -1: no hit"""
+        last = np.ascontiguousarray(last, np.float64)
+        x = np.ascontiguousarray(x, np.float64)
+        out = np.zeros(3)
+        rc = lib().om_bc_side_intersection(self.h, int(b), _p(last, _f64p), _p(x, _f64p), _p(out, _f64p))
+        if rc != 0:
+            raise RuntimeError(lib().om_last_error().decode())
+        return out
 
 
 def legendre_gauss_nodes(N):

@@ -174,6 +174,16 @@ struct om_state {
   double *dXCL1;           /* [e][8][9] dXCL_NGeo at the 8 corner CL points   */
   double *epsOneCell;      /* [e] ElemEpsOneCell                              */
   uint8_t *openElem;       /* [e] element has an open particle boundary side  */
+  /* boundary tracking with RefMapping (pic_oracle_bc.h) */
+  int cartPeriodic;        /* every boundary periodic: PeriodicMovement (CartesianPeriodic = T) */
+  int nBC;                 /* boundary sides (HOPR side rows with a BC)        */
+  int32_t *bcSide, *sideToBC;
+  double *bz;              /* [nBC][NGeo+1][NGeo+1][3] BezierControlPoints3D   */
+  uint8_t *sideType, *bbEmpty;
+  double *sideNorm, *sideDist, *slabN, *slabI, *baseV, *bcOrigin, *bcRadius;
+  int32_t *e2bOff, *e2bSide; /* ElemToBCSides (CSR) and SideBCMetrics side    */
+  double *e2bDist;           /* SideBCMetrics(BCSIDE_DISTANCE)                */
+  double bcHalo;
   double gmin[3], gmax[3]; /* GEO%x/y/zminglob, maxglob                       */
   /* self-consistent field through a fixed linear operator (adjoint workload) */
   double *Gop;             /* [3*nUnique][nElems]: unique-node E from element-mean rho */
@@ -269,6 +279,8 @@ static void vandermonde(int Nin, const double *xin, int Nout, const double *xout
 }
 
 /* ------------------------------------------------------------ mesh init */
+#include "pic_oracle_bc.h"
+
 static int build_mesh(om_state *s, const om_mesh_desc *m) {
   s->nElems = m->nElems; s->nSides = m->nSides; s->nNodes = m->nNodes;
   s->NGeo = m->NGeo; s->N = m->N;
@@ -630,6 +642,7 @@ static int build_mesh(om_state *s, const om_mesh_desc *m) {
       }
       s->epsOneCell[e] = 1.0 + sqrt(3.0 * (mx / mn) * s->par.refmapping_eps);
     }
+    if (build_bc_tables(s) != 0) return -1;
   }
 
   /* Periodic node twins (InitializePeriodicNodes, pic_depo.f90:1218-): every
@@ -1092,7 +1105,9 @@ void om_destroy(om_state *s) {
                   s->bary, s->radius, s->xiBasis, s->slen, s->xGPe, s->sJ, s->perOff, s->perNodes,
                   s->NodeVolume, s->binOff, s->binElems, s->E, s->PS, s->NodeSource, s->PartState,
                   s->LastPartPos, s->Field, s->elem, s->lastElem, s->species, s->inside, s->isNew,
-                  s->curved, s->dXCL1, s->epsOneCell, s->openElem, s->PartPosRef, s->Gop, s->rhoBar, s->Enode};
+                  s->curved, s->dXCL1, s->epsOneCell, s->openElem, s->PartPosRef, s->Gop, s->rhoBar, s->Enode,
+                  s->bcSide, s->sideToBC, s->bz, s->sideType, s->bbEmpty, s->sideNorm, s->sideDist, s->slabN,
+                  s->slabI, s->baseV, s->bcOrigin, s->bcRadius, s->e2bOff, s->e2bSide, s->e2bDist};
   for (size_t i = 0; i < sizeof ptrs / sizeof ptrs[0]; i++) free(ptrs[i]);
   for (int t = 0; t < s->nPriv; t++) free(s->priv[t]);
   free(s->priv);
@@ -1207,6 +1222,35 @@ int om_position_in_ref_elem(om_state *s, int32_t n, const double *x, const int32
 int om_eval_field_at_ref_pos(om_state *s, int32_t n, const double *xi, const int32_t *elem, double *U) {
   for (int i = 0; i < n; i++) eval_field_at_ref_pos(s, &xi[i * 3], elem[i] - 1, &U[i * 3]);
   return 0;
+}
+int32_t om_get_n_bc_sides(const om_state *s) { return s->nBC; }
+int om_get_bc_sides(const om_state *s, int32_t *side, uint8_t *type, double *bez, double *origin, double *radius) {
+  const int g1 = s->NGeo + 1;
+  for (int b = 0; b < s->nBC; b++) {
+    side[b] = s->bcSide[b];
+    type[b] = s->sideType[b];
+    radius[b] = s->bcRadius[b];
+    for (int c = 0; c < 3; c++) origin[b * 3 + c] = s->bcOrigin[b * 3 + c];
+  }
+  memcpy(bez, s->bz, (size_t)s->nBC * g1 * g1 * 3 * sizeof(double));
+  return 0;
+}
+int32_t om_get_bc_list_size(const om_state *s) { return s->e2bOff ? s->e2bOff[s->nElems] : 0; }
+int om_get_bc_elem_lists(const om_state *s, int32_t *off, int32_t *bcside, double *dist) {
+  if (!s->e2bOff) return -1;
+  memcpy(off, s->e2bOff, (size_t)(s->nElems + 1) * sizeof(int32_t));
+  memcpy(bcside, s->e2bSide, (size_t)s->e2bOff[s->nElems] * sizeof(int32_t));
+  memcpy(dist, s->e2bDist, (size_t)s->e2bOff[s->nElems] * sizeof(double));
+  return 0;
+}
+int om_bc_side_intersection(om_state *s, int32_t b, const double *last, const double *x, double *out) {
+  if (b < 0 || b >= s->nBC) { set_err("boundary side out of range"); return -1; }
+  double traj[3] = {x[0] - last[0], x[1] - last[1], x[2] - last[2]};
+  const double len = sqrt(dot3(traj, traj));
+  for (int c = 0; c < 3; c++) traj[c] = traj[c] / len;
+  const int sd = s->bcSide[b];
+  const int flip = SIDEINFO(sd, SIDE_ID) > 0 ? 0 : SIDEINFO(sd, SIDE_FLIP) % 10;
+  return side_intersection(s, b, flip, last, x, traj, len, &out[0], &out[1], &out[2]);
 }
 int om_em_field_dw(om_state *s, int32_t n, const double *x, const int32_t *elem, double *U) {
   for (int i = 0; i < n; i++) em_field_dw(s, elem[i] - 1, &x[i * 3], &U[i * 3]);
@@ -1518,23 +1562,33 @@ static size_t bgm_cell(const om_state *s, const double x[3]) {
   return ((size_t)b3[2] * s->nb[1] + b3[1]) * s->nb[0] + b3[0];
 }
 
-/* ParticleRefTracking, particle_reftracking.f90:30-414, for CartesianPeriodic = T
- * (periodic boundaries by PeriodicMovement) and open walls.  The reference
- * sends particles of elements with boundary sides through ParticleBCTracking,
- * which intersects the trajectory with the (curved) boundary faces; here a
- * particle that no element holds after the move and whose last element has an
- * open side has left through that wall and is removed.  Both decide the same
- * unless a trajectory leaves and re-enters the domain within one step. */
+static int single_point_to_element(const om_state *s, const double x[3]);
+static int is_bc_elem(const om_state *s, int e) { return s->e2bOff && s->e2bOff[e + 1] > s->e2bOff[e]; }
+
+/* ParticleRefTracking, particle_reftracking.f90:30-414.  When every boundary
+ * is periodic (CartesianPeriodic = T) periodic boundaries are crossed by
+ * PeriodicMovement; otherwise particles of elements with boundary sides in
+ * range go through ParticleBCTracking (open walls and periodic sides by
+ * intersecting the trajectory with the boundary faces). */
 static int ref_track_one(om_state *s, int i) {
   double *x = &s->PartState[(size_t)i * 6], *last = &s->LastPartPos[(size_t)i * 3];
   double *ref = &s->PartPosRef[(size_t)i * 3];
   const int LastElemID = s->lastElem[i];
   int ab;
-  if (s->nPV > 0)
-    for (int g = 0; g < 4; g++)
-      if (!periodic_movement(s, x, last)) break;
-  position_in_ref_elem(s, x, ref, LastElemID, 0, 0, &ab);
-  if (max_abs3(ref) < 1.0) { s->elem[i] = LastElemID; return 0; }
+  if (is_bc_elem(s, LastElemID)) {
+    int ElemID = LastElemID, done = 0, moved = 0;
+    double len0 = 0.;
+    if (bc_tracking(s, i, &len0, &ElemID, 1, &done, &moved) != 0) return -1;
+    if (done) return 0;
+    position_in_ref_elem(s, x, ref, ElemID, 0, 0, &ab);
+    if (max_abs3(ref) < 1.0) { s->elem[i] = ElemID; return 0; }
+  } else {
+    if (s->cartPeriodic && s->nPV > 0)
+      for (int g = 0; g < 4; g++)
+        if (!periodic_movement(s, x, last)) break;
+    position_in_ref_elem(s, x, ref, LastElemID, 0, 0, &ab);
+    if (max_abs3(ref) < 1.0) { s->elem[i] = LastElemID; return 0; }
+  }
   const int oldElemID = LastElemID;
   const size_t b = bgm_cell(s, x);
   const int n = s->binOff[b + 1] - s->binOff[b];
@@ -1573,10 +1627,6 @@ static int ref_track_one(om_state *s, int i) {
       newElemID = e;
     }
   }
-  if (s->openElem[LastElemID]) { /* left the domain through an open wall: RemoveParticle */
-    s->inside[i] = 0;
-    return 0;
-  }
   int TestElem;
   if (max_abs3(oldXi) < max_abs3(newXi)) {
     for (int c = 0; c < 3; c++) ref[c] = oldXi[c];
@@ -1586,9 +1636,36 @@ static int ref_track_one(om_state *s, int i) {
     TestElem = newElemID;
   }
   s->elem[i] = TestElem;
-  if (TestElem < 0 || max_abs3(ref) > s->epsOneCell[TestElem]) {
+  if (TestElem >= 0 && max_abs3(ref) <= s->epsOneCell[TestElem]) return 0;
+  if (TestElem < 0 || !is_bc_elem(s, TestElem)) {   /* Tolerance Issue with internal element: abort */
     set_err("RefMapping: particle %d not inside of element %d", i + 1, TestElem + 1);
     return -1;
+  }
+  /* boundary element (:325-405): FallBackFaceIntersection (NGeo > 1), then
+   * ParticleBCTracking again, then the element or LocateParticleInElement */
+  if (s->NGeo > 1) {
+    const int rc = fallback_face_intersection(s, i, TestElem);
+    if (rc < 0) return -1;
+    if (rc == 1) {   /* no face hit: LocateParticleInElement; not found aborts */
+      const int e = single_point_to_element(s, x);
+      if (e < 0) { s->inside[i] = 0; s->lost++; return 0; }
+      s->elem[i] = e;
+      position_in_ref_elem(s, x, ref, e, 0, 0, &ab);
+      return 0;
+    }
+  }
+  int done = 0, moved = 0;
+  double len0 = 0.;
+  if (bc_tracking(s, i, &len0, &TestElem, 1, &done, &moved) != 0) return -1;
+  if (done) return 0;
+  position_in_ref_elem(s, x, ref, TestElem, 0, 0, &ab);
+  if (max_abs3(ref) > s->epsOneCell[TestElem]) {   /* Tolerance Issue with BC element, relocating */
+    const int e = single_point_to_element(s, x);
+    if (e < 0) { s->inside[i] = 0; s->lost++; return 0; }
+    s->elem[i] = e;
+    position_in_ref_elem(s, x, ref, e, 0, 0, &ab);
+  } else {
+    s->elem[i] = TestElem;
   }
   return 0;
 }

@@ -129,6 +129,19 @@ int  om_eval_field_at_ref_pos(om_state *s, int32_t n, const double *xi, const in
 /* GetEMFieldDW (inverse-distance weights to the Gauss points) at physical
  * positions, whatever RefElemNewton would say: for pinning the loop exit. */
 int  om_em_field_dw(om_state *s, int32_t n, const double *x, const int32_t *elem, double *U);
+/* RefMapping boundary tracking tables (CartesianPeriodic = F meshes):
+ * om_get_n_bc_sides: boundary sides (0 when every boundary is periodic);
+ * om_get_bc_sides: per boundary side its 0-based HOPR side row, SideType,
+ *   Bezier control points [nBC][NGeo+1][NGeo+1][3] (p, q, xyz), origin and radius;
+ * om_get_bc_elem_lists: CSR of every element's sides in range (offsets
+ *   [nElems+1], boundary side index, distance) in the order of the lists;
+ * om_bc_side_intersection: intersection of the segment last -> x with one
+ *   boundary side as ParticleBCTracking computes it (alpha = -1: no hit). */
+int32_t om_get_n_bc_sides(const om_state *s);
+int  om_get_bc_sides(const om_state *s, int32_t *side, uint8_t *type, double *bez, double *origin, double *radius);
+int32_t om_get_bc_list_size(const om_state *s);
+int  om_get_bc_elem_lists(const om_state *s, int32_t *off, int32_t *bcside, double *dist);
+int  om_bc_side_intersection(om_state *s, int32_t b, const double *last, const double *x, double *alpha_xi_eta);
 
 /* Analysis: integral of rho (CalcDepositedCharge) and electric field energy (CalcPotentialEnergy) */
 double om_deposited_charge(const om_state *s);

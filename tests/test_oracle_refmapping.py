@@ -178,10 +178,11 @@ def test_open_wall_removes_and_periodic_z_wraps(oracle_mod):
                           charge=(0.0,), mass=(1.0,), mpf=(1.0,))
     o.set_field(np.zeros((m.nElems, 64, 3)))
     # 0: radially outward through r = 1; 1: inward through r = 0.5; 2: +z through z = 1; 3: -z through z = 0
+    # (2 and 3 inside a sector, away from the radial faces between sectors)
     ps = np.array([[0.97, 0.01, 0.5, 1.0, 0.0, 0.0],
                    [0.52, 0.02, 0.5, -1.0, 0.0, 0.0],
-                   [0.0, 0.75, 0.98, 0.0, 0.0, 1.0],
-                   [-0.8, 0.0, 0.01, 0.0, 0.0, -1.0]])
+                   [0.2, 0.75, 0.98, 0.0, 0.0, 1.0],
+                   [-0.8, 0.05, 0.01, 0.0, 0.0, -1.0]])
     o.set_particles(ps, elem=np.zeros(4, np.int32))
     assert o.locate() == 0
     o.time_step(0.05, 1)
