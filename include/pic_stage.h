@@ -248,10 +248,18 @@ int  pic_adjoint(pic_handle *h, double dt, int32_t K, int32_t checkpoint_every, 
  *              synchronisation between steps; per-stage timers then report the
  *              replays as push/track time)
  *   sf_kernel  0 default, 1 shape-function sums on the vector ALUs, 2 on the
- *              matrix cores (N = 3) */
+ *              matrix cores (N = 3)
+ *   sf_persistent  1: the matrix-core shape-function sums run as persistent
+ *              waves, as many as the device holds at once, that take target
+ *              elements from atomic counters (one per XCD range of
+ *              elements) instead of one wave per target element; for meshes
+ *              whose elements hold very different particle counts.  Each
+ *              target is summed by one wave in the same order either way, so
+ *              the result is bit-identical. */
 typedef struct {
   int32_t use_graph;
   int32_t sf_kernel;
+  int32_t sf_persistent;
 } pic_tuning;
 int  pic_set_tuning(pic_handle *h, const pic_tuning *t);
 int  pic_get_stats(pic_handle *h, pic_stats *st);

@@ -57,6 +57,22 @@ struct DevMesh {
   // mixed precision: E and the Gauss basis in FP32
   const float *E32;         // [e][nDof][3]
   float xGP32[8], wBaryGP32[8];
+  // RefMapping boundary tracking (ParticleBCTracking); null unless some
+  // boundary is not periodic (then cartPeriodic = 0)
+  int cartPeriodic;         // every boundary periodic: PeriodicMovement
+  double bcHalo;            // BC_halo_eps of the element lists
+  const int32_t *bcElemNb;  // [nBC] element across (periodic), -1
+  const uint8_t *bcKind;    // [nBC] PIC_BC_*
+  const int8_t *bcPV;       // [nBC] periodic vector id
+  const uint8_t *bcFlip;    // [nBC]
+  const double *bz;         // [nBC][NGeo+1][NGeo+1][3] Bezier control points
+  const uint8_t *sideType;  // [nBC]
+  const uint8_t *bbEmpty;   // [nBC]
+  const double *sideNorm, *sideDist, *slabN, *slabI, *baseV;
+  const int32_t *e2bOff;    // [nElems+1] boundary sides in range of each element
+  const int32_t *e2bSide;   //   boundary side index
+  const double *e2bDist;    //   distance (sorted ascending per element)
+  const double *e2bSlab;    //   [6] the side's normal slab (n, lo, hi of n . X, 0)
 };
 
 struct DevParts {
@@ -209,9 +225,20 @@ __device__ inline bool ref_elem_newton(const DevMesh &M, int e, const double xin
   return ok;
 }
 
+// A wave-uniform pointer in scalar registers: the scalar loads below take
+// their address from SGPRs, and a uniform value that the compiler keeps in
+// vector registers (across divergent control flow) is read from the first
+// active lane (no instruction when it is already scalar).
+__device__ inline const double *uniform_ptr(const double *p) {
+  const uint64_t a = reinterpret_cast<uint64_t>(p);
+  const uint32_t lo = __builtin_amdgcn_readfirstlane(uint32_t(a)), hi = __builtin_amdgcn_readfirstlane(uint32_t(a >> 32));
+  return reinterpret_cast<const double *>((uint64_t(hi) << 32) | uint64_t(lo));
+}
+
 // GetRefNewtonStartValue, interpolation/eval_xyz.f90:498-612 (guesses 1, 2, 4)
 // the 3 coordinates of one point by scalar loads (p wave-uniform; exactly 24 bytes read)
 __device__ inline void sload_xyz(const double *p, double out[3]) {
+  p = uniform_ptr(p);
   typedef uint32_t dw4 __attribute__((ext_vector_type(4)));
   typedef uint32_t dw2 __attribute__((ext_vector_type(2)));
   dw4 a;
@@ -285,14 +312,14 @@ __device__ inline void ref_newton_start(const DevMesh &M, int e, const double x[
 // EvaluateFieldAtRefPos with NVar=3 (interpolation/eval_xyz.f90:167-295),
 // summed by factors: U = sum_k L_k sum_j L_j sum_i L_i U_ijk (the reference
 // sums sum_ijk U_ijk L_i (L_j L_k); the two agree to round-off).
+// (Ue: the element's field values)
 template <int N>
-__device__ inline void eval_field(const DevMesh &M, int e, const double xi[3], double U[3]) {
+__device__ inline void eval_field_p(const DevMesh &M, const double *Ue, const double xi[3], double U[3]) {
 #pragma clang fp contract(fast)
   constexpr int n1 = N + 1;
   double L[3][n1];
 #pragma unroll
   for (int d = 0; d < 3; ++d) lagrange<n1>(xi[d], M.xGP, M.wBaryGP, L[d]);
-  const double *Ue = M.E + (size_t)e * n1 * n1 * n1 * 3;
   U[0] = U[1] = U[2] = 0.0;
 #pragma unroll 1
   for (int k = 0; k < n1; ++k) {
@@ -316,6 +343,10 @@ __device__ inline void eval_field(const DevMesh &M, int e, const double xi[3], d
     U[2] += s2 * L[2][k];
   }
 }
+template <int N>
+__device__ inline void eval_field(const DevMesh &M, int e, const double xi[3], double U[3]) {
+  eval_field_p<N>(M, M.E + (size_t)e * (N + 1) * (N + 1) * (N + 1) * 3, xi, U);
+}
 
 // ---- wave-uniform element: table reads by scalar loads ---------------------
 // When every lane of a wave maps in the same element, the element's tables
@@ -335,7 +366,7 @@ __device__ inline double dw_to_d(uint32_t lo, uint32_t hi) {
 // size, nothing past the table is read)
 template <int NV>
 __device__ inline void sload_chunk(const double *p, int c, double buf[16]) {
-  const double *q = p + 16 * c;
+  const double *q = uniform_ptr(p + 16 * c);
   const int m = NV - 16 * c;
   if (m >= 16) {
     // two loads in flight, one wait
@@ -420,8 +451,11 @@ __device__ inline void eval_field_s(const DevMesh &M, int e, const double xi[3],
 // visited before are zeroed, that point weighs 1, the rest of its m-row is
 // skipped and the loop continues with the next l.  Two passes restate that
 // without a per-point weight array.
+// Kept out of line and unrolled once: it runs only where Newton failed, and
+// inlined into the push kernels it raised their register count (affine push
+// 118 -> 188 VGPRs).
 template <int N>
-__device__ inline void em_field_dw(const DevMesh &M, int e, const double x[3], double U[3]) {
+__device__ __attribute__((noinline)) void em_field_dw(const DevMesh &M, int e, const double x[3], double U[3]) {
   constexpr int n1 = N + 1, nd = n1 * n1 * n1;
   const double *xg = M.xGPe + (size_t)e * nd * 3;
   const double *Ue = M.E + (size_t)e * nd * 3;
@@ -432,8 +466,11 @@ __device__ inline void em_field_dw(const DevMesh &M, int e, const double x[3], d
   // pass 1: DistSum and the (last) Gauss point hit, in loop order
   double sum = 0.0;
   int hitK = -1, hitL = -1, hitM = -1;
+#pragma unroll 1
   for (int k = 0; k < n1; ++k)
+#pragma unroll 1
     for (int l = 0; l < n1; ++l)
+#pragma unroll 1
       for (int m = 0; m < n1; ++m) {
         const double nrm = norm_of((m * n1 + l) * n1 + k);
         if (nrm > 0.) sum = sum + 1. / nrm;
@@ -441,8 +478,11 @@ __device__ inline void em_field_dw(const DevMesh &M, int e, const double x[3], d
       }
   // pass 2: weights; before the hit they are zero, in the hit's m-row after it too
   U[0] = U[1] = U[2] = 0.0;
+#pragma unroll 1
   for (int k = 0; k < n1; ++k)
+#pragma unroll 1
     for (int l = 0; l < n1; ++l)
+#pragma unroll 1
       for (int m = 0; m < n1; ++m) {
         const int r = (m * n1 + l) * n1 + k;
         double w;
@@ -484,12 +524,11 @@ __device__ inline void lagrange_f32(float x, const float *xg, const float *wb, f
 }
 
 template <int N>
-__device__ inline void eval_field_f32(const DevMesh &M, int e, const double xi[3], double U[3]) {
+__device__ inline void eval_field_f32_p(const DevMesh &M, const float *Ue, const double xi[3], double U[3]) {
   constexpr int n1 = N + 1;
   float L[3][n1];
 #pragma unroll
   for (int d = 0; d < 3; ++d) lagrange_f32<n1>(float(xi[d]), M.xGP32, M.wBaryGP32, L[d]);
-  const float *Ue = M.E32 + (size_t)e * n1 * n1 * n1 * 3;
   float u0 = 0.f, u1 = 0.f, u2 = 0.f;
 #pragma unroll 1
   for (int k = 0; k < n1; ++k)
@@ -506,6 +545,10 @@ __device__ inline void eval_field_f32(const DevMesh &M, int e, const double xi[3
       }
     }
   U[0] = u0; U[1] = u1; U[2] = u2;
+}
+template <int N>
+__device__ inline void eval_field_f32(const DevMesh &M, int e, const double xi[3], double U[3]) {
+  eval_field_f32_p<N>(M, M.E32 + (size_t)e * (N + 1) * (N + 1) * (N + 1) * 3, xi, U);
 }
 
 // eval_field_f32 for a wave whose lanes are all in element e (e wave-uniform):

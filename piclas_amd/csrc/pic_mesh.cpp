@@ -112,6 +112,378 @@ void change_basis(int Nin, int Nout, const double *V, const double *in, double *
       out[((ko * b + jo) * b + io) * 3 + c] += V[ko * a + k] * b2[((k * b + jo) * b + io) * 3 + c];
 }
 
+// ---- RefMapping boundary tracking tables (ParticleBCTracking) -------------
+// Local side ids of the reference (piclas.h:204-209).
+enum { kZetaMinus = 1, kEtaMinus = 2, kXiPlus = 3, kEtaPlus = 4, kXiMinus = 5, kZetaPlus = 6 };
+
+double choose_d(int n, int k) {
+  double r = 1.0;
+  for (int q = 1; q <= k; ++q) r = r * double(n - q + 1) / double(q);
+  return r;
+}
+double ipow(double x, int k) {
+  double r = 1.0;
+  for (int q = 0; q < k; ++q) r = r * x;
+  return r;
+}
+
+// sVdm_Bezier = INVERSE(Vdm_Bezier) on the CL nodes (BuildBezierVdm,
+// basis.f90:275-358; INVERSE's closed forms for 2x2 and 3x3, mathtools.f90:44-102)
+void bezier_svdm(int ng, const double *xi, double sV[3][3]) {
+  double A[3][3];
+  for (int i = 0; i <= ng; ++i)
+    for (int j = 0; j <= ng; ++j)
+      A[i][j] = (1.0 / double(1 << ng)) * choose_d(ng, j) * ipow(xi[i] + 1.0, j) * ipow(1.0 - xi[i], ng - j);
+  if (ng == 1) {
+    const double sdet = 1.0 / (A[0][0] * A[1][1] - A[0][1] * A[1][0]);
+    sV[0][0] = (A[1][1]) * sdet; sV[0][1] = (-A[0][1]) * sdet;
+    sV[1][0] = (-A[1][0]) * sdet; sV[1][1] = (A[0][0]) * sdet;
+    return;
+  }
+  const double sdet = 1.0 / ((A[0][0] * A[1][1] - A[0][1] * A[1][0]) * A[2][2] +
+                             (A[0][1] * A[1][2] - A[0][2] * A[1][1]) * A[2][0] +
+                             (A[0][2] * A[1][0] - A[0][0] * A[1][2]) * A[2][1]);
+  sV[0][0] = (A[1][1] * A[2][2] - A[1][2] * A[2][1]) * sdet;
+  sV[0][1] = (A[0][2] * A[2][1] - A[0][1] * A[2][2]) * sdet;
+  sV[0][2] = (A[0][1] * A[1][2] - A[0][2] * A[1][1]) * sdet;
+  sV[1][0] = (A[1][2] * A[2][0] - A[1][0] * A[2][2]) * sdet;
+  sV[1][1] = (A[0][0] * A[2][2] - A[0][2] * A[2][0]) * sdet;
+  sV[1][2] = (A[0][2] * A[1][0] - A[0][0] * A[1][2]) * sdet;
+  sV[2][0] = (A[1][0] * A[2][1] - A[1][1] * A[2][0]) * sdet;
+  sV[2][1] = (A[0][1] * A[2][0] - A[0][0] * A[2][1]) * sdet;
+  sV[2][2] = (A[0][0] * A[1][1] - A[0][1] * A[1][0]) * sdet;
+}
+
+// the CL points of local side ls, tmp[a][b] over the two free volume indices
+void side_slice(const double *X, int g1, int ls, double tmp[3][3][3]) {
+  const int ng = g1 - 1;
+  for (int a = 0; a < g1; ++a)
+    for (int b = 0; b < g1; ++b) {
+      int i, j, k;
+      switch (ls) {
+        case kXiMinus: i = 0; j = a; k = b; break;
+        case kXiPlus: i = ng; j = a; k = b; break;
+        case kEtaMinus: i = a; j = 0; k = b; break;
+        case kEtaPlus: i = a; j = ng; k = b; break;
+        case kZetaMinus: i = a; j = b; k = 0; break;
+        default: i = a; j = b; k = ng; break;
+      }
+      for (int c = 0; c < 3; ++c) tmp[a][b][c] = X[((k * g1 + j) * g1 + i) * 3 + c];
+    }
+}
+
+// BezierControlPoints3D of local side ls (CalcBezierControlPoints,
+// particle_mesh_tools.f90:1718-1741): the slice in the Bernstein basis
+// (ChangeBasis2D, first index then second), in the CGNS side orientation
+// (CGNS_SideToVol2, mesh/mapping.f90:320-356)
+void side_bezier(const double *X, int ng, int ls, const double sV[3][3], double out[3][3][3]) {
+  const int g1 = ng + 1;
+  double tmp[3][3][3], buf[3][3][3], t2[3][3][3];
+  side_slice(X, g1, ls, tmp);
+  for (int jn = 0; jn < g1; ++jn)
+    for (int io = 0; io < g1; ++io) {
+      double v[3] = {0, 0, 0};
+      for (int in = 0; in < g1; ++in)
+        for (int c = 0; c < 3; ++c) v[c] = v[c] + sV[io][in] * tmp[in][jn][c];
+      for (int c = 0; c < 3; ++c) buf[io][jn][c] = v[c];
+    }
+  for (int jo = 0; jo < g1; ++jo)
+    for (int io = 0; io < g1; ++io) {
+      double v[3] = {0, 0, 0};
+      for (int jn = 0; jn < g1; ++jn)
+        for (int c = 0; c < 3; ++c) v[c] = v[c] + sV[jo][jn] * buf[io][jn][c];
+      for (int c = 0; c < 3; ++c) t2[io][jo][c] = v[c];
+    }
+  for (int q = 0; q < g1; ++q)
+    for (int p = 0; p < g1; ++p) {
+      int a = p, b = q;
+      if (ls == kXiMinus || ls == kZetaMinus) { a = q; b = p; }
+      else if (ls == kEtaPlus) { a = ng - p; b = q; }
+      for (int c = 0; c < 3; ++c) out[p][q][c] = t2[a][b][c];
+    }
+}
+
+double dot3(const double a[3], const double b[3]) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+void unit3(double v[3]) {
+  const double l = std::sqrt(dot3(v, v));
+  v[0] = v[0] / l; v[1] = v[1] / l; v[2] = v[2] / l;
+}
+void crossnorm3(const double a[3], const double b[3], double c[3]) {
+  c[0] = a[1] * b[2] - a[2] * b[1];
+  c[1] = a[2] * b[0] - a[0] * b[2];
+  c[2] = a[0] * b[1] - a[1] * b[0];
+  unit3(c);
+}
+
+// GetSideSlabNormalsAndIntervals (particle_surfaces.f90:619-816); false for a
+// degenerated side (the reference aborts)
+bool side_slab(int ng, double B[3][3][3], double N[3][3], double I[6], uint8_t &empty) {
+  double n2[3];
+  for (int c = 0; c < 3; ++c) N[0][c] = B[ng][0][c] - B[0][0][c] + B[ng][ng][c] - B[0][ng][c];
+  if (N[0][0] == 0 && N[0][1] == 0 && N[0][2] == 0) return false;
+  const double l0 = std::sqrt(dot3(N[0], N[0]));
+  for (int c = 0; c < 3; ++c) N[0][c] = N[0][c] / l0;
+  for (int c = 0; c < 3; ++c) n2[c] = B[0][ng][c] - B[0][0][c] + B[ng][ng][c] - B[ng][0][c];
+  crossnorm3(N[0], n2, N[1]);
+  crossnorm3(N[1], N[0], N[2]);
+  for (int c = 0; c < 6; ++c) I[c] = 0.0;
+  for (int q = 0; q <= ng; ++q)
+    for (int p = 0; p <= ng; ++p) {
+      if (p == 0 && q == 0) continue;
+      const double d[3] = {B[p][q][0] - B[0][0][0], B[p][q][1] - B[0][0][1], B[p][q][2] - B[0][0][2]};
+      for (int a = 0; a < 3; ++a) {
+        const double sp = dot3(d, N[a]);
+        if (sp < 0.) I[2 * a] = std::fmin(I[2 * a], sp);
+        else if (sp > 0.) I[2 * a + 1] = std::fmax(I[2 * a + 1], sp);
+      }
+    }
+  const double dx = std::fabs(I[1] - I[0]), dy = std::fabs(I[3] - I[2]), dz = std::fabs(I[5] - I[4]);
+  double h = dy;
+  const double w = std::fmin(dx, dz), l = std::fmax(dx, dz);
+  const double dMax = std::fmax(std::fmax(h, l), w);
+  if (l / dMax < 1.0e-6 || w / dMax < 1.0e-6) return false;
+  if (h / dMax < 1.0e-6) { I[2] = 0.; I[3] = 0.; h = 0.; }
+  empty = std::fabs(h / std::sqrt(l * l + w * w + h * h)) <= 2.22e-16;
+  return true;
+}
+
+// Boundary side tables and the element -> boundary side lists for
+// ParticleBCTracking, built when not every boundary is periodic
+// (CartesianPeriodic = F): SideType / SideNormVec / SideDistance
+// (IdentifyElemAndSideType, particle_mesh_tools.f90:976-1155), BaseVectors
+// and BCSideMetrics (particle_mesh_build.f90:1760-1930), ElemToBCSides and
+// SideBCMetrics sorted by distance (BuildBCElemDistance, non-fullMesh
+// branch, particle_mesh_build.f90:876-1080) with BC_halo_eps = twice the
+// largest ElemRadiusNGeo.
+std::string build_bc_tables(const pic_mesh_desc &m, HostMesh &hm) {
+  const int ne = hm.nElems, ng = hm.NGeo, g1 = ng + 1, nGP = hm.nGP;
+  auto SI = [&](int s, int c) { return m.SideInfo[size_t(s) * 5 + c]; };
+  hm.cartPeriodic = hm.nPV > 0;
+  int nb = 0;
+  for (int sd = 0; sd < m.nSides; ++sd) {
+    const int bc = SI(sd, 4);
+    if (bc <= 0) continue;
+    ++nb;
+    if (m.PartBoundCond[bc - 1] != PIC_BC_PERIODIC) hm.cartPeriodic = false;
+  }
+  if (hm.cartPeriodic || nb == 0) return "";
+  hm.nBC = nb;
+  hm.bcElemNb.assign(nb, -1);
+  hm.bcKind.assign(nb, 0);
+  hm.bcPV.assign(nb, 0);
+  hm.bcFlip.assign(nb, 0);
+  hm.bz.assign(size_t(nb) * g1 * g1 * 3, 0.0);
+  hm.sideType.assign(nb, 0);
+  hm.bbEmpty.assign(nb, 0);
+  hm.sideNorm.assign(size_t(nb) * 3, 0.0);
+  hm.sideDist.assign(nb, 0.0);
+  hm.slabN.assign(size_t(nb) * 9, 0.0);
+  hm.slabI.assign(size_t(nb) * 6, 0.0);
+  hm.baseV.assign(size_t(nb) * 9, 0.0);
+  std::vector<double> origin(size_t(nb) * 3), bradius(nb);
+  std::vector<int32_t> bcElem(nb), sideToBC(m.nSides, -1), sideElem(m.nSides, 0), sideLocal(m.nSides, 0);
+  for (int e = 0; e < ne; ++e)
+    for (int l = 0; l < 6; ++l) {
+      const int sd = m.ElemInfo[size_t(e) * 6 + 2] + l;
+      sideElem[sd] = e;
+      sideLocal[sd] = l + 1;
+    }
+  double sV[3][3];
+  bezier_svdm(ng, hm.xiCL, sV);
+  double V1[8 * 2];
+  {
+    const double x1[2] = {-1.0, 1.0};
+    double w1[2];
+    bary_weights(1, x1, w1);
+    for (int io = 0; io <= ng; ++io) lagrange(hm.xiCL[io], 1, x1, w1, &V1[io * 2]);
+  }
+  std::vector<double> X1(24), Xn(size_t(nGP) * 3);
+  int b = 0;
+  for (int sd = 0; sd < m.nSides; ++sd) {
+    const int bcid = SI(sd, 4);
+    if (bcid <= 0) continue;
+    sideToBC[sd] = b;
+    const int e = sideElem[sd], ls = sideLocal[sd];
+    bcElem[b] = e;
+    hm.bcElemNb[b] = SI(sd, 2) - 1;
+    hm.bcKind[b] = uint8_t(m.PartBoundCond[bcid - 1]);
+    hm.bcPV[b] = int8_t(m.BCType[(bcid - 1) * 4 + 3]);
+    const int flip = SI(sd, 1) > 0 ? 0 : SI(sd, 3) % 10;
+    hm.bcFlip[b] = uint8_t(flip);
+    const double *X = &hm.XCL[size_t(e) * nGP * 3];
+    double B[3][3][3];
+    side_bezier(X, ng, ls, sV, B);
+    for (int p = 0; p < g1; ++p) for (int q = 0; q < g1; ++q) for (int c = 0; c < 3; ++c)
+      hm.bz[((size_t(b) * g1 + p) * g1 + q) * 3 + c] = B[p][q][c];
+    double N[3][3], I[6];
+    uint8_t empty = 0;
+    if (!side_slab(ng, B, N, I, empty)) return "boundary side " + std::to_string(sd + 1) + ": degenerated side slab";
+    for (int a = 0; a < 3; ++a) for (int c = 0; c < 3; ++c) hm.slabN[size_t(b) * 9 + a * 3 + c] = N[a][c];
+    for (int c = 0; c < 6; ++c) hm.slabI[size_t(b) * 6 + c] = I[c];
+    hm.bbEmpty[b] = empty;
+    double *nv = &hm.sideNorm[size_t(b) * 3];
+    double v1[3], v2[3];
+    for (int c = 0; c < 3; ++c) {
+      v1[c] = -B[0][0][c] + B[ng][0][c] - B[0][ng][c] + B[ng][ng][c];
+      v2[c] = -B[0][0][c] - B[ng][0][c] + B[0][ng][c] + B[ng][ng][c];
+    }
+    auto orient = [&]() {   // outward for flip = 0, inward otherwise; SideDistance
+      crossnorm3(v1, v2, nv);
+      double c4[3];
+      for (int c = 0; c < 3; ++c) c4[c] = 0.25 * (B[0][0][c] + B[ng][0][c] + B[0][ng][c] + B[ng][ng][c]);
+      const double d[3] = {c4[0] - hm.bary[e * 3], c4[1] - hm.bary[e * 3 + 1], c4[2] - hm.bary[e * 3 + 2]};
+      if (flip == 0) { if (dot3(d, nv) < 0) for (int c = 0; c < 3; ++c) nv[c] = -nv[c]; }
+      else if (dot3(d, nv) > 0) for (int c = 0; c < 3; ++c) nv[c] = -nv[c];
+      hm.sideDist[b] = dot3(c4, nv);
+    };
+    int type = -1;
+    bool planar = false, signedTol = false;
+    if (!hm.curved[e]) {
+      if (empty) planar = true;
+      else type = kSideBilinear;
+    } else {
+      for (int t = 0; t < 8; ++t) {
+        const int i = (t & 1) * ng, j = ((t >> 1) & 1) * ng, k = ((t >> 2) & 1) * ng;
+        for (int c = 0; c < 3; ++c) X1[t * 3 + c] = X[((k * g1 + j) * g1 + i) * 3 + c];
+      }
+      change_basis(1, ng, V1, X1.data(), Xn.data());
+      double So[3][3][3], Sn[3][3][3];
+      side_slice(X, g1, ls, So);
+      side_slice(Xn.data(), g1, ls, Sn);
+      bool curvedSide = false;
+      for (int a = 0; a < g1 && !curvedSide; ++a)
+        for (int bb = 0; bb < g1 && !curvedSide; ++bb)
+          if (std::fabs(Sn[a][bb][0] - So[a][bb][0]) > 1e-14 || std::fabs(Sn[a][bb][1] - So[a][bb][1]) > 1e-14 ||
+              std::fabs(Sn[a][bb][2] - So[a][bb][2]) > 1e-14)
+            curvedSide = true;
+      if (curvedSide) {
+        if (empty) { type = kSidePlanarCurved; orient(); }
+        else type = kSideCurved;
+      } else if (empty) {
+        planar = true;
+        signedTol = true;
+      } else {
+        type = kSideBilinear;
+      }
+    }
+    if (planar) {
+      orient();
+      double w1[3], w2[3], w3[3];
+      for (int c = 0; c < 3; ++c) {
+        w1[c] = B[0][ng][c] - B[0][0][c];
+        w2[c] = B[ng][0][c] - B[0][0][c];
+        w3[c] = B[ng][ng][c] - B[0][ng][c];
+      }
+      unit3(w1); unit3(w2); unit3(w3);
+      // flat sides of curved elements test the signed product against 1e-14
+      auto notOrtho = [&](const double *a, const double *c) {
+        return signedTol ? (dot3(a, c) > 1e-14) : !(std::fabs(dot3(a, c)) <= 2.22e-16);
+      };
+      bool rect = !notOrtho(w1, w2) && !notOrtho(w1, w3);
+      if (rect) {
+        for (int c = 0; c < 3; ++c) w1[c] = B[ng][ng][c] - B[ng][0][c];
+        unit3(w1);
+        rect = !notOrtho(w1, w2) && !notOrtho(w1, w3);
+      }
+      type = rect ? kSidePlanarRect : kSidePlanarNonRect;
+    }
+    if (type == kSideBilinear || type == kSidePlanarNonRect)
+      return "RefMapping with non-periodic boundaries: bilinear or non-rectangular planar boundary side " +
+             std::to_string(sd + 1) + " is not supported";
+    hm.sideType[b] = uint8_t(type);
+    double *bv = &hm.baseV[size_t(b) * 9];
+    for (int c = 0; c < 3; ++c) {
+      bv[c] = (+B[0][0][c] + B[ng][0][c] + B[0][ng][c] + B[ng][ng][c]);
+      bv[3 + c] = (-B[0][0][c] + B[ng][0][c] - B[0][ng][c] + B[ng][ng][c]);
+      bv[6 + c] = (-B[0][0][c] - B[ng][0][c] + B[0][ng][c] + B[ng][ng][c]);
+    }
+    // BCSideMetrics: DeCasteljauInterpolation at xi = eta = 0 (basis.f90:429-476), radius
+    double R[3][3][3];
+    std::copy(&B[0][0][0], &B[0][0][0] + 27, &R[0][0][0]);
+    const double Xi = 0.5, Eta = 0.5, MXi = 1.0 - Xi, MEta = 1.0 - Eta;
+    int l = ng - 1;
+    for (int it = 1; it <= ng; ++it) {
+      for (int q = 0; q <= l; ++q)
+        for (int p = 0; p <= l; ++p)
+          for (int c = 0; c < 3; ++c)
+            R[p][q][c] = MXi * R[p][q][c] * MEta + MXi * R[p][q + 1][c] * Eta + Xi * R[p + 1][q][c] * MEta +
+                         Xi * R[p + 1][q + 1][c] * Eta;
+      l = l - 1;
+    }
+    for (int c = 0; c < 3; ++c) origin[b * 3 + c] = R[0][0][c];
+    double rmax = 0.;
+    for (int q = 0; q <= ng; ++q)
+      for (int p = 0; p <= ng; ++p) {
+        const double d[3] = {B[p][q][0] - origin[b * 3], B[p][q][1] - origin[b * 3 + 1], B[p][q][2] - origin[b * 3 + 2]};
+        rmax = std::fmax(rmax, dot3(d, d));
+      }
+    bradius[b] = std::sqrt(rmax);
+    ++b;
+  }
+  double rmaxE = 0.0;
+  for (int e = 0; e < ne; ++e) rmaxE = std::fmax(rmaxE, hm.radius[e]);
+  hm.bcHalo = 2.0 * rmaxE;
+  hm.e2bOff.assign(ne + 1, 0);
+  hm.e2bSide.clear();
+  hm.e2bDist.clear();
+  std::vector<int32_t> lst;
+  for (int e = 0; e < ne; ++e) {
+    lst.clear();
+    for (int l = 0; l < 6; ++l) {
+      const int sd = m.ElemInfo[size_t(e) * 6 + 2] + l;
+      if (SI(sd, 4) > 0) lst.push_back(sideToBC[sd]);
+    }
+    for (int q = 0; q < nb; ++q) {
+      const int be = bcElem[q];
+      if (be == e) continue;
+      const double d[3] = {hm.bary[e * 3] - hm.bary[be * 3], hm.bary[e * 3 + 1] - hm.bary[be * 3 + 1],
+                           hm.bary[e * 3 + 2] - hm.bary[be * 3 + 2]};
+      if (std::sqrt(dot3(d, d)) > hm.bcHalo + hm.radius[e] + hm.radius[be]) continue;
+      const double d2[3] = {hm.bary[e * 3] - origin[q * 3], hm.bary[e * 3 + 1] - origin[q * 3 + 1],
+                            hm.bary[e * 3 + 2] - origin[q * 3 + 2]};
+      if (std::sqrt(dot3(d2, d2)) <= hm.bcHalo + hm.radius[e] + bradius[q]) lst.push_back(q);
+    }
+    const size_t base = hm.e2bSide.size();
+    hm.e2bOff[e] = int32_t(base);
+    for (int32_t bs : lst) {
+      const double d[3] = {hm.bary[e * 3] - origin[bs * 3], hm.bary[e * 3 + 1] - origin[bs * 3 + 1],
+                           hm.bary[e * 3 + 2] - origin[bs * 3 + 2]};
+      hm.e2bSide.push_back(bs);
+      hm.e2bDist.push_back(std::sqrt(dot3(d, d)) - hm.radius[e] - bradius[bs]);
+    }
+    for (size_t a = 1; a < lst.size(); ++a) {   // InsertionSort by distance (stable)
+      const double dv = hm.e2bDist[base + a];
+      const int32_t sv = hm.e2bSide[base + a];
+      long j = long(a) - 1;
+      while (j >= 0 && hm.e2bDist[base + j] > dv) {
+        hm.e2bDist[base + j + 1] = hm.e2bDist[base + j];
+        hm.e2bSide[base + j + 1] = hm.e2bSide[base + j];
+        --j;
+      }
+      hm.e2bDist[base + j + 1] = dv;
+      hm.e2bSide[base + j + 1] = sv;
+    }
+  }
+  hm.e2bOff[ne] = int32_t(hm.e2bSide.size());
+  // the normal slab of each listed side: unit normal n (SideSlabNormals(:,2))
+  // and the interval of n . X that holds the patch (its control points'
+  // interval, SideSlabIntervals 3:4, widened by 1e-3 of the side's extent):
+  // the device's exact no-hit test for a step
+  hm.e2bSlab.resize(hm.e2bSide.size() * 6);
+  for (size_t k = 0; k < hm.e2bSide.size(); ++k) {
+    const int bs = hm.e2bSide[k];
+    const double *N1 = &hm.slabN[size_t(bs) * 9 + 3], *I = &hm.slabI[size_t(bs) * 6];
+    const double *B0 = &hm.bz[size_t(bs) * g1 * g1 * 3];
+    const double c = dot3(N1, B0), mrg = 1e-3 * std::fmax(I[1] - I[0], I[5] - I[4]);
+    double *r = &hm.e2bSlab[k * 6];
+    r[0] = N1[0]; r[1] = N1[1]; r[2] = N1[2];
+    r[3] = c + I[2] - mrg;
+    r[4] = c + I[3] + mrg;
+    r[5] = 0.0;
+  }
+  return "";
+}
+
 struct UnionFind {
   std::vector<int32_t> p;
   explicit UnionFind(int n) : p(n) { std::iota(p.begin(), p.end(), 0); }
@@ -510,6 +882,8 @@ std::string build_host_mesh(const pic_mesh_desc &m, const pic_params &par, HostM
     for (size_t q = 0; q < nbins; ++q)
       if (hm.binOff[q + 1] - hm.binOff[q] > kMaxBinElems)
         return "RefMapping: a background cell holds more than " + std::to_string(kMaxBinElems) + " elements";
+    std::string err = build_bc_tables(m, hm);
+    if (!err.empty()) return err;
   }
 
   if (!refmap) {

@@ -4,6 +4,7 @@
 #pragma once
 
 #include "pic_device.h"
+#include "pic_bctrack.h"
 
 namespace pic {
 
@@ -295,28 +296,19 @@ __device__ inline int next_candidate(const DevMesh &M, int b0, int n, const doub
   return bq;
 }
 
-enum RmStatus { kRmInside = 0, kRmRemoved = 1, kRmNotFound = 2 };
+enum RmStatus { kRmInside = 0, kRmRemoved = 1, kRmNotFound = 2, kRmDeferred = 3 };
+// first part, in the last element e0 (ParticleRefTracking :87-161): whether
+// the particle is still inside e0 (xi: PartPosRef there, else the mapping in
+// e0 for the candidate search), and whether it is a boundary candidate: a
+// particle of a boundary element whose step reaches the nearest boundary
+// side in range, for which ParticleBCTracking can find an intersection
+// (decided by bc_may_hit before rm_bc_track runs)
+struct RmHead {
+  bool inside, cand;
+};
 
-// ParticleRefTracking (particles/tracking/particle_reftracking.f90:30-414) for
-// CartesianPeriodic = T and open walls.  The reference sends particles of
-// boundary elements through ParticleBCTracking (trajectory against the
-// boundary faces); here a particle that no element holds after the move and
-// whose last element has an open side has left through that wall.  Both
-// decide alike unless a trajectory leaves and re-enters within one step.
-// Returns the new owner, or -1 with status kRmRemoved / kRmNotFound (where
-// the reference aborts).  xi: PartPosRef in and out.
-// first part: periodic movement and the mapping in the last element
-template <int N, int G>
-__device__ inline bool rm_track_head(const DevMesh &M, int e0, double x[3], double last[3], double xi[3]) {
-  if (M.nPV > 0)
-    for (int g = 0; g < 4; ++g)
-      if (!periodic_movement(M, x, last)) break;
-  rm_position<N, G>(M, e0, x, xi);
-  return max_abs3(xi) < 1.0;
-}
-
-// rm_position / rm_track_head for a wave whose lanes all map in element e
-// (e wave-uniform): the Newton reads the map by scalar loads
+// rm_position for a wave whose lanes all map in element e (e wave-uniform):
+// the Newton reads the map by scalar loads
 template <int N, int G>
 __device__ inline void rm_position_s(const DevMesh &M, int e, const double x[3], double xi[3]) {
   rm_start<N, G, true>(M, e, x, xi);
@@ -328,18 +320,60 @@ __device__ inline void rm_position_s(const DevMesh &M, int e, const double x[3],
 }
 
 template <int N, int G>
-__device__ inline bool rm_track_head_s(const DevMesh &M, int e0, double x[3], double last[3], double xi[3]) {
-  if (M.nPV > 0)
+__device__ inline int rm_locate(const DevMesh &M, const double x[3], double xi[3]);
+// PeriodicMovement (CartesianPeriodic meshes, which have no boundary
+// tables), or for a particle of a boundary element the reference's early
+// return when it did not move (PartPosRef kept, done) and the candidate
+// test; then the mapping in e0.  Nothing of the step but x and xi is live
+// across the Newton iteration.
+template <int N, int G, bool UNI = false>
+__device__ inline RmHead rm_track_head(const DevMesh &M, int e0, double x[3], double last[3], double xi[3]) {
+  RmHead h = {false, false};
+  if (M.cartPeriodic && M.nPV > 0) {
     for (int g = 0; g < 4; ++g)
       if (!periodic_movement(M, x, last)) break;
-  rm_position_s<N, G>(M, e0, x, xi);
-  return max_abs3(xi) < 1.0;
+  } else if (bc_elem(M, e0)) {
+    const double d0 = x[0] - last[0], d1 = x[1] - last[1], d2 = x[2] - last[2];
+    const double len = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+    if (bz_almost_zero(len / M.radius[e0])) {   // PARTHASMOVED: PartPosRef kept
+      h.inside = true;
+      return h;
+    }
+    // Of the sides in range (SideBCMetrics distance <= the step), the
+    // intersection routines can report a hit only where the step reaches
+    // the side's normal slab: the patch lies in the hull of its control
+    // points, and the clip and Newton tolerances accept points at most
+    // ~1e-6 of the side's extent outside it, far inside the slab's 1e-3
+    // margin.  A step that reaches no slab hits nothing, so the boundary
+    // tracking would leave it as it is; only the others are candidates.
+    const int first = M.e2bOff[e0], nloc = M.e2bOff[e0 + 1] - first;
+    for (int k = 0; k < nloc; ++k) {
+      if (M.e2bDist[first + k] > len) break;
+      const double *sl = M.e2bSlab + size_t(first + k) * 6;
+      const double a = sl[0] * last[0] + sl[1] * last[1] + sl[2] * last[2];
+      const double c = sl[0] * x[0] + sl[1] * x[1] + sl[2] * x[2];
+      if (fmax(a, c) >= sl[3] && fmin(a, c) <= sl[4]) {
+        h.cand = true;
+        break;
+      }
+    }
+  }
+  // (e0 may sit in a vector register; the scalar loads need it wave-uniform)
+  if (UNI) rm_position_s<N, G>(M, __builtin_amdgcn_readfirstlane(e0), x, xi);
+  else rm_position<N, G>(M, e0, x, xi);
+  h.inside = max_abs3(xi) < 1.0;
+  return h;
 }
 
-// second part, for particles that left e0: the candidate search; xi holds
-// the mapping in e0 on entry (oldXi)
-template <int N, int G>
-__device__ inline int rm_search(const DevMesh &M, int e0, const double x[3], double xi[3], int &status) {
+// second part, for particles that left: the candidate search (:167-324)
+// around FIBGM cell of x, excluding LastElemID e0; xi holds oldXi on entry.
+// Not found within ElemEpsOneCell in a boundary element: the fallback
+// (:325-405) with FallBackFaceIntersection, ParticleBCTracking again and
+// LocateParticleInElement.  NOBC: a particle that needs that fallback is
+// left as it was (status kRmDeferred, x and last untouched) for a kernel
+// that holds the boundary code.
+template <int N, int G, bool NOBC = false>
+__device__ inline int rm_search(const DevMesh &M, int e0, double x[3], double last[3], double xi[3], int &status) {
   status = kRmInside;
   // FIBGM cell (:180-186)
   int b3[3];
@@ -372,10 +406,6 @@ __device__ inline int rm_search(const DevMesh &M, int e0, const double x[3], dou
       newElem = e;
     }
   }
-  if (M.openElem[e0]) {
-    status = kRmRemoved;
-    return -1;
-  }
   int test;
   if (max_abs3(oldXi) < max_abs3(newXi)) {
     xi[0] = oldXi[0]; xi[1] = oldXi[1]; xi[2] = oldXi[2];
@@ -384,18 +414,71 @@ __device__ inline int rm_search(const DevMesh &M, int e0, const double x[3], dou
     xi[0] = newXi[0]; xi[1] = newXi[1]; xi[2] = newXi[2];
     test = newElem;
   }
-  if (test < 0 || max_abs3(xi) > M.epsOneCell[test]) {
+  if (test >= 0 && max_abs3(xi) <= M.epsOneCell[test]) return test;
+  if (test < 0 || !bc_elem(M, test)) {   // Tolerance Issue with internal element: the reference aborts
     status = kRmNotFound;
     return -1;
+  }
+  if (NOBC) {
+    status = kRmDeferred;
+    return -1;
+  }
+  if (M.NGeo > 1) {
+    const int rc = bc_fallback(M, test, x, last);
+    if (rc == 1) {   // no face hit: LocateParticleInElement; not found aborts
+      const int e = rm_locate<N, G>(M, x, xi);
+      if (e < 0) status = kRmNotFound;
+      return e;
+    }
+    if (rc != kBcOk) { status = kRmNotFound; return -1; }
+  }
+  bool done, removed;
+  if (bc_tracking(M, test, x, last, done, removed) != kBcOk) { status = kRmNotFound; return -1; }
+  if (removed) { status = kRmRemoved; return -1; }
+  if (done) return test;
+  rm_position<N, G>(M, test, x, xi);
+  if (max_abs3(xi) > M.epsOneCell[test]) {   // Tolerance Issue with BC element, relocating
+    const int e = rm_locate<N, G>(M, x, xi);
+    if (e < 0) status = kRmNotFound;
+    return e;
   }
   return test;
 }
 
+// ParticleBCTracking, the mapping in the element it ends in and, if that
+// fails, the candidate search, for a particle of boundary element e0
+template <int N, int G>
+__device__ inline int rm_bc_track(const DevMesh &M, int e0, double x[3], double last[3], double xi[3], int &status) {
+  status = kRmInside;
+  int e = e0;
+  bool done, removed;
+  if (bc_tracking(M, e, x, last, done, removed) != kBcOk) { status = kRmNotFound; return -1; }
+  if (removed) { status = kRmRemoved; return -1; }
+  if (done) return e;
+  rm_position<N, G>(M, e, x, xi);
+  if (max_abs3(xi) < 1.0) return e;
+  return rm_search<N, G>(M, e0, x, last, xi, status);
+}
+
+// the rest of ParticleRefTracking after the head (xi: the head's mapping in
+// e0); FILTERED: the candidate bit already holds the outcome of bc_may_hit
+template <int N, int G, bool FILTERED = false>
+__device__ inline int rm_track_tail(const DevMesh &M, int e0, RmHead h, double x[3], double last[3], double xi[3],
+                                    int &status) {
+  status = kRmInside;
+  if (h.cand) {
+    if (FILTERED) return rm_bc_track<N, G>(M, e0, x, last, xi, status);
+    const double d0 = x[0] - last[0], d1 = x[1] - last[1], d2 = x[2] - last[2];
+    if (bc_may_hit(M, e0, x, last, sqrt(d0 * d0 + d1 * d1 + d2 * d2))) return rm_bc_track<N, G>(M, e0, x, last, xi, status);
+  }
+  if (h.inside) return e0;
+  return rm_search<N, G>(M, e0, x, last, xi, status);
+}
+
 template <int N, int G>
 __device__ inline int rm_track(const DevMesh &M, int e0, double x[3], double last[3], double xi[3], int &status) {
-  status = kRmInside;
-  if (rm_track_head<N, G>(M, e0, x, last, xi)) return e0;
-  return rm_search<N, G>(M, e0, x, xi, status);
+  const RmHead h = rm_track_head<N, G>(M, e0, x, last, xi);
+  return rm_track_tail<N, G>(M, e0, h, x, last, xi, status);
 }
 
 // SinglePointToElement for RefMapping (particles/tracking/particle_localization.f90:81-203)

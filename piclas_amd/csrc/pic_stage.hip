@@ -172,6 +172,8 @@ struct Movers {
   int32_t *count;   // [1]
   int32_t *idx;     // [cap]
   double *last;     // [cap][3] LastPartPos
+  uint8_t *flag;    // [cap] RefMapping: bit 0 boundary candidate, bit 1 still inside its element
+  int32_t *slot2;   // [cap] RefMapping: the slots that k_rm_bc_filter passes on (count at count[1])
 };
 
 template <int N, bool MIXED = false, bool AFFINE = false>
@@ -276,35 +278,7 @@ __device__ inline void rm_removed(DevParts &P, int64_t i, const double x[3], con
   if (status == kRmNotFound) atomicAdd(lostCount, 1ull);
 }
 
-// fused step: interpolate at PartPosRef + push + ParticleRefTracking
-template <int N, int G, bool MIXED = false>
-__global__ __launch_bounds__(kBlock) void k_rm_step(DevMesh M, DevParts P, int64_t n, double dt, int32_t *newElem,
-                                                    unsigned long long *lostCount, uint8_t *insideById,
-                                                    double *stateById, const int32_t *nDev) {
-  const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
-  if (nDev) n = *nDev;
-  if (i >= n) return;
-  const int e0 = P.elem[i];
-  if (e0 < 0) { newElem[i] = -1; return; }
-  double x[3] = {P.x[i], P.y[i], P.z[i]};
-  double v[3] = {P.vx[i], P.vy[i], P.vz[i]};
-  double last[3] = {x[0], x[1], x[2]};
-  double xi[3] = {P.rx[i], P.ry[i], P.rz[i]};
-  const int sp = P.spec[i];
-  double F[6] = {0, 0, 0, 0, 0, 0};
-  if (fabs(M.charge[sp]) > 0.0) field_at_ref<N, MIXED>(M, e0, xi, F);
-  boris_push(M.charge[sp], M.mass[sp], P.isNew[i] != 0, F, dt, x, v);
-  int status;
-  const int e = rm_track<N, G>(M, e0, x, last, xi, status);
-  P.x[i] = x[0]; P.y[i] = x[1]; P.z[i] = x[2];
-  P.vx[i] = v[0]; P.vy[i] = v[1]; P.vz[i] = v[2];
-  P.rx[i] = xi[0]; P.ry[i] = xi[1]; P.rz[i] = xi[2];
-  P.isNew[i] = 0;
-  newElem[i] = e;
-  if (e < 0) rm_removed(P, i, x, v, status, lostCount, insideById, stateById);
-}
-
-// Split form of k_rm_step for throughput (as for TriaTracking): kernel A
+// Interpolation, push and ParticleRefTracking in two kernels (as for TriaTracking): kernel A
 // interpolates, pushes and maps the particle in its last element; the few
 // that left it go to a list that kernel B searches, from the same state.
 template <int N, int G, bool MIXED = false>
@@ -318,12 +292,14 @@ __global__ __launch_bounds__(kBlock) void k_rm_push_inside(DevMesh M, DevParts P
   const int eAll = i < n ? P.elem[i] : -1;
   const int eu = __builtin_amdgcn_readfirstlane(eAll);
   const bool uni = eu >= 0 && __ballot(i < n && eAll != eu) == 0;
+  RmHead head = {true, false};
+  double x[3] = {0, 0, 0};
   if (i < n) {
     const int e0 = eAll;
     if (e0 < 0) {
       newElem[i] = -1;
     } else {
-      double x[3] = {P.x[i], P.y[i], P.z[i]};
+      x[0] = P.x[i]; x[1] = P.y[i]; x[2] = P.z[i];
       double v[3] = {P.vx[i], P.vy[i], P.vz[i]};
       double last[3] = {x[0], x[1], x[2]};
       double xi[3] = {P.rx[i], P.ry[i], P.rz[i]};
@@ -334,13 +310,16 @@ __global__ __launch_bounds__(kBlock) void k_rm_push_inside(DevMesh M, DevParts P
         else field_at_ref<N, MIXED>(M, e0, xi, F);
       }
       boris_push(M.charge[sp], M.mass[sp], P.isNew[i] != 0, F, dt, x, v);
-      const bool in = uni ? rm_track_head_s<N, G>(M, eu, x, last, xi) : rm_track_head<N, G>(M, e0, x, last, xi);
-      P.x[i] = x[0]; P.y[i] = x[1]; P.z[i] = x[2];
       P.vx[i] = v[0]; P.vy[i] = v[1]; P.vz[i] = v[2];
-      P.rx[i] = xi[0]; P.ry[i] = xi[1]; P.rz[i] = xi[2];
       P.isNew[i] = 0;
-      if (in) newElem[i] = e0;
-      else mover = true;
+      head = uni ? rm_track_head<N, G, true>(M, eu, x, last, xi) : rm_track_head<N, G>(M, e0, x, last, xi);
+      P.rx[i] = xi[0]; P.ry[i] = xi[1]; P.rz[i] = xi[2];
+      if (head.inside && !head.cand) {
+        newElem[i] = e0;
+        P.x[i] = x[0]; P.y[i] = x[1]; P.z[i] = x[2];
+      } else {
+        mover = true;   // x is stored after LastPartPos is read back below
+      }
     }
   }
   const unsigned long long m = __ballot(mover);
@@ -350,20 +329,107 @@ __global__ __launch_bounds__(kBlock) void k_rm_push_inside(DevMesh M, DevParts P
   int base = 0;
   if (lane == leader) base = atomicAdd(mv.count, __popcll(m));
   base = __shfl(base, leader, kWave);
-  if (mover) mv.idx[base + __popcll(m & ((1ull << lane) - 1ull))] = int32_t(i);
+  if (mover) {
+    const int slot = base + __popcll(m & ((1ull << lane) - 1ull));
+    mv.idx[slot] = int32_t(i);
+    mv.flag[slot] = uint8_t((head.cand ? 1 : 0) | (head.inside ? 2 : 0));
+    // LastPartPos: the position before the push, read back from the arrays
+    // (not held in registers across the Newton iteration; after a
+    // PeriodicMovement it is not the shifted one, which no later stage reads
+    // on CartesianPeriodic meshes)
+    mv.last[(size_t)slot * 3 + 0] = P.x[i];
+    mv.last[(size_t)slot * 3 + 1] = P.y[i];
+    mv.last[(size_t)slot * 3 + 2] = P.z[i];
+    P.x[i] = x[0]; P.y[i] = x[1]; P.z[i] = x[2];
+  }
 }
 
+// Kernel B for RefMapping: the particles that left their element, and those
+// of boundary elements whose step reaches a boundary side's range.  The
+// boundary test (bc_may_hit: the early exits of the face intersections of
+// every boundary side in range) and the candidate search run here; a
+// particle whose step may hit a boundary side, or that needs the fallback
+// of the search, is left untouched for k_rm_bc_movers, which holds the
+// intersection code (its registers and stack would slow the many movers
+// for the few).
 template <int N, int G>
 __global__ __launch_bounds__(kBlock) void k_rm_movers(DevMesh M, DevParts P, int32_t *newElem, Movers mv,
                                                       unsigned long long *lostCount, uint8_t *insideById,
                                                       double *stateById) {
   const int cnt = *mv.count;
-  for (int k = blockIdx.x * kBlock + threadIdx.x; k < cnt; k += gridDim.x * kBlock) {
+  const int lane = lane_id();
+  const int stride = gridDim.x * kBlock;
+  // wave-uniform trip count: every lane takes part in the ballots
+  for (int k0 = blockIdx.x * kBlock + (threadIdx.x & ~(kWave - 1)); k0 < cnt; k0 += stride) {
+    const int k = k0 + lane;
+    bool defer = false;
+    if (k < cnt) {
+      const int i = mv.idx[k];
+      uint8_t f = mv.flag[k];
+      const int e0 = P.elem[i];
+      double x[3] = {P.x[i], P.y[i], P.z[i]};
+      double last[3] = {mv.last[(size_t)k * 3], mv.last[(size_t)k * 3 + 1], mv.last[(size_t)k * 3 + 2]};
+      bool done = false;
+      if (f & 1) {
+        const double d0 = x[0] - last[0], d1 = x[1] - last[1], d2 = x[2] - last[2];
+        if (bc_may_hit(M, e0, x, last, sqrt(d0 * d0 + d1 * d1 + d2 * d2))) {
+          defer = true;
+        } else if (f & 2) {
+          newElem[i] = e0;
+          done = true;
+        } else {
+          f = uint8_t(f & ~1);
+          mv.flag[k] = f;
+        }
+      } else if (f & 2) {   // (not a mover: kept for completeness)
+        newElem[i] = e0;
+        done = true;
+      }
+      if (!defer && !done) {
+        double xi[3] = {P.rx[i], P.ry[i], P.rz[i]};
+        int status;
+        const int e = rm_search<N, G, true>(M, e0, x, last, xi, status);
+        if (status == kRmDeferred) {
+          defer = true;
+        } else {
+          P.rx[i] = xi[0]; P.ry[i] = xi[1]; P.rz[i] = xi[2];
+          newElem[i] = e;
+          if (e < 0) {
+            const double v[3] = {P.vx[i], P.vy[i], P.vz[i]};
+            rm_removed(P, i, x, v, status, lostCount, insideById, stateById);
+          }
+        }
+      }
+    }
+    const unsigned long long m = __ballot(defer);
+    if (m == 0) continue;
+    const int leader = __ffsll((long long)m) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(mv.count + 1, __popcll(m));
+    base = __shfl(base, leader, kWave);
+    if (defer) mv.slot2[base + __popcll(m & ((1ull << lane) - 1ull))] = k;
+  }
+}
+
+// Kernel C for RefMapping: ParticleBCTracking for the movers whose step may
+// hit a boundary side, and the search's fallback, from the state kernel A
+// left (the candidate bit holds the outcome of bc_may_hit)
+template <int N, int G>
+__global__ __launch_bounds__(kBlock) void k_rm_bc_movers(DevMesh M, DevParts P, int32_t *newElem, Movers mv,
+                                                         unsigned long long *lostCount, uint8_t *insideById,
+                                                         double *stateById) {
+  const int cnt = mv.count[1];
+  for (int j = blockIdx.x * kBlock + threadIdx.x; j < cnt; j += gridDim.x * kBlock) {
+    const int k = mv.slot2[j];
     const int i = mv.idx[k];
-    const double x[3] = {P.x[i], P.y[i], P.z[i]};
+    const uint8_t f = mv.flag[k];
+    double x[3] = {P.x[i], P.y[i], P.z[i]};
+    double last[3] = {mv.last[(size_t)k * 3], mv.last[(size_t)k * 3 + 1], mv.last[(size_t)k * 3 + 2]};
     double xi[3] = {P.rx[i], P.ry[i], P.rz[i]};
     int status;
-    const int e = rm_search<N, G>(M, P.elem[i], x, xi, status);
+    const RmHead h = {(f & 2) != 0, (f & 1) != 0};
+    const int e = rm_track_tail<N, G, true>(M, P.elem[i], h, x, last, xi, status);
+    P.x[i] = x[0]; P.y[i] = x[1]; P.z[i] = x[2];
     P.rx[i] = xi[0]; P.ry[i] = xi[1]; P.rz[i] = xi[2];
     newElem[i] = e;
     if (e < 0) {
@@ -741,6 +807,7 @@ struct SFArgs {
   int cartesian;        // every element's Gauss points form an axis-aligned tensor grid
   int alpha;
   int linear;   // NGeo == 1: the barycentre cull is implied by the DOF-box test
+  int32_t *work;   // [8] target counters of the persistent gathers (zeroed before each launch)
 };
 
 template <int N, int ALPHA>
@@ -891,21 +958,35 @@ __global__ __launch_bounds__(kBlock) void k_sf_records(DevParts P, const int32_t
   rec[2 * i + 1] = make_double4(P.vx[i] * f, P.vy[i] * f, P.vz[i] * f, f);
 }
 
+// The next target of a persistent gather wave: targets are taken in
+// order from 8 contiguous ranges, one per XCD (the XCD that runs a workgroup
+// is its block index mod 8), each by an atomic counter; a wave whose range
+// is exhausted helps in the next.  Every target is computed by one wave in
+// the same way, so the results do not depend on which wave takes it.
+__device__ inline int sf_next_target(const SFArgs &A, int nE, int &range, int lane) {
+  for (int tried = 0; tried < 8; ++tried) {
+    int v = 0;
+    if (lane == 0) v = atomicAdd(&A.work[range], 1);
+    v = __builtin_amdgcn_readfirstlane(v);
+    const int lo = int(int64_t(nE) * range / 8) + v, hi = int(int64_t(nE) * (range + 1) / 8);
+    if (lo < hi) return lo;
+    range = (range + 1) & 7;
+  }
+  return nE;
+}
+
+// Per wave a ring of 128 staged records (x, y, z relative to the target
+// barycentre, |x|^2/R^2, f vx, f vy, f vz, f): particles that pass the cull
+// are appended, and groups of 4 are consumed once 64 are pending, so groups
+// span source chunks and only the very last group is padded.
+constexpr int kRingMfma = 128;
 template <int ALPHA, bool LINEAR>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE_MX))) void k_sf_gather_mfma(DevMesh M, const int32_t *start, SFArgs A, double *PS) {
+__device__ __attribute__((always_inline)) inline void sf_gather_mfma_target(const DevMesh &M, const int32_t *start,
+                                                                            const SFArgs &A, double *PS, int t,
+                                                                            double (*L)[8], int lane) {
 #pragma clang fp contract(fast)
   constexpr int nd = 64;
-  // Per wave a ring of 128 staged records (x, y, z relative to the target
-  // barycentre, |x|^2/R^2, f vx, f vy, f vz, f): particles that pass the cull
-  // are appended, and groups of 4 are consumed once 64 are pending, so groups
-  // span source chunks and only the very last group is padded.
-  constexpr int kRing = 128;
-  __shared__ double lds[kBlock / kWave][kRing][8];
-  const int wib = threadIdx.x / kWave;
-  const int lane = lane_id();
-  const int lb = xcd_block(blockIdx.x, gridDim.x);
-  const int t = __builtin_amdgcn_readfirstlane(lb * (kBlock / kWave) + wib);
-  if (t >= M.nElems) return;
+  constexpr int kRing = kRingMfma;
   const double bx = M.bary[t * 3], by = M.bary[t * 3 + 1], bz = M.bary[t * 3 + 2];
   const double R2i0 = A.r2_sf_inv;
   // operand-side DOF constants, one DOF per quarter:  S = cd - qq + m.q
@@ -932,7 +1013,6 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE_MX))
   // the reference's element cull |x - bary| > r_sf + ElemRadius, compared squared
   const double thr = A.r_sf + M.radius[t], thr2 = thr * thr;
   const double R2box = A.r2_sf * (1.0 + 1e-12);
-  double (*L)[8] = lds[wib];
   const int kp = lane >> 4, jc = lane & 3;
   // groups read 64 consecutive ring records from a multiple of 64 (no wrap)
   auto group = [&](const double (*Lh)[8], int g) {
@@ -1025,6 +1105,26 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE_MX))
   for (int Q = 0; Q < 4; ++Q) PS[((size_t)t * nd + 16 * Q + dofLo) * 4 + jc] = acc[Q];
 }
 
+// One wave per target element (PERSIST false: target from the block index,
+// XCD-contiguous), or persistent waves that take targets from the counters
+// of sf_next_target (PERSIST true: launched at the resident capacity).
+template <int ALPHA, bool LINEAR, bool PERSIST>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE_MX))) void k_sf_gather_mfma(DevMesh M, const int32_t *start, SFArgs A, double *PS) {
+  __shared__ double lds[kBlock / kWave][kRingMfma][8];
+  const int wib = threadIdx.x / kWave;
+  const int lane = lane_id();
+  if (!PERSIST) {
+    const int t = __builtin_amdgcn_readfirstlane(xcd_block(blockIdx.x, gridDim.x) * (kBlock / kWave) + wib);
+    if (t < M.nElems) sf_gather_mfma_target<ALPHA, LINEAR>(M, start, A, PS, t, lds[wib], lane);
+    return;
+  }
+  int range = blockIdx.x & 7;
+  for (int t = sf_next_target(A, M.nElems, range, lane); t < M.nElems; t = sf_next_target(A, M.nElems, range, lane)) {
+    sf_gather_mfma_target<ALPHA, LINEAR>(M, start, A, PS, t, lds[wib], lane);
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
 // The same gather for targets whose Gauss points form an axis-aligned tensor
 // grid (Cartesian elements): with u = x/r_sf, S = 1 - (u_i - pu)^2 -
 // (v_j - pv)^2 - (w_k - pw)^2 = [K_ij + 2 u_i pu + 2 v_j pv] - C_k with the
@@ -1033,20 +1133,17 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE_MX))
 // two fused multiply-adds per lane and group of 4 particles; a pair then costs
 // a subtraction, the clamp and the square.  Lane and accumulator layout as
 // k_sf_gather_mfma, with quarter Q = k and lane%16 = i + 4j.
+// ring of 128 staged records: pu, pv, C_k for k = 0..3, f vx, f vy, f vz, f
+// (staging writes pw in the place of C_0; a pass with one record per lane
+// forms the four C_k before each batch of 64 is consumed)
+constexpr int kRingCart = 128, kRecCart = 10;
 template <int ALPHA>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE_MX))) void k_sf_gather_cart(DevMesh M, const int32_t *start, SFArgs A, double *PS) {
+__device__ __attribute__((always_inline)) inline void sf_gather_cart_target(const DevMesh &M, const int32_t *start,
+                                                                            const SFArgs &A, double *PS, int t,
+                                                                            double (*L)[kRecCart], int lane) {
 #pragma clang fp contract(fast)
   constexpr int nd = 64;
-  // ring of 128 staged records: pu, pv, C_k for k = 0..3, f vx, f vy, f vz, f
-  // (staging writes pw in the place of C_0; a pass with one record per lane
-  // forms the four C_k before each batch of 64 is consumed)
-  constexpr int kRing = 128, kRec = 10;
-  __shared__ double lds[kBlock / kWave][kRing][kRec];
-  const int wib = threadIdx.x / kWave;
-  const int lane = lane_id();
-  const int lb = xcd_block(blockIdx.x, gridDim.x);
-  const int t = __builtin_amdgcn_readfirstlane(lb * (kBlock / kWave) + wib);
-  if (t >= M.nElems) return;
+  constexpr int kRing = kRingCart, kRec = kRecCart;
   const double bx = M.bary[t * 3], by = M.bary[t * 3 + 1], bz = M.bary[t * 3 + 2];
   const double Ri = A.r_inv;
   const double *xg = M.xGPe + (size_t)t * nd * 3;
@@ -1073,7 +1170,6 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE_MX))
   const double Kij = 1.0 - ui * ui - vj * vj, tu = 2.0 * ui, tv = 2.0 * vj;
   const double box2 = 1.0 + 1e-12;
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  double (*L)[kRec] = lds[wib];
   const int kp = lane >> 4, jc = lane & 3;
   // live[Q] bit r: staged record r of the batch reaches some DOF of the
   // plane k = Q; a group of 4 records with no live bit in a plane has S = 0
@@ -1225,6 +1321,23 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE_MX))
   const int dofLo = 4 * ((lane >> 2) & 3) + (lane >> 4);
 #pragma unroll
   for (int Q = 0; Q < 4; ++Q) PS[((size_t)t * nd + 16 * Q + dofLo) * 4 + jc] = acc[Q];
+}
+
+template <int ALPHA, bool PERSIST>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE_MX))) void k_sf_gather_cart(DevMesh M, const int32_t *start, SFArgs A, double *PS) {
+  __shared__ double lds[kBlock / kWave][kRingCart][kRecCart];
+  const int wib = threadIdx.x / kWave;
+  const int lane = lane_id();
+  if (!PERSIST) {
+    const int t = __builtin_amdgcn_readfirstlane(xcd_block(blockIdx.x, gridDim.x) * (kBlock / kWave) + wib);
+    if (t < M.nElems) sf_gather_cart_target<ALPHA>(M, start, A, PS, t, lds[wib], lane);
+    return;
+  }
+  int range = blockIdx.x & 7;
+  for (int t = sf_next_target(A, M.nElems, range, lane); t < M.nElems; t = sf_next_target(A, M.nElems, range, lane)) {
+    sf_gather_cart_target<ALPHA>(M, start, A, PS, t, lds[wib], lane);
+    __builtin_amdgcn_wave_barrier();
+  }
 }
 
 // Mixed-precision shape-function gather for N = 3 on general elements: S and
@@ -1643,6 +1756,9 @@ struct pic_handle {
   bool sfMatrix = true;   // N = 3 shape function: accumulate on the matrix cores
   bool recFresh = false;  // sf.rec holds the records of the current particle arrays
   bool useGraph = false;  // replay captured two-step graphs in pic_time_step
+  bool sfPersist = false; // N = 3 matrix-core gathers: persistent waves that take targets from counters
+  int nCU = 0;            // compute units of the device
+  std::vector<std::pair<const void *, unsigned>> persistGrid;   // resident blocks per persistent kernel
   hipGraph_t graph = nullptr;
   hipGraphExec_t graphExec = nullptr;
   double graphDt = 0;
@@ -1659,6 +1775,19 @@ struct pic_handle {
 };
 
 namespace {
+
+// Blocks of a persistent kernel that the device holds at once (its
+// occupancy per CU times the CUs, a multiple of 8 so that every XCD range of
+// the target counters starts with its own blocks), found once per kernel.
+unsigned persist_grid(pic_handle *h, const void *kernel) {
+  for (auto &pg : h->persistGrid)
+    if (pg.first == kernel) return pg.second;
+  int perCU = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, kernel, kBlock, 0) != hipSuccess || perCU < 1) perCU = 1;
+  const unsigned g = std::max(8u, unsigned(perCU * h->nCU) / 8u * 8u);
+  h->persistGrid.emplace_back(kernel, g);
+  return g;
+}
 
 template <typename T>
 int dalloc(pic_handle *h, T **p, size_t count) {
@@ -1793,18 +1922,34 @@ int run_deposition(pic_handle *h, bool timed) {
         hipLaunchKernelGGL((k_sf_gather_mixed<0>), dim3(nblk), dim3(kBlock), 0, h->stream, h->M, h->start, h->sf, h->PS);
     } else if (N == 3 && h->sfMatrix) {
       const bool lin = h->sf.linear != 0;
+      // persistent waves: the counters start at zero (a fill kernel, which a
+      // captured graph replays), the grid is what the device holds at once
+      if (h->sfPersist)
+        hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(kBlock), 0, h->stream, h->sf.work, int64_t(8), 0);
+#define SF_GATHER(K, ...)                                                                                    \
+  do {                                                                                                     \
+    if (h->sfPersist) {                                                                                    \
+      auto kp = K<__VA_ARGS__, true>;                                                                      \
+      hipLaunchKernelGGL(kp, dim3(persist_grid(h, (const void *)kp)), dim3(kBlock), 0, h->stream, h->M,    \
+                         h->start, h->sf, h->PS);                                                          \
+    } else {                                                                                               \
+      auto kp = K<__VA_ARGS__, false>;                                                                     \
+      hipLaunchKernelGGL(kp, dim3(nblk), dim3(kBlock), 0, h->stream, h->M, h->start, h->sf, h->PS);         \
+    }                                                                                                      \
+  } while (0)
       if (h->sf.cartesian && h->sf.alpha == 2)
-        hipLaunchKernelGGL((k_sf_gather_cart<2>), dim3(nblk), dim3(kBlock), 0, h->stream, h->M, h->start, h->sf, h->PS);
+        SF_GATHER(k_sf_gather_cart, 2);
       else if (h->sf.cartesian)
-        hipLaunchKernelGGL((k_sf_gather_cart<0>), dim3(nblk), dim3(kBlock), 0, h->stream, h->M, h->start, h->sf, h->PS);
+        SF_GATHER(k_sf_gather_cart, 0);
       else if (h->sf.alpha == 2 && lin)
-        hipLaunchKernelGGL((k_sf_gather_mfma<2, true>), dim3(nblk), dim3(kBlock), 0, h->stream, h->M, h->start, h->sf, h->PS);
+        SF_GATHER(k_sf_gather_mfma, 2, true);
       else if (h->sf.alpha == 2)
-        hipLaunchKernelGGL((k_sf_gather_mfma<2, false>), dim3(nblk), dim3(kBlock), 0, h->stream, h->M, h->start, h->sf, h->PS);
+        SF_GATHER(k_sf_gather_mfma, 2, false);
       else if (lin)
-        hipLaunchKernelGGL((k_sf_gather_mfma<0, true>), dim3(nblk), dim3(kBlock), 0, h->stream, h->M, h->start, h->sf, h->PS);
+        SF_GATHER(k_sf_gather_mfma, 0, true);
       else
-        hipLaunchKernelGGL((k_sf_gather_mfma<0, false>), dim3(nblk), dim3(kBlock), 0, h->stream, h->M, h->start, h->sf, h->PS);
+        SF_GATHER(k_sf_gather_mfma, 0, false);
+#undef SF_GATHER
     } else if (h->sf.alpha == 2) {
       LAUNCH_N2(k_sf_gather, N, 2, dim3(nblk), h->M, h->P, h->start, h->sf, h->PS);
     } else {
@@ -2006,6 +2151,26 @@ int pic_create(const pic_mesh_desc *mesh, const pic_params *par, pic_handle **ou
     rc |= upload(h, const_cast<uint8_t **>(&M.curved), hm.curved);
     rc |= upload(h, const_cast<double **>(&M.epsOneCell), hm.epsOneCell);
     rc |= upload(h, const_cast<uint8_t **>(&M.openElem), hm.openElem);
+    M.cartPeriodic = hm.cartPeriodic ? 1 : 0;
+    if (hm.nBC > 0) {   // ParticleBCTracking tables
+      M.bcHalo = hm.bcHalo;
+      rc |= upload(h, const_cast<int32_t **>(&M.bcElemNb), hm.bcElemNb);
+      rc |= upload(h, const_cast<uint8_t **>(&M.bcKind), hm.bcKind);
+      rc |= upload(h, const_cast<int8_t **>(&M.bcPV), hm.bcPV);
+      rc |= upload(h, const_cast<uint8_t **>(&M.bcFlip), hm.bcFlip);
+      rc |= upload(h, const_cast<double **>(&M.bz), hm.bz);
+      rc |= upload(h, const_cast<uint8_t **>(&M.sideType), hm.sideType);
+      rc |= upload(h, const_cast<uint8_t **>(&M.bbEmpty), hm.bbEmpty);
+      rc |= upload(h, const_cast<double **>(&M.sideNorm), hm.sideNorm);
+      rc |= upload(h, const_cast<double **>(&M.sideDist), hm.sideDist);
+      rc |= upload(h, const_cast<double **>(&M.slabN), hm.slabN);
+      rc |= upload(h, const_cast<double **>(&M.slabI), hm.slabI);
+      rc |= upload(h, const_cast<double **>(&M.baseV), hm.baseV);
+      rc |= upload(h, const_cast<int32_t **>(&M.e2bOff), hm.e2bOff);
+      rc |= upload(h, const_cast<int32_t **>(&M.e2bSide), hm.e2bSide);
+      rc |= upload(h, const_cast<double **>(&M.e2bDist), hm.e2bDist);
+      rc |= upload(h, const_cast<double **>(&M.e2bSlab), hm.e2bSlab);
+    }
   }
   rc |= upload(h, const_cast<int32_t **>(&M.binOff), hm.binOff);
   rc |= upload(h, const_cast<int32_t **>(&M.binElems), hm.binElems);
@@ -2091,6 +2256,9 @@ int pic_create(const pic_mesh_desc *mesh, const pic_params *par, pic_handle **ou
     double4 *rec = nullptr;
     rc |= dalloc(h, &rec, size_t(cap) * 2);
     h->sf.rec = rec;
+    rc |= dalloc(h, &h->sf.work, 8);
+    if (hipDeviceGetAttribute(&h->nCU, hipDeviceAttributeMultiprocessorCount, h->device) != hipSuccess || h->nCU < 1)
+      h->nCU = 1;
   }
   rc |= dalloc(h, &h->start, hm.nElems + 1);
   rc |= dalloc(h, &h->startNew, hm.nElems + 1);
@@ -2102,9 +2270,13 @@ int pic_create(const pic_mesh_desc *mesh, const pic_params *par, pic_handle **ou
   rc |= dalloc(h, &h->arrCnt, hm.nElems);
   rc |= dalloc(h, &h->insideById, cap);
   rc |= dalloc(h, &h->stateById, cap * 6);
-  rc |= dalloc(h, &h->mv.count, 1);
+  rc |= dalloc(h, &h->mv.count, 2);
   rc |= dalloc(h, &h->mv.idx, cap);
   rc |= dalloc(h, &h->mv.last, cap * 3);
+  if (par->tracking == PIC_TRACK_REFMAPPING) {
+    rc |= dalloc(h, &h->mv.flag, cap);
+    rc |= dalloc(h, &h->mv.slot2, cap);
+  }
   rc |= dalloc(h, &h->lostCount, 1);
   if (rc) { pic_destroy(h); return PIC_EHIP; }
   HIPCHK(hipMemset(h->lostCount, 0, sizeof(unsigned long long)));
@@ -2348,7 +2520,7 @@ int enqueue_step(pic_handle *h, double dt, int64_t nGrid, bool timed) {
   const bool mixed = h->par.precision == PIC_PREC_MIXED;   // N = 3
   const dim3 grid(grid_for(std::max<int64_t>(nGrid, 1)));
   if (h->par.tracking == PIC_TRACK_REFMAPPING) {
-    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(kBlock), 0, h->stream, h->mv.count, int64_t(1), 0);
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(kBlock), 0, h->stream, h->mv.count, int64_t(2), 0);
     if (mixed && h->hm.NGeo == 1)
       hipLaunchKernelGGL((k_rm_push_inside<3, 2, true>), grid, dim3(kBlock), 0, h->stream, h->M, h->P, nGrid, dt,
                          h->newElem, h->mv, nDev);
@@ -2357,8 +2529,11 @@ int enqueue_step(pic_handle *h, double dt, int64_t nGrid, bool timed) {
                          h->newElem, h->mv, nDev);
     else
       LAUNCH_NG(k_rm_push_inside, h->hm.N, h->hm.NGeo, grid, h->M, h->P, nGrid, dt, h->newElem, h->mv, nDev);
-    LAUNCH_NG(k_rm_movers, h->hm.N, h->hm.NGeo, dim3(1024), h->M, h->P, h->newElem, h->mv, h->lostCount,
-              h->insideById, h->stateById);
+    LAUNCH_NG(k_rm_movers, h->hm.N, h->hm.NGeo, dim3(std::min<unsigned>(grid.x, 4096u)), h->M, h->P, h->newElem, h->mv,
+              h->lostCount, h->insideById, h->stateById);
+    if (h->hm.nBC > 0)
+      LAUNCH_NG(k_rm_bc_movers, h->hm.N, h->hm.NGeo, dim3(1024), h->M, h->P, h->newElem, h->mv, h->lostCount,
+                h->insideById, h->stateById);
   } else {
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(kBlock), 0, h->stream, h->mv.count, int64_t(1), 0);
     // every element affine (N = 3): xi without Newton
@@ -2464,6 +2639,7 @@ int pic_time_step(pic_handle *h, double dt, int32_t nsteps) {
 int pic_set_tuning(pic_handle *h, const pic_tuning *t) {
   if (!h || !t) return fail(PIC_EINVAL, "pic_set_tuning: null argument");
   if (t->sf_kernel < 0 || t->sf_kernel > 2) return fail(PIC_EINVAL, "pic_set_tuning: sf_kernel must be 0, 1 or 2");
+  if (t->sf_persistent < 0 || t->sf_persistent > 1) return fail(PIC_EINVAL, "pic_set_tuning: sf_persistent must be 0 or 1");
   if (set_device(h)) return PIC_EHIP;
   // a captured graph holds the kernel choice it was recorded with, and the
   // record refresh before a replay follows the current choice: any change
@@ -2472,6 +2648,7 @@ int pic_set_tuning(pic_handle *h, const pic_tuning *t) {
   drop_graph(h);
   h->useGraph = t->use_graph != 0;
   if (t->sf_kernel != 0) h->sfMatrix = t->sf_kernel == 2;
+  h->sfPersist = t->sf_persistent != 0 && h->sf.work != nullptr;
   return PIC_OK;
 }
 

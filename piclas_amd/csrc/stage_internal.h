@@ -84,7 +84,31 @@ struct HostMesh {
   double PV[3][3] = {{0}};
   // domain bounding box
   double lo[3] = {0}, hi[3] = {0};
+  // RefMapping boundary tracking (ParticleBCTracking), built when not every
+  // boundary is periodic: per boundary side (a HOPR side row with a BC)
+  bool cartPeriodic = false;       // every boundary periodic: PeriodicMovement
+  int nBC = 0;
+  std::vector<int32_t> bcElemNb;   // [nBC] 0-based neighbour across the side, -1
+  std::vector<uint8_t> bcKind;     // [nBC] PIC_BC_*
+  std::vector<int8_t> bcPV;        // [nBC] BC_ALPHA (periodic vector id)
+  std::vector<uint8_t> bcFlip;     // [nBC] flip as the tracking reads it
+  std::vector<double> bz;          // [nBC][NGeo+1][NGeo+1][3] BezierControlPoints3D
+  std::vector<uint8_t> sideType;   // [nBC] kSide*
+  std::vector<uint8_t> bbEmpty;    // [nBC] BoundingBoxIsEmpty
+  std::vector<double> sideNorm;    // [nBC][3] SideNormVec
+  std::vector<double> sideDist;    // [nBC] SideDistance
+  std::vector<double> slabN;       // [nBC][3][3] SideSlabNormals (slab, xyz)
+  std::vector<double> slabI;       // [nBC][6] SideSlabIntervals
+  std::vector<double> baseV;       // [nBC][3][3] BaseVectors0..2
+  std::vector<int32_t> e2bOff;     // [nElems+1] ElemToBCSides
+  std::vector<int32_t> e2bSide;    // SideBCMetrics(BCSIDE_SIDEID) as boundary side index
+  std::vector<double> e2bDist;     // SideBCMetrics(BCSIDE_DISTANCE)
+  std::vector<double> e2bSlab;     // [list entry][6] the side's normal slab: n, lo, hi, 0
+  double bcHalo = 0;               // BC_halo_eps
 };
+
+// SideType values (piclas.h:262-266)
+enum { kSidePlanarRect = 0, kSidePlanarNonRect = 1, kSideBilinear = 2, kSidePlanarCurved = 3, kSideCurved = 4 };
 
 // Builds every table above.  Returns empty string on success, else a message.
 std::string build_host_mesh(const pic_mesh_desc &m, const pic_params &p, HostMesh &hm);

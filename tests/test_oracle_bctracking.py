@@ -214,3 +214,53 @@ def test_leave_and_reenter_is_removed(oracle_mod):
     assert list(ins) == [False, True]
     assert np.allclose(out[1, :3], [0.60, 0.2, 0.5])
     assert o.lost() == 0
+
+
+def side_normal_slab(P):
+    """Unit normal of the side's normal slab (the cross product of the xi and
+    eta edge directions, as SideSlabNormals) and the interval of n . X over
+    the control points."""
+    N = P.shape[0] - 1
+    t = P[N, 0] - P[0, 0] + P[N, N] - P[0, N]
+    t /= np.linalg.norm(t)
+    u = P[0, N] - P[0, 0] + P[N, N] - P[N, 0]
+    n = np.cross(t, u)
+    n /= np.linalg.norm(n)
+    proj = P.reshape(-1, 3) @ n
+    return n, proj.min(), proj.max()
+
+
+def test_steps_outside_normal_slab_hit_nothing(oracle_mod):
+    """The device takes as boundary candidates only particles whose step
+    reaches the normal slab of a listed side (the control points' interval of
+    n . X, widened by 1e-3 of the side's extent).  Steps that stay outside
+    it, on either side and in any direction within the slab's plane region,
+    get no intersection from the oracle's routines, on the curved walls and
+    the planar z faces; steps that cross the patch do reach the slab."""
+    m, o = annulus_oracle(oracle_mod)
+    side, typ, bez, org, rad = o.bc_sides()
+    rng = np.random.default_rng(11)
+    checked = 0
+    for b in rng.choice(len(side), 40, replace=False):
+        n, lo, hi = side_normal_slab(bez[b])
+        ext = 2 * rad[b]
+        for _ in range(6):
+            above = rng.random() < 0.5
+            gap = ext * rng.uniform(1e-3, 2e-2)
+            c0 = bez[b].reshape(-1, 3).mean(0)
+            c0 = c0 + (((hi + gap) if above else (lo - gap)) - c0 @ n) * n
+            tang = rng.normal(size=3)
+            tang -= (tang @ n) * n
+            # steps that stay on one side of the slab: normal coordinate moves
+            # away from it or parallel to it
+            away = (1.0 if above else -1.0) * rng.uniform(0.0, 0.5) * ext
+            last = c0 + rng.uniform(-0.5, 0.5) * ext * tang / np.linalg.norm(tang)
+            x = last + rng.uniform(0.2, 1.5) * ext * tang / np.linalg.norm(tang) + away * n
+            a, xi, eta = o.bc_side_intersection(b, last, x)
+            assert a == -1.0, (b, typ[b], a, xi, eta)
+            checked += 1
+        # a step through the middle of the patch crosses it and reaches the slab
+        c = patch(bez[b], 0.1, -0.1)
+        last, x = c - 0.01 * ext * n, c + 0.01 * ext * n
+        assert max(last @ n, x @ n) >= lo and min(last @ n, x @ n) <= hi
+    assert checked == 240
