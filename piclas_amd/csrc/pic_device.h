@@ -34,6 +34,7 @@ struct DevMesh {
   const double *E;          // [e][nDof][3]
   double xiCL[2], wBaryCL[2];   // linear CL basis (corner map)
   double xGP[8], wBaryGP[8];
+  double cGP[8];            // 1 / prod_{m != i} (xGP_i - xGP_m): product-form Lagrange factors
   double PV[3][3];
   double ext[6];
   double charge[8], mass[8], mpf[8];
@@ -56,7 +57,7 @@ struct DevMesh {
   const double *affine;
   // mixed precision: E and the Gauss basis in FP32
   const float *E32;         // [e][nDof][3]
-  float xGP32[8], wBaryGP32[8];
+  float xGP32[8], wBaryGP32[8], cGP32[8];
   // RefMapping boundary tracking (ParticleBCTracking); null unless some
   // boundary is not periodic (then cartPeriodic = 0)
   int cartPeriodic;         // every boundary periodic: PeriodicMovement
@@ -116,6 +117,32 @@ __device__ inline void lagrange(double x, const double *xg, const double *wb, do
   for (int i = 0; i < NP; ++i) { L[i] = wb[i] / (x - xg[i]); sum = sum + L[i]; }
 #pragma unroll
   for (int i = 0; i < NP; ++i) L[i] = L[i] / sum;
+}
+
+// The Lagrange basis at the Gauss points for the field evaluation, in
+// product form L_i(x) = c_i prod_{m != i} (x - x_m) with c_i =
+// 1 / prod_{m != i} (x_i - x_m): the polynomial of LagrangeInterpolationPolys'
+// barycentric form (interpolation/basis.f90:1223-1264; the two agree to
+// round-off) with no division per particle; a node hit within ALMOSTEQUAL
+// gives the unit vector as there.
+template <int NP>
+__device__ inline void lagrange_prod(double x, const double *xg, const double *c, double *L) {
+  bool eq = false;
+#pragma unroll
+  for (int i = 0; i < NP; ++i) {
+    L[i] = 0.0;
+    if (almost_equal_unity(x, xg[i])) { L[i] = 1.0; eq = true; }
+  }
+  if (eq) return;
+  double d[NP];
+#pragma unroll
+  for (int i = 0; i < NP; ++i) d[i] = x - xg[i];
+  double pre = 1.0;
+#pragma unroll
+  for (int i = 0; i < NP; ++i) { L[i] = pre; pre = pre * d[i]; }
+  double suf = 1.0;
+#pragma unroll
+  for (int i = NP - 1; i >= 0; --i) { L[i] = L[i] * suf * c[i]; suf = suf * d[i]; }
 }
 
 // getDet / getInv, interpolation/eval_xyz.f90:448-495
@@ -319,7 +346,7 @@ __device__ inline void eval_field_p(const DevMesh &M, const double *Ue, const do
   constexpr int n1 = N + 1;
   double L[3][n1];
 #pragma unroll
-  for (int d = 0; d < 3; ++d) lagrange<n1>(xi[d], M.xGP, M.wBaryGP, L[d]);
+  for (int d = 0; d < 3; ++d) lagrange_prod<n1>(xi[d], M.xGP, M.cGP, L[d]);
   U[0] = U[1] = U[2] = 0.0;
 #pragma unroll 1
   for (int k = 0; k < n1; ++k) {
@@ -418,7 +445,7 @@ __device__ inline void eval_field_s(const DevMesh &M, int e, const double xi[3],
   constexpr int n1 = N + 1, NP = n1 * n1 * 3;
   double L[3][n1];
 #pragma unroll
-  for (int d = 0; d < 3; ++d) lagrange<n1>(xi[d], M.xGP, M.wBaryGP, L[d]);
+  for (int d = 0; d < 3; ++d) lagrange_prod<n1>(xi[d], M.xGP, M.cGP, L[d]);
   const double *Ue = M.E + (size_t)e * n1 * n1 * n1 * 3;
   U[0] = U[1] = U[2] = 0.0;
 #pragma unroll 1
@@ -502,8 +529,9 @@ __device__ __attribute__((noinline)) void em_field_dw(const DevMesh &M, int e, c
 
 // Mixed precision: LagrangeInterpolationPolys and EvaluateFieldAtRefPos in
 // FP32 (nodes equal to the FP32 machine epsilon count as hit)
+// (product form, as lagrange_prod; c: the factors in FP32)
 template <int NP>
-__device__ inline void lagrange_f32(float x, const float *xg, const float *wb, float *L) {
+__device__ inline void lagrange_f32(float x, const float *xg, const float *c, float *L) {
   bool eq = false;
 #pragma unroll
   for (int i = 0; i < NP; ++i) {
@@ -516,11 +544,15 @@ __device__ inline void lagrange_f32(float x, const float *xg, const float *wb, f
     }
   }
   if (eq) return;
-  float sum = 0.f;
+  float d[NP];
 #pragma unroll
-  for (int i = 0; i < NP; ++i) { L[i] = wb[i] / (x - xg[i]); sum = sum + L[i]; }
+  for (int i = 0; i < NP; ++i) d[i] = x - xg[i];
+  float pre = 1.f;
 #pragma unroll
-  for (int i = 0; i < NP; ++i) L[i] = L[i] / sum;
+  for (int i = 0; i < NP; ++i) { L[i] = pre; pre = pre * d[i]; }
+  float suf = 1.f;
+#pragma unroll
+  for (int i = NP - 1; i >= 0; --i) { L[i] = L[i] * suf * c[i]; suf = suf * d[i]; }
 }
 
 template <int N>
@@ -528,7 +560,7 @@ __device__ inline void eval_field_f32_p(const DevMesh &M, const float *Ue, const
   constexpr int n1 = N + 1;
   float L[3][n1];
 #pragma unroll
-  for (int d = 0; d < 3; ++d) lagrange_f32<n1>(float(xi[d]), M.xGP32, M.wBaryGP32, L[d]);
+  for (int d = 0; d < 3; ++d) lagrange_f32<n1>(float(xi[d]), M.xGP32, M.cGP32, L[d]);
   float u0 = 0.f, u1 = 0.f, u2 = 0.f;
 #pragma unroll 1
   for (int k = 0; k < n1; ++k)
@@ -558,7 +590,7 @@ __device__ inline void eval_field_f32_s(const DevMesh &M, int e, const double xi
   constexpr int n1 = N + 1, NP = n1 * n1 * 3;
   float L[3][n1];
 #pragma unroll
-  for (int d = 0; d < 3; ++d) lagrange_f32<n1>(float(xi[d]), M.xGP32, M.wBaryGP32, L[d]);
+  for (int d = 0; d < 3; ++d) lagrange_f32<n1>(float(xi[d]), M.xGP32, M.cGP32, L[d]);
   if constexpr (NP % 2 != 0) {   // planes not 8-byte aligned: the vector loads (mixed runs at N = 3)
     eval_field_f32<N>(M, e, xi, U);
     return;
@@ -678,9 +710,11 @@ __device__ inline void boris_push(double q, double m, bool isNew, const double F
     const double vpl[3] = {vm[0] + f2 * cr[0], vm[1] + f2 * cr[1], vm[2] + f2 * cr[2]};
     const double vn1[3] = {vpl[0] + c_1 * F[0], vpl[1] + c_1 * F[1], vpl[2] + c_1 * F[2]};
     const double den = sqrt(1 + (vn1[0] * vn1[0] + vn1[1] * vn1[1] + vn1[2] * vn1[2]) * c2i);
-    v[0] = vn1[0] / den;
-    v[1] = vn1[1] / den;
-    v[2] = vn1[2] / den;
+    // v_n+1 / gamma_n+1 as one reciprocal and three products (round-off apart)
+    const double rden = 1.0 / den;
+    v[0] = vn1[0] * rden;
+    v[1] = vn1[1] * rden;
+    v[2] = vn1[2] * rden;
   }
   x[0] = x[0] + v[0] * dt;
   x[1] = x[1] + v[1] * dt;

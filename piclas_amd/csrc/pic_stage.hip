@@ -2174,7 +2174,14 @@ int pic_create(const pic_mesh_desc *mesh, const pic_params *par, pic_handle **ou
   }
   rc |= upload(h, const_cast<int32_t **>(&M.binOff), hm.binOff);
   rc |= upload(h, const_cast<int32_t **>(&M.binElems), hm.binElems);
-  for (int i = 0; i <= hm.N; ++i) { M.xGP[i] = hm.xGP[i]; M.wBaryGP[i] = hm.wBaryGP[i]; }
+  for (int i = 0; i <= hm.N; ++i) {
+    M.xGP[i] = hm.xGP[i];
+    M.wBaryGP[i] = hm.wBaryGP[i];
+    double p = 1.0;
+    for (int m = 0; m <= hm.N; ++m)
+      if (m != i) p *= hm.xGP[i] - hm.xGP[m];
+    M.cGP[i] = 1.0 / p;
+  }
   for (int v = 0; v < 3; ++v)
     for (int c = 0; c < 3; ++c) M.PV[v][c] = hm.PV[v][c];
   for (int c = 0; c < 6; ++c) M.ext[c] = par->external_field[c];
@@ -2183,7 +2190,11 @@ int pic_create(const pic_mesh_desc *mesh, const pic_params *par, pic_handle **ou
   if (par->precision == PIC_PREC_MIXED) {
     rc |= dalloc(h, &h->E32, size_t(hm.nElems) * hm.nDof * 3);
     M.E32 = h->E32;
-    for (int i = 0; i <= hm.N; ++i) { M.xGP32[i] = float(hm.xGP[i]); M.wBaryGP32[i] = float(hm.wBaryGP[i]); }
+    for (int i = 0; i <= hm.N; ++i) {
+      M.xGP32[i] = float(hm.xGP[i]);
+      M.wBaryGP32[i] = float(hm.wBaryGP[i]);
+      M.cGP32[i] = float(M.cGP[i]);
+    }
   }
   rc |= upload(h, &h->sJ, hm.sJ);
   rc |= dalloc(h, &h->elemPart, hm.nElems);
