@@ -190,6 +190,30 @@ def cpu_baseline(w, fraction, steps, threads):
                                                                                      t_priv)}
 
 
+# Algorithmic bytes of the push kernels (interpolation, Boris push, the test
+# in the last element) per particle: reads x, v (48), elem (4), species (4),
+# IsNewPart (1); writes x, v (48), IsNewPart (1), the new owner (4); with
+# RefMapping also PartPosRef read and written (24 + 24).  Element tables are
+# counted once per element: field at the Gauss points (1,536 B), affine map
+# (96 B) and side corners (576 B) on the affine mesh; field, XCL_NGeo (648 B),
+# dXCL_NGeo (1,944 B) and the Gauss points of the start guess (1,536 B) with
+# RefMapping at NGeo = 2.
+PUSH_BYTES_AFFINE = 110
+PUSH_BYTES_REFMAP = 158
+
+
+def push_roofline(st, n, steps, kernel, bytes_per_particle, table_bytes):
+    """HBM roofline of the push kernel (kernel A) from its HIP-event time over
+    the eager steps."""
+    ms = st["ms_push_main"] / max(steps, 1)
+    if ms <= 0:
+        return None
+    gbs = (bytes_per_particle * n + table_bytes) / (ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+            "frac": round(gbs / PEAK_HBM_GBS, 4), "kernel": kernel, "avg_launch_ms": round(ms, 4),
+            "algorithmic_bytes_per_particle": bytes_per_particle, "element_table_bytes": table_bytes}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -254,6 +278,8 @@ def main():
             "kernel": kname, "avg_launch_ms": round(avg_ms, 4),
             "algorithmic_flops_per_particle": round(fpp, 1), "note": note}
     ms_step = t / args.steps * 1e3
+    push = push_roofline(st, n_now, args.steps, "k_push_inside<3, false, true>", PUSH_BYTES_AFFINE,
+                         (args.nx ** 3) * 2208)
     out = {"metric": METRIC, "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f64", "data": "synthetic",
@@ -264,7 +290,7 @@ def main():
            "stage_ms_per_step": {"deposit": st["ms_deposit"] / args.steps,
                                  "interp_push_track": st["ms_push_track"] / args.steps,
                                  "sort": st["ms_sort"] / args.steps},
-           "roofline": roof}
+           "roofline": roof, "push_roofline": push}
     if not args.no_mixed:
         out["mixed"] = mixed_report(w, args, r64)
     if not args.no_exchange:
@@ -313,6 +339,8 @@ def bench_annulus(args):
                       "particles": n, "particles_placed": n_in, "deposition": "shape_function R=0.03 m alpha=2",
                       "tracking": "refmapping", "parallelism": "1 GPU"},
            "setup_s_locate_and_sort": round(t_locate, 3), "tuning": tuning,
+           "push_roofline": push_roofline(st, st["n_particles"], args.steps, "k_rm_push_inside<3, 3, false>",
+                                          PUSH_BYTES_REFMAP, w.mesh.nElems * 5664),
            "stage_ms_per_step": {"deposit": st["ms_deposit"] / args.steps,
                                  "interp_push_track": st["ms_push_track"] / args.steps,
                                  "sort": st["ms_sort"] / args.steps},

@@ -113,6 +113,9 @@ typedef struct {
   int32_t n_deposit_launches;     /* launches of the dominant deposition kernel   */
   double  ms_deposit_main;        /* device time of that kernel alone             */
   int32_t deposit_kernel;         /* that kernel: PIC_KERNEL_*                    */
+  double  ms_push_main;           /* device time of the push kernel alone (eager
+                                     steps; interpolation, push and the test in
+                                     the last element)                           */
 } pic_stats;
 
 typedef struct pic_handle pic_handle;

@@ -177,7 +177,7 @@ struct Movers {
 };
 
 template <int N, bool MIXED = false, bool AFFINE = false>
-__global__ __launch_bounds__(kBlock) void k_push_inside(DevMesh M, DevParts P, int64_t n, double dt, int32_t *newElem,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE_PUSH))) void k_push_inside(DevMesh M, DevParts P, int64_t n, double dt, int32_t *newElem,
                                                         Movers mv, const int32_t *nDev) {
   const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
   if (nDev) n = *nDev;   // particle count on the device (graph replay)
@@ -282,7 +282,7 @@ __device__ inline void rm_removed(DevParts &P, int64_t i, const double x[3], con
 // interpolates, pushes and maps the particle in its last element; the few
 // that left it go to a list that kernel B searches, from the same state.
 template <int N, int G, bool MIXED = false>
-__global__ __launch_bounds__(kBlock) void k_rm_push_inside(DevMesh M, DevParts P, int64_t n, double dt, int32_t *newElem,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE_RM))) void k_rm_push_inside(DevMesh M, DevParts P, int64_t n, double dt, int32_t *newElem,
                                                            Movers mv, const int32_t *nDev) {
   const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
   if (nDev) n = *nDev;
@@ -1768,8 +1768,8 @@ struct pic_handle {
   bool stagedValid = false;   // LastPartPos/FieldAtParticle refer to the current order
   std::vector<std::pair<int64_t, std::vector<double>>> unplaced;  // particles no element holds at set-up
   // statistics
-  hipEvent_t ev[9] = {};
-  double msDeposit = 0, msPushTrack = 0, msSort = 0, msDepositMain = 0;
+  hipEvent_t ev[11] = {};   // ev[9], ev[10]: around the push kernel (kernel A)
+  double msDeposit = 0, msPushTrack = 0, msSort = 0, msDepositMain = 0, msPushMain = 0;
   int nDepositLaunches = 0;
   int64_t nSteps = 0;
 };
@@ -2532,6 +2532,7 @@ int enqueue_step(pic_handle *h, double dt, int64_t nGrid, bool timed) {
   const dim3 grid(grid_for(std::max<int64_t>(nGrid, 1)));
   if (h->par.tracking == PIC_TRACK_REFMAPPING) {
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(kBlock), 0, h->stream, h->mv.count, int64_t(2), 0);
+    if (timed) HIPCHK(hipEventRecord(h->ev[9], h->stream));
     if (mixed && h->hm.NGeo == 1)
       hipLaunchKernelGGL((k_rm_push_inside<3, 2, true>), grid, dim3(kBlock), 0, h->stream, h->M, h->P, nGrid, dt,
                          h->newElem, h->mv, nDev);
@@ -2540,6 +2541,7 @@ int enqueue_step(pic_handle *h, double dt, int64_t nGrid, bool timed) {
                          h->newElem, h->mv, nDev);
     else
       LAUNCH_NG(k_rm_push_inside, h->hm.N, h->hm.NGeo, grid, h->M, h->P, nGrid, dt, h->newElem, h->mv, nDev);
+    if (timed) HIPCHK(hipEventRecord(h->ev[10], h->stream));
     LAUNCH_NG(k_rm_movers, h->hm.N, h->hm.NGeo, dim3(std::min<unsigned>(grid.x, 4096u)), h->M, h->P, h->newElem, h->mv,
               h->lostCount, h->insideById, h->stateById);
     if (h->hm.nBC > 0)
@@ -2547,6 +2549,7 @@ int enqueue_step(pic_handle *h, double dt, int64_t nGrid, bool timed) {
                 h->insideById, h->stateById);
   } else {
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(kBlock), 0, h->stream, h->mv.count, int64_t(1), 0);
+    if (timed) HIPCHK(hipEventRecord(h->ev[9], h->stream));
     // every element affine (N = 3): xi without Newton
     const bool aff = h->M.affine != nullptr && h->hm.N == 3;
     if (mixed && aff)
@@ -2559,6 +2562,7 @@ int enqueue_step(pic_handle *h, double dt, int64_t nGrid, bool timed) {
       hipLaunchKernelGGL((k_push_inside<3, false, true>), grid, dim3(kBlock), 0, h->stream, h->M, h->P, nGrid, dt,
                          h->newElem, h->mv, nDev);
     else { LAUNCH_N(k_push_inside, h->hm.N, grid, h->M, h->P, nGrid, dt, h->newElem, h->mv, nDev); }
+    if (timed) HIPCHK(hipEventRecord(h->ev[10], h->stream));
     hipLaunchKernelGGL(k_track_movers, dim3(1024), dim3(kBlock), 0, h->stream, h->M, h->P, h->newElem, h->mv,
                        h->lostCount, h->insideById, h->stateById);
   }
@@ -2640,6 +2644,7 @@ int pic_time_step(pic_handle *h, double dt, int32_t nsteps) {
     h->msDeposit += elapsed(h->ev[0], h->ev[2]);
     if (h->par.deposition != PIC_DEPO_NONE) { h->msDepositMain += elapsed(h->ev[8], h->ev[1]); h->nDepositLaunches++; }
     h->msPushTrack += elapsed(h->ev[3], h->ev[4]);
+    h->msPushMain += elapsed(h->ev[9], h->ev[10]);
     h->msSort += elapsed(h->ev[4], h->ev[5]);
     h->nSteps++;
   }
@@ -2824,13 +2829,14 @@ int pic_get_stats(pic_handle *h, pic_stats *st) {
   st->ms_sort = h->msSort;
   st->n_deposit_launches = h->nDepositLaunches;
   st->ms_deposit_main = h->msDepositMain;
+  st->ms_push_main = h->msPushMain;
   st->deposit_kernel = deposit_kernel(h);
   return PIC_OK;
 }
 
 int pic_reset_stats(pic_handle *h) {
   if (!h) return fail(PIC_EINVAL, "pic_reset_stats: null handle");
-  h->msDeposit = h->msPushTrack = h->msSort = h->msDepositMain = 0;
+  h->msDeposit = h->msPushTrack = h->msSort = h->msDepositMain = h->msPushMain = 0;
   h->nDepositLaunches = 0;
   h->nSteps = 0;
   return PIC_OK;

@@ -70,7 +70,7 @@ class PicStats(ctypes.Structure):
     _fields_ = [("n_particles", ctypes.c_int64), ("n_lost", ctypes.c_int64), ("n_steps", ctypes.c_int64),
                 ("ms_deposit", ctypes.c_double), ("ms_push_track", ctypes.c_double), ("ms_sort", ctypes.c_double),
                 ("n_deposit_launches", ctypes.c_int32), ("ms_deposit_main", ctypes.c_double),
-                ("deposit_kernel", ctypes.c_int32)]
+                ("deposit_kernel", ctypes.c_int32), ("ms_push_main", ctypes.c_double)]
 
 
 class PicTuning(ctypes.Structure):
