@@ -265,10 +265,10 @@ def main():
     achieved = fpp * n_now / (avg_ms * 1e-3) / 1e12
     a = w.sf_alpha if w.sf_alpha == 2 else 0
     kname, note = {
-        stage.KERNEL_SF_CARTESIAN: ("k_sf_gather_cart<%d>" % a,
+        stage.KERNEL_SF_CARTESIAN: ("k_sf_gather_cart<%d, false>" % a,
                                     "FP64 issue bound: per pair a subtraction, the clamp and the square on the VALU "
                                     "(axis-aligned tensor grid of Gauss points), products summed on FP64 MFMA (4x4x4)"),
-        stage.KERNEL_SF_MATRIX: ("k_sf_gather_mfma<%d, true>" % a,
+        stage.KERNEL_SF_MATRIX: ("k_sf_gather_mfma<%d, true, false>" % a,
                                  "FP64 issue bound: S on the VALU, its products summed on FP64 MFMA (4x4x4)"),
     }.get(st["deposit_kernel"], ("k_sf_gather<%d, %d>" % (w.N, a), "FP64 issue bound: all on the VALU"))
     note += "; peak = MI355X FP64 vector peak (the FP64 matrix peak is the same 78.6 TFLOP/s)"
@@ -280,6 +280,8 @@ def main():
     ms_step = t / args.steps * 1e3
     push = push_roofline(st, n_now, args.steps, "k_push_inside<3, false, true>", PUSH_BYTES_AFFINE,
                          (args.nx ** 3) * 2208)
+    if push is not None:
+        push["traffic"] = measured_traffic(push["kernel"]) if (args.nx, args.ppc) == (48, 200) else None
     out = {"metric": METRIC, "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f64", "data": "synthetic",
