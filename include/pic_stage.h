@@ -258,11 +258,18 @@ int  pic_adjoint(pic_handle *h, double dt, int32_t K, int32_t checkpoint_every, 
  *              elements) instead of one wave per target element; for meshes
  *              whose elements hold very different particle counts.  Each
  *              target is summed by one wave in the same order either way, so
- *              the result is bit-identical. */
+ *              the result is bit-identical.
+ *   sf_shared_loads  1: on meshes that are not axis-aligned tensor grids, the
+ *              four waves of a workgroup take four consecutive target
+ *              elements and stage each record of the union of their source
+ *              lists once in LDS for the four (fewer record reads, more
+ *              culls per wave); bit-identical as well.  Takes precedence
+ *              over sf_persistent there. */
 typedef struct {
   int32_t use_graph;
   int32_t sf_kernel;
   int32_t sf_persistent;
+  int32_t sf_shared_loads;
 } pic_tuning;
 int  pic_set_tuning(pic_handle *h, const pic_tuning *t);
 int  pic_get_stats(pic_handle *h, pic_stats *st);

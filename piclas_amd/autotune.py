@@ -2,8 +2,9 @@
 
 The stage has a few execution knobs that do not change its results beyond
 round-off in the order of the shape-function sums (pic_set_tuning): graph
-replay of the time step, the shape-function kernel, and persistent waves for
-the matrix-core shape-function sums (bit-identical to one wave per target).
+replay of the time step, the shape-function kernel, persistent waves for the
+matrix-core shape-function sums, and record loads shared by the four targets
+of a workgroup (both bit-identical to one wave per target).
 ``autotune`` times each candidate on a scratch stage built for the same mesh
 and particles (the caller's stage is not touched) and returns the fastest as
 keyword arguments of ``PICStage.set_tuning``.
@@ -21,8 +22,8 @@ def candidates(N, deposition=stage.DEPO_SF):
     out = []
     for g in (False, True):
         for k in ((1, 2) if N == 3 and deposition == stage.DEPO_SF else (0,)):
-            for p in ((False, True) if k == 2 else (False,)):
-                out.append({"use_graph": g, "sf_kernel": k, "sf_persistent": p})
+            for p, c in (((False, False), (True, False), (False, True)) if k == 2 else ((False, False),)):
+                out.append({"use_graph": g, "sf_kernel": k, "sf_persistent": p, "sf_shared_loads": c})
     return out
 
 

@@ -1170,6 +1170,33 @@ std::string build_host_mesh(const pic_mesh_desc &m, const pic_params &par, HostM
       }
       hm.sfRunOff[t + 1] = int32_t(hm.sfRunFirst.size());
     }
+    // runs of the union of the lists of four consecutive targets (the
+    // workgroup-cooperative gather stages each of their records once for
+    // the four), ordered as every target's own runs: by shift, then element
+    const int ng4 = (ne + 3) / 4;
+    hm.sfGrpRunOff.assign(ng4 + 1, 0);
+    hm.sfGrpRunFirst.clear();
+    hm.sfGrpRunEnd.clear();
+    hm.sfGrpRunShift.clear();
+    for (int g = 0; g < ng4; ++g) {
+      cand.clear();
+      for (int t = 4 * g; t < std::min(ne, 4 * g + 4); ++t)
+        for (int q = hm.sfOff[t]; q < hm.sfOff[t + 1]; ++q)
+          cand.push_back(int64_t(hm.sfShift[q]) * (int64_t(1) << 32) + hm.sfSrc[q]);
+      std::sort(cand.begin(), cand.end());
+      cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
+      for (size_t a = 0; a < cand.size();) {
+        const int64_t sh = cand[a] >> 32;
+        const int32_t first = int32_t(cand[a] & 0xffffffff);
+        size_t b = a + 1;
+        while (b < cand.size() && (cand[b] >> 32) == sh && int32_t(cand[b] & 0xffffffff) == first + int32_t(b - a)) ++b;
+        hm.sfGrpRunFirst.push_back(first);
+        hm.sfGrpRunEnd.push_back(first + int32_t(b - a));
+        hm.sfGrpRunShift.push_back(uint8_t(sh));
+        a = b;
+      }
+      hm.sfGrpRunOff[g + 1] = int32_t(hm.sfGrpRunFirst.size());
+    }
     // inverse lists: source element -> (target, shift), targets ascending
     hm.sfSrcOff.assign(ne + 1, 0);
     for (int32_t sv : hm.sfSrc) hm.sfSrcOff[sv + 1]++;

@@ -808,6 +808,8 @@ struct SFArgs {
   int alpha;
   int linear;   // NGeo == 1: the barycentre cull is implied by the DOF-box test
   int32_t *work;   // [8] target counters of the persistent gathers (zeroed before each launch)
+  const int32_t *groff, *grfirst, *grend;   // runs of the union of the lists of 4 consecutive targets
+  const uint8_t *grshift;
 };
 
 template <int N, int ALPHA>
@@ -1123,6 +1125,147 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE_MX))
     sf_gather_mfma_target<ALPHA, LINEAR>(M, start, A, PS, t, lds[wib], lane);
     __builtin_amdgcn_wave_barrier();
   }
+}
+
+// The general gather with the loads shared by a workgroup: its four waves
+// take four consecutive target elements (neighbours that share most of their
+// sources) and walk the union of their lists in chunks of 128 records, staged
+// once in LDS for the four; each wave culls every staged record against its
+// own target.  A target's own records pass its cull in the order of its own
+// runs (a record of an element that is not in its list lies farther than
+// r_sf from its DOF box), so each target's ring, groups and sums are those of
+// k_sf_gather_mfma, bit for bit; the records are read from memory once per
+// workgroup instead of once per wave, for ~1.3x the cull work per wave.
+constexpr int kStageCoop = 128;
+template <int ALPHA, bool LINEAR>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE_MX))) void k_sf_gather_mfma_coop(DevMesh M, const int32_t *start, SFArgs A, double *PS) {
+#pragma clang fp contract(fast)
+  constexpr int nd = 64;
+  constexpr int kRing = kRingMfma;
+  __shared__ double lds[kBlock / kWave][kRingMfma][8];
+  __shared__ double4 stg[kStageCoop][2];
+  const int wib = threadIdx.x / kWave;
+  const int lane = lane_id();
+  const int grp = xcd_block(blockIdx.x, gridDim.x);               // block-uniform
+  if (grp * (kBlock / kWave) >= M.nElems) return;                 // blocks of the XCD padding
+  const int t = __builtin_amdgcn_readfirstlane(grp * (kBlock / kWave) + wib);
+  const bool live = t < M.nElems;                                 // wave-uniform
+  const int te = live ? t : 0;
+  double (*L)[8] = lds[wib];
+  const double bx = M.bary[te * 3], by = M.bary[te * 3 + 1], bz = M.bary[te * 3 + 2];
+  const double R2i0 = A.r2_sf_inv;
+  double mx[4], my[4], mz[4], cd[4], acc[4];
+  double blo[3] = {1e300, 1e300, 1e300}, bhi[3] = {-1e300, -1e300, -1e300};
+#pragma unroll
+  for (int Q = 0; Q < 4; ++Q) {
+    const int d = 16 * Q + (lane & 15);
+    const double *xg = M.xGPe + ((size_t)te * nd + d) * 3;
+    const double dx = xg[0] - bx, dy = xg[1] - by, dz = xg[2] - bz;
+    mx[Q] = 2.0 * R2i0 * dx; my[Q] = 2.0 * R2i0 * dy; mz[Q] = 2.0 * R2i0 * dz;
+    cd[Q] = 1.0 - R2i0 * (dx * dx + dy * dy + dz * dz);
+    blo[0] = fmin(blo[0], dx); blo[1] = fmin(blo[1], dy); blo[2] = fmin(blo[2], dz);
+    bhi[0] = fmax(bhi[0], dx); bhi[1] = fmax(bhi[1], dy); bhi[2] = fmax(bhi[2], dz);
+    acc[Q] = 0.0;
+  }
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1)
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      blo[a] = fmin(blo[a], __shfl_xor(blo[a], off, kWave));
+      bhi[a] = fmax(bhi[a], __shfl_xor(bhi[a], off, kWave));
+    }
+  const double thr = A.r_sf + M.radius[te], thr2 = thr * thr;
+  const double R2box = A.r2_sf * (1.0 + 1e-12);
+  const int kp = lane >> 4, jc = lane & 3;
+  auto group = [&](const double (*Lh)[8], int g) {
+    const double *rec = Lh[4 * g + kp];
+    const double qx = rec[0], qy = rec[1], qz = rec[2], qq = rec[3];
+    const double bval = rec[4 + jc];
+#pragma unroll
+    for (int Q = 0; Q < 4; ++Q) {
+      const double S = clamp01(mx[Q] * qx + (my[Q] * qy + (mz[Q] * qz + (cd[Q] - qq))));
+      double S1 = S * S;
+      if (ALPHA == 0) {
+        for (int ex = 3; ex <= A.alpha; ++ex) S1 = S * S1;
+      } else {
+#pragma unroll
+        for (int ex = 3; ex <= ALPHA; ++ex) S1 = S * S1;
+      }
+      acc[Q] = __builtin_amdgcn_mfma_f64_4x4x4f64(S1, bval, acc[Q], 0, 0, 0);
+    }
+  };
+  int head = 0, tail = 0;
+  const int q0 = A.groff[grp], qend = A.groff[grp + 1];           // block-uniform
+  for (int qb = q0; qb < qend; qb += kWave) {
+    int ms0 = 0, ms1 = 0, msi = 0;
+    if (qb + lane < qend) {
+      ms0 = start[A.grfirst[qb + lane]]; ms1 = start[A.grend[qb + lane]]; msi = A.grshift[qb + lane];
+    }
+    const int jn = min(kWave, qend - qb);
+    for (int j = 0; j < jn; ++j) {
+      const int s0 = __builtin_amdgcn_readlane(ms0, j), s1 = __builtin_amdgcn_readlane(ms1, j);
+      const int si = __builtin_amdgcn_readlane(msi, j);
+      const double ox = A.sh[si][0] - bx, oy = A.sh[si][1] - by, oz = A.sh[si][2] - bz;
+      const int cnt = s1 - s0;
+      for (int base = 0; base < cnt; base += kStageCoop) {
+        // stage the chunk once for the workgroup (the previous chunk's reads
+        // ended at the barrier below)
+        if (threadIdx.x < kStageCoop && base + int(threadIdx.x) < cnt) {
+          const double4 *src = A.rec + 2 * (size_t)(s0 + base + int(threadIdx.x));
+          stg[threadIdx.x][0] = src[0];
+          stg[threadIdx.x][1] = src[1];
+        }
+        __syncthreads();
+        if (live) {
+          for (int sub = 0; sub < kStageCoop && base + sub < cnt; sub += kWave) {
+            const int k = sub + lane;
+            const bool valid = base + k < cnt;
+            const double4 ra = stg[valid ? k : 0][0], rb = stg[valid ? k : 0][1];
+            const double px = ra.x + ox, py = ra.y + oy, pz = ra.z + oz;
+            const double gx = fmax(0.0, fmax(blo[0] - px, px - bhi[0]));
+            const double gy = fmax(0.0, fmax(blo[1] - py, py - bhi[1]));
+            const double gz = fmax(0.0, fmax(blo[2] - pz, pz - bhi[2]));
+            bool pass = valid && (rb.w != 0.0) && (gx * gx + gy * gy + gz * gz <= R2box);
+            if (!LINEAR) pass = pass && !(px * px + py * py + pz * pz > thr2);
+            const unsigned long long m = __ballot(pass);
+            if (m == 0) continue;
+            if (pass) {
+              double *r = L[(tail + __popcll(m & ((1ull << lane) - 1ull))) & (kRing - 1)];
+              r[0] = px; r[1] = py; r[2] = pz;
+              r[3] = R2i0 * (px * px + py * py + pz * pz);
+              r[4] = rb.x; r[5] = rb.y; r[6] = rb.z; r[7] = rb.w;
+            }
+            tail += __popcll(m);
+            if (tail - head < kWave) continue;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            for (int g = 0; g < kWave / 4; ++g) group(L + (head & (kRing - 1)), g);
+            head += kWave;
+            __builtin_amdgcn_wave_barrier();
+          }
+        }
+        __syncthreads();
+      }
+    }
+  }
+  if (!live) return;
+  const int rest = tail - head;
+  if (rest > 0) {
+    const int ng = (rest + 3) >> 2;
+    if (lane >= rest && lane < 4 * ng) {
+      double *r = L[(head + lane) & (kRing - 1)];
+#pragma unroll
+      for (int c = 0; c < 8; ++c) r[c] = 0.0;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (int g = 0; g < ng; ++g) group(L + (head & (kRing - 1)), g);
+  }
+  const int dofLo = 4 * ((lane >> 2) & 3) + (lane >> 4);
+#pragma unroll
+  for (int Q = 0; Q < 4; ++Q) PS[((size_t)t * nd + 16 * Q + dofLo) * 4 + jc] = acc[Q];
 }
 
 // The same gather for targets whose Gauss points form an axis-aligned tensor
@@ -1757,6 +1900,7 @@ struct pic_handle {
   bool recFresh = false;  // sf.rec holds the records of the current particle arrays
   bool useGraph = false;  // replay captured two-step graphs in pic_time_step
   bool sfPersist = false; // N = 3 matrix-core gathers: persistent waves that take targets from counters
+  bool sfCoop = false;    // general matrix-core gather: a workgroup's four targets share their record loads
   int nCU = 0;            // compute units of the device
   std::vector<std::pair<const void *, unsigned>> persistGrid;   // resident blocks per persistent kernel
   hipGraph_t graph = nullptr;
@@ -1937,7 +2081,16 @@ int run_deposition(pic_handle *h, bool timed) {
       hipLaunchKernelGGL(kp, dim3(nblk), dim3(kBlock), 0, h->stream, h->M, h->start, h->sf, h->PS);         \
     }                                                                                                      \
   } while (0)
-      if (h->sf.cartesian && h->sf.alpha == 2)
+      if (h->sfCoop && !h->sf.cartesian) {
+        if (h->sf.alpha == 2 && lin)
+          hipLaunchKernelGGL((k_sf_gather_mfma_coop<2, true>), dim3(nblk), dim3(kBlock), 0, h->stream, h->M, h->start, h->sf, h->PS);
+        else if (h->sf.alpha == 2)
+          hipLaunchKernelGGL((k_sf_gather_mfma_coop<2, false>), dim3(nblk), dim3(kBlock), 0, h->stream, h->M, h->start, h->sf, h->PS);
+        else if (lin)
+          hipLaunchKernelGGL((k_sf_gather_mfma_coop<0, true>), dim3(nblk), dim3(kBlock), 0, h->stream, h->M, h->start, h->sf, h->PS);
+        else
+          hipLaunchKernelGGL((k_sf_gather_mfma_coop<0, false>), dim3(nblk), dim3(kBlock), 0, h->stream, h->M, h->start, h->sf, h->PS);
+      } else if (h->sf.cartesian && h->sf.alpha == 2)
         SF_GATHER(k_sf_gather_cart, 2);
       else if (h->sf.cartesian)
         SF_GATHER(k_sf_gather_cart, 0);
@@ -2216,6 +2369,10 @@ int pic_create(const pic_mesh_desc *mesh, const pic_params *par, pic_handle **ou
     rc |= upload(h, const_cast<int32_t **>(&h->sf.rfirst), hm.sfRunFirst);
     rc |= upload(h, const_cast<int32_t **>(&h->sf.rend), hm.sfRunEnd);
     rc |= upload(h, const_cast<uint8_t **>(&h->sf.rshift), hm.sfRunShift);
+    rc |= upload(h, const_cast<int32_t **>(&h->sf.groff), hm.sfGrpRunOff);
+    rc |= upload(h, const_cast<int32_t **>(&h->sf.grfirst), hm.sfGrpRunFirst);
+    rc |= upload(h, const_cast<int32_t **>(&h->sf.grend), hm.sfGrpRunEnd);
+    rc |= upload(h, const_cast<uint8_t **>(&h->sf.grshift), hm.sfGrpRunShift);
     for (int s = 0; s < 27; ++s)
       for (int c = 0; c < 3; ++c) h->sf.sh[s][c] = hm.shift[s][c];
     const double a = par->sf_alpha, r = par->sf_radius;
@@ -2656,6 +2813,8 @@ int pic_set_tuning(pic_handle *h, const pic_tuning *t) {
   if (!h || !t) return fail(PIC_EINVAL, "pic_set_tuning: null argument");
   if (t->sf_kernel < 0 || t->sf_kernel > 2) return fail(PIC_EINVAL, "pic_set_tuning: sf_kernel must be 0, 1 or 2");
   if (t->sf_persistent < 0 || t->sf_persistent > 1) return fail(PIC_EINVAL, "pic_set_tuning: sf_persistent must be 0 or 1");
+  if (t->sf_shared_loads < 0 || t->sf_shared_loads > 1)
+    return fail(PIC_EINVAL, "pic_set_tuning: sf_shared_loads must be 0 or 1");
   if (set_device(h)) return PIC_EHIP;
   // a captured graph holds the kernel choice it was recorded with, and the
   // record refresh before a replay follows the current choice: any change
@@ -2665,6 +2824,7 @@ int pic_set_tuning(pic_handle *h, const pic_tuning *t) {
   h->useGraph = t->use_graph != 0;
   if (t->sf_kernel != 0) h->sfMatrix = t->sf_kernel == 2;
   h->sfPersist = t->sf_persistent != 0 && h->sf.work != nullptr;
+  h->sfCoop = t->sf_shared_loads != 0 && h->sf.groff != nullptr;
   return PIC_OK;
 }
 

@@ -76,6 +76,8 @@ struct HostMesh {
   std::vector<int32_t> sfRunOff;   // [nElems+1]
   std::vector<int32_t> sfRunFirst, sfRunEnd;   // elements [first, end)
   std::vector<uint8_t> sfRunShift;
+  std::vector<int32_t> sfGrpRunOff, sfGrpRunFirst, sfGrpRunEnd;   // runs of the union of 4 consecutive targets' lists
+  std::vector<uint8_t> sfGrpRunShift;
   // the same pairs by source element: source -> (target element, shift)
   std::vector<int32_t> sfSrcOff, sfTgt;
   std::vector<uint8_t> sfTgtShift;

@@ -74,7 +74,8 @@ class PicStats(ctypes.Structure):
 
 
 class PicTuning(ctypes.Structure):
-    _fields_ = [("use_graph", ctypes.c_int32), ("sf_kernel", ctypes.c_int32), ("sf_persistent", ctypes.c_int32)]
+    _fields_ = [("use_graph", ctypes.c_int32), ("sf_kernel", ctypes.c_int32), ("sf_persistent", ctypes.c_int32),
+                ("sf_shared_loads", ctypes.c_int32)]
 
 
 # Entry points declared in include/pic_stage.h: name -> (restype, argtypes)
@@ -453,10 +454,10 @@ class PICStage:
                                         None if gv is None else _p(gv, _f64p)), "pic_adjoint")
         return float(loss[0]), gx, gv
 
-    def set_tuning(self, use_graph=False, sf_kernel=0, sf_persistent=False):
+    def set_tuning(self, use_graph=False, sf_kernel=0, sf_persistent=False, sf_shared_loads=False):
         """Execution tuning: graph replay of pic_time_step, shape-function kernel (0 default, 1 VALU, 2 matrix),
-        persistent matrix-core shape-function waves."""
-        t = PicTuning(int(bool(use_graph)), int(sf_kernel), int(bool(sf_persistent)))
+        persistent matrix-core shape-function waves, record loads shared by a workgroup's four targets."""
+        t = PicTuning(int(bool(use_graph)), int(sf_kernel), int(bool(sf_persistent)), int(bool(sf_shared_loads)))
         _chk(load_library().pic_set_tuning(self.h, ctypes.byref(t)), "pic_set_tuning")
 
     def stats(self):
