@@ -514,22 +514,57 @@ __global__ __launch_bounds__(kBlock) void k_tria_locate(DevMesh M, int64_t n, co
 // One wave per old element segment: stayers get their rank among the
 // stayers of the segment (order preserved); leavers take an arrival slot of
 // their new element (slot order fixed later by k_fix_arrivals).
+// One wave per old element segment: the particles that stay get their rank
+// among the element's stayers, ordered by 8 bins of their z across the
+// element's Gauss points and within a bin by their old order; arrivals take
+// a slot of their new element by an atomic (put in caller-id order by
+// k_fix_arrivals).  Ordered by z, the groups of 4 records the shape-function
+// gathers stage together reach the same z-planes of a target's DOFs more
+// often, and more of their plane products are skipped.
+constexpr int kZBins = 8;
 __global__ __launch_bounds__(kBlock) void k_seg_rank(int nElems, const int32_t *oldStart, const int32_t *newElem,
+                                                     const double *pz, const double *xGPe, int nDof,
                                                      int32_t *stayCnt, int32_t *arrCnt, int32_t *rank) {
   const int w = (blockIdx.x * kBlock + threadIdx.x) / kWave;
   const int lane = lane_id();
   if (w >= nElems) return;
   const int s0 = oldStart[w], s1 = oldStart[w + 1];
+  const double za = xGPe[(size_t)w * nDof * 3 + 2], zb = xGPe[((size_t)w * nDof + nDof - 1) * 3 + 2];
+  const double zlo = fmin(za, zb), zinv = zb != za ? double(kZBins) / fabs(zb - za) : 0.0;
+  auto bin_of = [&](int i) { return int(fmin(fmax((pz[i] - zlo) * zinv, 0.0), double(kZBins - 1))); };
+  const unsigned long long lt = (1ull << lane) - 1ull;
+  // pass 1: stayers per bin
+  int off[kZBins];
+#pragma unroll
+  for (int b = 0; b < kZBins; ++b) off[b] = 0;
+  for (int base = s0; base < s1; base += kWave) {
+    const int i = base + lane;
+    const bool st = i < s1 && newElem[i] == w;
+    const int bin = st ? bin_of(i) : -1;
+#pragma unroll
+    for (int b = 0; b < kZBins; ++b) off[b] += __popcll(__ballot(bin == b));
+  }
   int stay = 0;
+#pragma unroll
+  for (int b = 0; b < kZBins; ++b) {
+    const int c = off[b];
+    off[b] = stay;
+    stay += c;
+  }
+  // pass 2: ranks
   for (int base = s0; base < s1; base += kWave) {
     const int i = base + lane;
     const bool valid = i < s1;
     const int ne = valid ? newElem[i] : -2;
     const bool st = valid && ne == w;
-    const unsigned long long m = __ballot(st);
-    if (st) rank[i] = stay + __popcll(m & ((1ull << lane) - 1ull));
-    else if (valid && ne >= 0) rank[i] = atomicAdd(&arrCnt[ne], 1);
-    stay += __popcll(m);
+    const int bin = st ? bin_of(i) : -1;
+#pragma unroll
+    for (int b = 0; b < kZBins; ++b) {
+      const unsigned long long m = __ballot(bin == b);
+      if (bin == b) rank[i] = off[b] + __popcll(m & lt);
+      off[b] += __popcll(m);
+    }
+    if (valid && !st && ne >= 0) rank[i] = atomicAdd(&arrCnt[ne], 1);
   }
   if (lane == 0) stayCnt[w] = stay;
 }
@@ -2126,7 +2161,7 @@ int enqueue_sort(pic_handle *h, int64_t nGrid, bool withStaged) {
   hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(ne)), dim3(kBlock), 0, h->stream, h->arrCnt, int64_t(ne), 0);
   hipLaunchKernelGGL(k_elem_of, dim3(grid_for(int64_t(ne) * kWave)), dim3(kBlock), 0, h->stream, ne, h->start, h->elemOf);
   hipLaunchKernelGGL(k_seg_rank, dim3(grid_for(int64_t(ne) * kWave)), dim3(kBlock), 0, h->stream, ne, h->start,
-                     h->newElem, h->stayCnt, h->arrCnt, h->rank);
+                     h->newElem, h->P.z, h->M.xGPe, h->hm.nDof, h->stayCnt, h->arrCnt, h->rank);
   hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, h->stream, ne, h->stayCnt, h->arrCnt, h->startNew);
   hipLaunchKernelGGL(k_dest, dim3(grid_for(n)), dim3(kBlock), 0, h->stream, n, h->elemOf, h->newElem, h->rank,
                      h->startNew, h->stayCnt, h->perm, nOld);
