@@ -33,6 +33,7 @@ struct DevMesh {
   const double *xGPe;       // [e][nDof][3]
   const double *E;          // [e][nDof][3]
   double xiCL[2], wBaryCL[2];   // linear CL basis (corner map)
+  double cCL[2];                // its product-form Lagrange factors
   double xGP[8], wBaryGP[8];
   double cGP[8];            // 1 / prod_{m != i} (xGP_i - xGP_m): product-form Lagrange factors
   double PV[3][3];
@@ -51,6 +52,7 @@ struct DevMesh {
   double gmin[3], gmax[3];
   int dirPV[3];
   double xiCLg[3], wBaryCLg[3];  // CL basis of NGeo
+  double cCLg[3];                // its product-form Lagrange factors
   // affine elements (parallelepipeds): corner-map centre and inverse
   // Jacobian, [e][12] = c(3), A^-1 (3x3 row-major); null unless every
   // element is affine
@@ -160,7 +162,7 @@ __device__ inline bool ref_elem_newton(const DevMesh &M, int e, const double xin
   double L[3][2], F[3], xo[3];
   bool ok = true;
 #pragma unroll
-  for (int d = 0; d < 3; ++d) lagrange<2>(xi[d], M.xiCL, M.wBaryCL, L[d]);
+  for (int d = 0; d < 3; ++d) lagrange_prod<2>(xi[d], M.xiCL, M.cCL, L[d]);
 #pragma unroll
   for (int c = 0; c < 3; ++c) F[c] = -xin[c];
 #pragma unroll
@@ -225,7 +227,7 @@ __device__ inline bool ref_elem_newton(const DevMesh &M, int e, const double xin
 #pragma unroll
       for (int d = 0; d < 3; ++d) xi[d] = xo[d] - lambda * dxi[d];
 #pragma unroll
-      for (int d = 0; d < 3; ++d) lagrange<2>(xi[d], M.xiCL, M.wBaryCL, L[d]);
+      for (int d = 0; d < 3; ++d) lagrange_prod<2>(xi[d], M.xiCL, M.cCL, L[d]);
 #pragma unroll
       for (int c = 0; c < 3; ++c) F[c] = -xin[c];
 #pragma unroll

@@ -1,6 +1,8 @@
 // Device routines of reference-mapping tracking (RefMapping) on linear and
 // curved (NGeo = 2) hexahedra.  Every routine restates one routine of the
-// reference (paths relative to its src/) with the same operation order.
+// reference (paths relative to its src/) with the same operation order, but
+// for the Lagrange basis of the map, formed in product form (lagrange_prod:
+// the same polynomials, no division per evaluation; round-off apart).
 #pragma once
 
 #include "pic_device.h"
@@ -13,11 +15,11 @@ namespace pic {
 // back-tracking until |dxi|^2 <= RefMappingEps; leaves the loop when a
 // component of xi exceeds 1.5.
 template <int G>
-__device__ inline void newton_map(const double *X, const double *dX, const double *xiCL, const double *wb,
+__device__ inline void newton_map(const double *X, const double *dX, const double *xiCL, const double *cf,
                                   double refEps, const double xin[3], double xi[3]) {
   double L[3][G], F[3], xo[3];
 #pragma unroll
-  for (int d = 0; d < 3; ++d) lagrange<G>(xi[d], xiCL, wb, L[d]);
+  for (int d = 0; d < 3; ++d) lagrange_prod<G>(xi[d], xiCL, cf, L[d]);
 #pragma unroll
   for (int c = 0; c < 3; ++c) F[c] = -xin[c];
 #pragma unroll
@@ -80,7 +82,7 @@ __device__ inline void newton_map(const double *X, const double *dX, const doubl
 #pragma unroll
       for (int d = 0; d < 3; ++d) xi[d] = xo[d] - lambda * dxi[d];
 #pragma unroll
-      for (int d = 0; d < 3; ++d) lagrange<G>(xi[d], xiCL, wb, L[d]);
+      for (int d = 0; d < 3; ++d) lagrange_prod<G>(xi[d], xiCL, cf, L[d]);
 #pragma unroll
       for (int c = 0; c < 3; ++c) F[c] = -xin[c];
 #pragma unroll 1
@@ -133,12 +135,12 @@ __device__ inline void map_s(const double *X, const double L[3][G], const double
 }
 
 template <int G>
-__device__ inline void newton_map_s(const double *X, const double *dX, const double *xiCL, const double *wb,
+__device__ inline void newton_map_s(const double *X, const double *dX, const double *xiCL, const double *cf,
                                     double refEps, const double xin[3], double xi[3]) {
   constexpr int NJ = G * G * G * 9;
   double L[3][G], F[3], xo[3];
 #pragma unroll
-  for (int d = 0; d < 3; ++d) lagrange<G>(xi[d], xiCL, wb, L[d]);
+  for (int d = 0; d < 3; ++d) lagrange_prod<G>(xi[d], xiCL, cf, L[d]);
   map_s<G, true>(X, L, xin, F);
   double deltaXi2 = (fabs(F[0]) < kEpsMach && fabs(F[1]) < kEpsMach && fabs(F[2]) < kEpsMach) ? 0.0 : 1.0;
   double NormF = F[0] * F[0] + F[1] * F[1] + F[2] * F[2];
@@ -191,7 +193,7 @@ __device__ inline void newton_map_s(const double *X, const double *dX, const dou
 #pragma unroll
       for (int d = 0; d < 3; ++d) xi[d] = xo[d] - lambda * dxi[d];
 #pragma unroll
-      for (int d = 0; d < 3; ++d) lagrange<G>(xi[d], xiCL, wb, L[d]);
+      for (int d = 0; d < 3; ++d) lagrange_prod<G>(xi[d], xiCL, cf, L[d]);
       map_s<G, false>(X, L, xin, F);
       lambda = 0.2 * lambda;
       iArmijo = iArmijo + 1;
@@ -238,10 +240,10 @@ template <int N, int G>
 __device__ inline void rm_position(const DevMesh &M, int e, const double x[3], double xi[3]) {
   rm_start<N, G>(M, e, x, xi);
   if (G == 2 || M.curved[e])
-    newton_map<G>(M.XCL + (size_t)e * G * G * G * 3, M.dXCLg + (size_t)e * G * G * G * 9, M.xiCLg, M.wBaryCLg,
+    newton_map<G>(M.XCL + (size_t)e * G * G * G * 3, M.dXCLg + (size_t)e * G * G * G * 9, M.xiCLg, M.cCLg,
                   M.refEps, x, xi);
   else
-    newton_map<2>(M.corner + (size_t)e * 24, M.dXCL + (size_t)e * 72, M.xiCL, M.wBaryCL, M.refEps, x, xi);
+    newton_map<2>(M.corner + (size_t)e * 24, M.dXCL + (size_t)e * 72, M.xiCL, M.cCL, M.refEps, x, xi);
 }
 
 __device__ inline double max_abs3(const double v[3]) { return fmax(fabs(v[0]), fmax(fabs(v[1]), fabs(v[2]))); }
@@ -313,10 +315,10 @@ template <int N, int G>
 __device__ inline void rm_position_s(const DevMesh &M, int e, const double x[3], double xi[3]) {
   rm_start<N, G, true>(M, e, x, xi);
   if (G == 2 || M.curved[e])
-    newton_map_s<G>(M.XCL + (size_t)e * G * G * G * 3, M.dXCLg + (size_t)e * G * G * G * 9, M.xiCLg, M.wBaryCLg,
+    newton_map_s<G>(M.XCL + (size_t)e * G * G * G * 3, M.dXCLg + (size_t)e * G * G * G * 9, M.xiCLg, M.cCLg,
                     M.refEps, x, xi);
   else
-    newton_map_s<2>(M.corner + (size_t)e * 24, M.dXCL + (size_t)e * 72, M.xiCL, M.wBaryCL, M.refEps, x, xi);
+    newton_map_s<2>(M.corner + (size_t)e * 24, M.dXCL + (size_t)e * 72, M.xiCL, M.cCL, M.refEps, x, xi);
 }
 
 template <int N, int G>

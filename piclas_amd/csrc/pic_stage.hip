@@ -2326,8 +2326,16 @@ int pic_create(const pic_mesh_desc *mesh, const pic_params *par, pic_handle **ou
   M.E = h->E;
   M.xiCL[0] = -1.0; M.xiCL[1] = 1.0;      // linear CL basis of the corner map
   M.wBaryCL[0] = -0.5; M.wBaryCL[1] = 0.5;
+  M.cCL[0] = -0.5; M.cCL[1] = 0.5;        // 1 / (xi_0 - xi_1), 1 / (xi_1 - xi_0)
   M.NGeo = hm.NGeo; M.nGP = hm.nGP; M.tracking = par->tracking; M.nPV = hm.nPV;
-  for (int i = 0; i <= hm.NGeo; ++i) { M.xiCLg[i] = hm.xiCL[i]; M.wBaryCLg[i] = hm.wBaryCL[i]; }
+  for (int i = 0; i <= hm.NGeo; ++i) {
+    M.xiCLg[i] = hm.xiCL[i];
+    M.wBaryCLg[i] = hm.wBaryCL[i];
+    double p = 1.0;
+    for (int m = 0; m <= hm.NGeo; ++m)
+      if (m != i) p *= hm.xiCL[i] - hm.xiCL[m];
+    M.cCLg[i] = 1.0 / p;
+  }
   for (int c = 0; c < 3; ++c) {
     M.gmin[c] = hm.gmin[c]; M.gmax[c] = hm.gmax[c]; M.dirPV[c] = hm.dirPV[c];
     M.bmin[c] = hm.bmin[c]; M.bdelta[c] = hm.bdelta[c]; M.nb[c] = hm.nb[c];
