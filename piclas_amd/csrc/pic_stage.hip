@@ -22,9 +22,6 @@
 #include "pic_device.h"
 #include "pic_refmap.h"
 #include "stage_internal.h"
-#ifndef SF_QSKIP
-#define SF_QSKIP 2
-#endif
 
 #pragma clang fp contract(off)
 
@@ -1088,12 +1085,8 @@ __device__ __attribute__((always_inline)) inline void sf_gather_mfma_target(cons
       // the run's records from a wave-uniform base, addressed by 32-bit offsets
       const char *rr = reinterpret_cast<const char *>(A.rec + 2 * (size_t)s0);
       const int last = s1 - 1 - s0;
-      for (int base = 0; base <= last; base += kWave) {
-        const int k = base + lane;
-        const bool valid = k <= last;
-        const unsigned o = unsigned(valid ? k : last) * 64u;
-        const double4 ra = *reinterpret_cast<const double4 *>(rr + o);
-        const double4 rb = *reinterpret_cast<const double4 *>(rr + o + 32u);
+      // cull, stage and (64 pending) consume one chunk of 64 records
+      auto chunk = [&](const double4 &ra, const double4 &rb, bool valid) {
         const double px = ra.x + ox, py = ra.y + oy, pz = ra.z + oz;
         // the exact test that some DOF of the target lies within r_sf, and on
         // curved elements the reference's element cull |x - bary| > r_sf + ElemRadius
@@ -1105,7 +1098,7 @@ __device__ __attribute__((always_inline)) inline void sf_gather_mfma_target(cons
         // barycentre, so the box test implies the cull)
         if (!LINEAR) pass = pass && !(px * px + py * py + pz * pz > thr2);
         const unsigned long long m = __ballot(pass);
-        if (m == 0) continue;
+        if (m == 0) return;
         if (pass) {
           double *r = L[(tail + __popcll(m & ((1ull << lane) - 1ull))) & (kRing - 1)];
           r[0] = px; r[1] = py; r[2] = pz;
@@ -1113,13 +1106,25 @@ __device__ __attribute__((always_inline)) inline void sf_gather_mfma_target(cons
           r[4] = rb.x; r[5] = rb.y; r[6] = rb.z; r[7] = rb.w;
         }
         tail += __popcll(m);
-        if (tail - head < kWave) continue;
+        if (tail - head < kWave) return;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         for (int g = 0; g < kWave / 4; ++g) group(L + (head & (kRing - 1)), g);
         head += kWave;
         __builtin_amdgcn_wave_barrier();
+      };
+      // two chunks per pass, both loads issued before either is culled (the
+      // second at a clamped index when the run ends within the first)
+      for (int base = 0; base <= last; base += 2 * kWave) {
+        const int k0 = base + lane, k1 = base + kWave + lane;
+        const unsigned o0 = unsigned(min(k0, last)) * 64u, o1 = unsigned(min(k1, last)) * 64u;
+        const double4 ra0 = *reinterpret_cast<const double4 *>(rr + o0);
+        const double4 rb0 = *reinterpret_cast<const double4 *>(rr + o0 + 32u);
+        const double4 ra1 = *reinterpret_cast<const double4 *>(rr + o1);
+        const double4 rb1 = *reinterpret_cast<const double4 *>(rr + o1 + 32u);
+        chunk(ra0, rb0, k0 <= last);
+        if (base + kWave <= last) chunk(ra1, rb1, k1 <= last);
       }
     }
   }
@@ -1356,41 +1361,11 @@ __device__ __attribute__((always_inline)) inline void sf_gather_cart_target(cons
     const double *rec = Lh[4 * g + kp];
     const double W = fma(tv, rec[1], fma(tu, rec[0], Kij));
     double bval = rec[6 + jc];
-#if SF_QSKIP == 0
-#pragma unroll
-    for (int Q = 0; Q < 4; ++Q) {
-      const double S = clamp01(W - rec[2 + Q]);
-      double S1 = S * S;
-      if (ALPHA == 0) {
-        for (int ex = 3; ex <= A.alpha; ++ex) S1 = S * S1;
-      } else {
-#pragma unroll
-        for (int ex = 3; ex <= ALPHA; ++ex) S1 = S * S1;
-      }
-      acc[Q] = __builtin_amdgcn_mfma_f64_4x4x4f64(S1, bval, acc[Q], 0, 0, 0);
-    }
-#elif SF_QSKIP == 1
-    double S1[4];
-#pragma unroll
-    for (int Q = 0; Q < 4; ++Q) {
-      const double S = clamp01(W - rec[2 + Q]);
-      S1[Q] = S * S;
-      if (ALPHA == 0) {
-        for (int ex = 3; ex <= A.alpha; ++ex) S1[Q] = S * S1[Q];
-      } else {
-#pragma unroll
-        for (int ex = 3; ex <= ALPHA; ++ex) S1[Q] = S * S1[Q];
-      }
-    }
-    asm volatile("" : "+v"(S1[0]), "+v"(S1[1]), "+v"(S1[2]), "+v"(S1[3]), "+v"(bval));
-#pragma unroll
-    for (int Q = 0; Q < 4; ++Q)
-      if (((live[Q] >> (4 * g)) & 0xFull) != 0) acc[Q] = __builtin_amdgcn_mfma_f64_4x4x4f64(S1[Q], bval, acc[Q], 0, 0, 0);
-#else
+    // the four C_k and f(v,1) are loaded together (the empty asm keeps the
+    // loads ahead of the plane tests); planes no record of the group reaches
+    // are skipped
     double C[4] = {rec[2], rec[3], rec[4], rec[5]};
-#if SF_QSKIP == 2
     asm volatile("" : "+v"(C[0]), "+v"(C[1]), "+v"(C[2]), "+v"(C[3]), "+v"(bval));
-#endif
 #pragma unroll
     for (int Q = 0; Q < 4; ++Q) {
       if (((live[Q] >> (4 * g)) & 0xFull) == 0) continue;
@@ -1404,7 +1379,6 @@ __device__ __attribute__((always_inline)) inline void sf_gather_cart_target(cons
       }
       acc[Q] = __builtin_amdgcn_mfma_f64_4x4x4f64(S1, bval, acc[Q], 0, 0, 0);
     }
-#endif
   };
   // C_k = (w_k - pw)^2 + pu^2 + pv^2 of the 64 records from Lh, one per lane
   // (rows past the staged ones are overwritten harmlessly and never read),
@@ -1440,12 +1414,8 @@ __device__ __attribute__((always_inline)) inline void sf_gather_cart_target(cons
       // the run's records from a wave-uniform base, addressed by 32-bit offsets
       const char *rr = reinterpret_cast<const char *>(A.rec + 2 * (size_t)s0);
       const int last = s1 - 1 - s0;
-      for (int base = 0; base <= last; base += kWave) {
-        const int k = base + lane;
-        const bool valid = k <= last;
-        const unsigned o = unsigned(valid ? k : last) * 64u;
-        const double4 ra = *reinterpret_cast<const double4 *>(rr + o);
-        const double4 rb = *reinterpret_cast<const double4 *>(rr + o + 32u);
+      // cull, stage and (64 pending) consume one chunk of 64 records
+      auto chunk = [&](const double4 &ra, const double4 &rb, bool valid) {
         // shifted position (GetPartPosShifted) relative to the DOF-box centre, scaled
         const double pu = fma(ra.x + ox, Ri, -uc), pv = fma(ra.y + oy, Ri, -vc), pw = fma(ra.z + oz, Ri, -wc);
         const double gx = fmax(fabs(pu) - uh, 0.0);
@@ -1453,14 +1423,14 @@ __device__ __attribute__((always_inline)) inline void sf_gather_cart_target(cons
         const double gz = fmax(fabs(pw) - wh, 0.0);
         const bool pass = valid && (rb.w != 0.0) && (gx * gx + gy * gy + gz * gz <= box2);
         const unsigned long long m = __ballot(pass);
-        if (m == 0) continue;
+        if (m == 0) return;
         if (pass) {
           double *r = L[(tail + __popcll(m & ((1ull << lane) - 1ull))) & (kRing - 1)];
           r[0] = pu; r[1] = pv; r[2] = pw;
           r[6] = rb.x; r[7] = rb.y; r[8] = rb.z; r[9] = rb.w;
         }
         tail += __popcll(m);
-        if (tail - head < kWave) continue;
+        if (tail - head < kWave) return;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -1473,6 +1443,18 @@ __device__ __attribute__((always_inline)) inline void sf_gather_cart_target(cons
         for (int g = 0; g < kWave / 4; ++g) group(L + (head & (kRing - 1)), g, live);
         head += kWave;
         __builtin_amdgcn_wave_barrier();
+      };
+      // two chunks per pass, both loads issued before either is culled (the
+      // second at a clamped index when the run ends within the first)
+      for (int base = 0; base <= last; base += 2 * kWave) {
+        const int k0 = base + lane, k1 = base + kWave + lane;
+        const unsigned o0 = unsigned(min(k0, last)) * 64u, o1 = unsigned(min(k1, last)) * 64u;
+        const double4 ra0 = *reinterpret_cast<const double4 *>(rr + o0);
+        const double4 rb0 = *reinterpret_cast<const double4 *>(rr + o0 + 32u);
+        const double4 ra1 = *reinterpret_cast<const double4 *>(rr + o1);
+        const double4 rb1 = *reinterpret_cast<const double4 *>(rr + o1 + 32u);
+        chunk(ra0, rb0, k0 <= last);
+        if (base + kWave <= last) chunk(ra1, rb1, k1 <= last);
       }
     }
   }
