@@ -204,7 +204,22 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE_PUSH
       P.vx[i] = v[0]; P.vy[i] = v[1]; P.vz[i] = v[2];
       P.isNew[i] = 0;
       double det[6][2];
-      const bool in = uni ? inside_quad<true>(M, eu, x, det) : inside_quad(M, e0, x, det);
+      bool in;
+      if (AFFINE) {
+        // On a parallelepiped (corner map affine to 1e-14) ParticleInsideQuad3D
+        // decides as |xi| <= 1 does, xi = A^-1 (x - c), except within rounding
+        // of a face: the reference's side test runs only for particles with
+        // max |xi| within 1e-10 of 1, the others are decided by xi.
+        double xi[3];
+        if (uni) affine_xi<true>(M, eu, x, xi);
+        else affine_xi(M, e0, x, xi);
+        const double mx = max_abs3(xi);
+        if (mx < 1.0 - 1e-10) in = true;
+        else if (mx > 1.0 + 1e-10) in = false;
+        else in = uni ? inside_quad<true>(M, eu, x, det) : inside_quad(M, e0, x, det);
+      } else {
+        in = uni ? inside_quad<true>(M, eu, x, det) : inside_quad(M, e0, x, det);
+      }
       if (in) newElem[i] = e0;
       else mover = true;
     }
