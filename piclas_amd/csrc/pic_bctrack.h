@@ -23,6 +23,11 @@
 
 namespace pic {
 
+// Curved and planar-curved boundary sides exist only on curved elements, and
+// RefMapping runs on NGeo = 1 or 2: the Bezier routines below work on
+// patches of degree 2, fixed at compile time so that the patch loops unroll
+// and the patches stay in registers.
+constexpr int kBzNG = 2;
 constexpr int kBzMaxInter = 6;     // BezierClipMaxIntersec = 2 (NGeo + 1)
 constexpr int kBzClipMax = 100;    // BezierClipMaxIter
 constexpr int kBzArr = 24;         // clip levels kept per direction (beyond: error)
@@ -177,7 +182,7 @@ struct BzCtx {
 __device__ inline void bz_intersection_point(const BzCtx &C, int nXi, int nEta, int &nInter) {
   const DevMesh &M = *C.M;
   BzWork &W = *C.W;
-  const int ng = M.NGeo, g1 = ng + 1;
+  constexpr int ng = kBzNG, g1 = ng + 1;
   double Xi, Eta;
   if (nXi == 0) Xi = 0.;
   else {
@@ -324,7 +329,7 @@ struct BzPending {
 // BezierClipRecursive + CheckXiClip / CheckEtaClip, depth first, with the
 // second halves of split patches on a stack
 __device__ __attribute__((noinline)) void bz_clip(const BzCtx &C, BzState st, int &nInter) {
-  const int ng = C.M->NGeo;
+  constexpr int ng = kBzNG;
   BzWork &W = *C.W;
   BzPending stack[kBzStack];
   int sp = 0;
@@ -423,7 +428,7 @@ __device__ __attribute__((noinline)) void bz_clip(const BzCtx &C, BzState st, in
 // derivative patch D_Bezier (BuildBezierDMat, basis.f90:361-410) on the CL nodes
 __device__ inline double bz_newton(const DevMesh &M, int b, double Xi[2], const BzPatch &B, const double last[3],
                                    const double traj[3], double len, bool &failed) {
-  const int ng = M.NGeo, g1 = ng + 1;
+  constexpr int ng = kBzNG, g1 = ng + 1;
   failed = false;
   double MinMax[2][2];
   bool hasInter = true;
@@ -513,7 +518,7 @@ __device__ inline double bz_newton(const DevMesh &M, int b, double Xi[2], const 
 
 __device__ inline void bz_project(const DevMesh &M, int b, const double last[3], const double na[3], const double nb[3],
                                   BzPatch &B) {
-  const int ng = M.NGeo;
+  constexpr int ng = kBzNG;
   for (int q = 0; q <= ng; ++q)
     for (int p = 0; p <= ng; ++p) {
       const double *bp = bz_pt(M, b, p, q);
@@ -601,7 +606,7 @@ __device__ inline int bc_planar_curved(const DevMesh &M, int b, int flip, const 
   if (failed) {
     // BezierClipLocalTol is a stale module value on this path in the
     // reference; the patch's own width is used
-    W.localTol = fmin(bz_patch_width(M.NGeo, B, 0), bz_patch_width(M.NGeo, B, 1)) * kBzClipTol;
+    W.localTol = fmin(bz_patch_width(kBzNG, B, 0), bz_patch_width(kBzNG, B, 1)) * kBzClipTol;
     W.locAlpha[0] = -1.0;
     BzState st;
     st.B = B;
@@ -636,7 +641,7 @@ __device__ inline int bc_curved(const DevMesh &M, int b, int flip, const double 
   BzWork W;
   W.err = 0;
   for (int k = 0; k < kBzMaxInter; ++k) W.locAlpha[k] = -1.0;
-  const double dXi = bz_patch_width(M.NGeo, B, 0), dEta = bz_patch_width(M.NGeo, B, 1);
+  const double dXi = bz_patch_width(kBzNG, B, 0), dEta = bz_patch_width(kBzNG, B, 1);
   W.localTol = fmin(dXi, dEta) * kBzClipTol;
   BzState st;
   st.B = B;
@@ -684,8 +689,12 @@ __device__ inline int bc_side_intersection(const DevMesh &M, int b, const double
   const int flip = M.bcFlip[b];
   switch (M.sideType[b]) {
     case kSidePlanarRect: return bc_planar_rect(M, b, flip, last, traj, len, alpha, xi, eta);
-    case kSidePlanarCurved: return bc_planar_curved(M, b, flip, last, traj, len, alpha, xi, eta);
-    case kSideCurved: return bc_curved(M, b, flip, last, x, traj, len, alpha, xi, eta);
+    case kSidePlanarCurved:
+      if (M.NGeo != kBzNG) { alpha = -1.0; return kBcUnsupported; }
+      return bc_planar_curved(M, b, flip, last, traj, len, alpha, xi, eta);
+    case kSideCurved:
+      if (M.NGeo != kBzNG) { alpha = -1.0; return kBcUnsupported; }
+      return bc_curved(M, b, flip, last, x, traj, len, alpha, xi, eta);
     default: alpha = -1.0; return kBcUnsupported;
   }
 }
@@ -743,10 +752,10 @@ __device__ inline bool bc_interaction(const DevMesh &M, int b, int &ElemID, doub
                                       bool &removed) {
   double n[3];
   if (M.sideType[b] == kSideCurved) {
-    const int g1 = M.NGeo + 1;
+    constexpr int g1 = kBzNG + 1;
     double G[2][3];
     const double X[2] = {xi, eta};
-    bz_eval(M.NGeo, 3, M.bz + size_t(b) * g1 * g1 * 3, X, nullptr, G);
+    bz_eval(kBzNG, 3, M.bz + size_t(b) * g1 * g1 * 3, X, nullptr, G);
     bz_crossnorm(G[0], G[1], n);
   } else {
     const double *sn = M.sideNorm + size_t(b) * 3;
