@@ -685,7 +685,8 @@ struct SpeciesFactor {
   double fsp[8];
 };
 __global__ __launch_bounds__(kBlock) void k_gather_state(const int32_t *total, const int32_t *perm, DevParts s, DevParts d,
-                                                         const int32_t *newElem, SpeciesFactor sf, double4 *rec) {
+                                                         const int32_t *newElem, SpeciesFactor sf, double4 *rec,
+                                                         double4 *recB) {
   const int64_t j = int64_t(blockIdx.x) * kBlock + threadIdx.x;
   if (j >= *total) return;
   const int32_t i = perm[j];
@@ -698,8 +699,8 @@ __global__ __launch_bounds__(kBlock) void k_gather_state(const int32_t *total, c
   if (s.rx) { d.rx[j] = s.rx[i]; d.ry[j] = s.ry[i]; d.rz[j] = s.rz[i]; }
   if (rec) {
     const double f = sf.fsp[sp];
-    rec[2 * j] = make_double4(x, y, z, 0.0);
-    rec[2 * j + 1] = make_double4(vx * f, vy * f, vz * f, f);
+    rec[j] = make_double4(x, y, z, f);
+    recB[j] = make_double4(vx * f, vy * f, vz * f, f);
   }
 }
 
@@ -848,7 +849,10 @@ struct SFArgs {
   double sh[27][3];
   double r_sf, r2_sf, r2_sf_inv, w_sf;
   double fsp[8];   // per species: ChargeIC * MacroParticleFactor * w_sf
-  const double4 *rec;   // per sorted particle: (x, y, z, 0), (f vx, f vy, f vz, f)
+  // per sorted particle, in two planes: rec[i] = (x, y, z, f) (what the culls
+  // read) and recB[i] = (f vx, f vy, f vz, f) (read for the staged records)
+  const double4 *rec;
+  const double4 *recB;
   double r_inv;         // 1 / r_sf
   int nspec;
   int cartesian;        // every element's Gauss points form an axis-aligned tensor grid
@@ -858,6 +862,12 @@ struct SFArgs {
   const int32_t *groff, *grfirst, *grend;   // runs of the union of the lists of 4 consecutive targets
   const uint8_t *grshift;
 };
+
+// record c of a run whose plane starts at the wave-uniform base b: a 32-bit
+// offset from a scalar base (the saddr + voffset form of the load)
+__device__ inline double4 rec_at(const double4 *b, int c) {
+  return *reinterpret_cast<const double4 *>(reinterpret_cast<const char *>(b) + unsigned(c) * 32u);
+}
 
 template <int N, int ALPHA>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) void k_sf_gather(DevMesh M, DevParts P, const int32_t *start, SFArgs A, double *PS) {
@@ -998,13 +1008,15 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
 // Deposition records of the sorted particles for the shape-function
 // gathers: (x, y, z, 0), (f vx, f vy, f vz, f) with f = ChargeIC *
 // MacroParticleFactor * w_sf of the particle's species, so that a gather
-// reads one 64-byte record per particle and target.
-__global__ __launch_bounds__(kBlock) void k_sf_records(DevParts P, const int32_t *count, SFArgs A, double4 *rec) {
+// culls on one 32-byte record per particle and target and reads the second
+// only for the particles it keeps.
+__global__ __launch_bounds__(kBlock) void k_sf_records(DevParts P, const int32_t *count, SFArgs A, double4 *rec,
+                                                       double4 *recB) {
   const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
   if (i >= *count) return;
   const double f = A.fsp[P.spec[i]];
-  rec[2 * i] = make_double4(P.x[i], P.y[i], P.z[i], 0.0);
-  rec[2 * i + 1] = make_double4(P.vx[i] * f, P.vy[i] * f, P.vz[i] * f, f);
+  rec[i] = make_double4(P.x[i], P.y[i], P.z[i], f);
+  recB[i] = make_double4(P.vx[i] * f, P.vy[i] * f, P.vz[i] * f, f);
 }
 
 // The next target of a persistent gather wave: targets are taken in
@@ -1097,21 +1109,24 @@ __device__ __attribute__((always_inline)) inline void sf_gather_mfma_target(cons
       const int si = __builtin_amdgcn_readlane(msi, j);
       // shifted position (GetPartPosShifted) relative to the target barycentre
       const double ox = A.sh[si][0] - bx, oy = A.sh[si][1] - by, oz = A.sh[si][2] - bz;
-      // the run's records from a wave-uniform base, addressed by 32-bit offsets
-      const char *rr = reinterpret_cast<const char *>(A.rec + 2 * (size_t)s0);
+      // the run's records from a wave-uniform base
+      const double4 *ra_s = A.rec + s0, *rb_s = A.recB + s0;
       const int last = s1 - 1 - s0;
-      // cull, stage and (64 pending) consume one chunk of 64 records
-      auto chunk = [&](const double4 &ra, const double4 &rb, bool valid) {
-        const double px = ra.x + ox, py = ra.y + oy, pz = ra.z + oz;
-        // the exact test that some DOF of the target lies within r_sf, and on
-        // curved elements the reference's element cull |x - bary| > r_sf + ElemRadius
+      // the exact test that some DOF of the target lies within r_sf, and on
+      // curved elements the reference's element cull |x - bary| > r_sf + ElemRadius
+      auto cull = [&](const double4 &ra, bool valid, double &px, double &py, double &pz) {
+        px = ra.x + ox; py = ra.y + oy; pz = ra.z + oz;
         const double gx = fmax(0.0, fmax(blo[0] - px, px - bhi[0]));
         const double gy = fmax(0.0, fmax(blo[1] - py, py - bhi[1]));
         const double gz = fmax(0.0, fmax(blo[2] - pz, pz - bhi[2]));
-        bool pass = valid && (rb.w != 0.0) && (gx * gx + gy * gy + gz * gz <= R2box);
+        bool pass = valid && (ra.w != 0.0) && (gx * gx + gy * gy + gz * gz <= R2box);
         // (on linear elements every DOF lies within ElemRadius of the
         // barycentre, so the box test implies the cull)
         if (!LINEAR) pass = pass && !(px * px + py * py + pz * pz > thr2);
+        return pass;
+      };
+      // stage the kept records of one chunk and consume 64 once pending
+      auto stage = [&](bool pass, double px, double py, double pz, const double4 &rb) {
         const unsigned long long m = __ballot(pass);
         if (m == 0) return;
         if (pass) {
@@ -1129,17 +1144,22 @@ __device__ __attribute__((always_inline)) inline void sf_gather_mfma_target(cons
         head += kWave;
         __builtin_amdgcn_wave_barrier();
       };
-      // two chunks per pass, both loads issued before either is culled (the
-      // second at a clamped index when the run ends within the first)
+      // two chunks of 64 per pass: both position records loaded before
+      // either is culled (the second at a clamped index when the run ends
+      // within the first), then the current records of the kept particles only
       for (int base = 0; base <= last; base += 2 * kWave) {
         const int k0 = base + lane, k1 = base + kWave + lane;
-        const unsigned o0 = unsigned(min(k0, last)) * 64u, o1 = unsigned(min(k1, last)) * 64u;
-        const double4 ra0 = *reinterpret_cast<const double4 *>(rr + o0);
-        const double4 rb0 = *reinterpret_cast<const double4 *>(rr + o0 + 32u);
-        const double4 ra1 = *reinterpret_cast<const double4 *>(rr + o1);
-        const double4 rb1 = *reinterpret_cast<const double4 *>(rr + o1 + 32u);
-        chunk(ra0, rb0, k0 <= last);
-        if (base + kWave <= last) chunk(ra1, rb1, k1 <= last);
+        const bool two = base + kWave <= last;
+        const double4 ra0 = rec_at(ra_s, min(k0, last));
+        const double4 ra1 = rec_at(ra_s, min(k1, last));
+        double px0, py0, pz0, px1, py1, pz1;
+        const bool p0 = cull(ra0, k0 <= last, px0, py0, pz0);
+        const bool p1 = cull(ra1, two && k1 <= last, px1, py1, pz1);
+        double4 rb0 = make_double4(0.0, 0.0, 0.0, 0.0), rb1 = rb0;
+        if (p0) rb0 = rec_at(rb_s, k0);
+        if (p1) rb1 = rec_at(rb_s, k1);
+        stage(p0, px0, py0, pz0, rb0);
+        if (two) stage(p1, px1, py1, pz1, rb1);
       }
     }
   }
@@ -1266,9 +1286,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE_MX))
         // stage the chunk once for the workgroup (the previous chunk's reads
         // ended at the barrier below)
         if (threadIdx.x < kStageCoop && base + int(threadIdx.x) < cnt) {
-          const double4 *src = A.rec + 2 * (size_t)(s0 + base + int(threadIdx.x));
-          stg[threadIdx.x][0] = src[0];
-          stg[threadIdx.x][1] = src[1];
+          const size_t r = size_t(s0 + base + int(threadIdx.x));
+          stg[threadIdx.x][0] = A.rec[r];
+          stg[threadIdx.x][1] = A.recB[r];
         }
         __syncthreads();
         if (live) {
@@ -1280,7 +1300,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE_MX))
             const double gx = fmax(0.0, fmax(blo[0] - px, px - bhi[0]));
             const double gy = fmax(0.0, fmax(blo[1] - py, py - bhi[1]));
             const double gz = fmax(0.0, fmax(blo[2] - pz, pz - bhi[2]));
-            bool pass = valid && (rb.w != 0.0) && (gx * gx + gy * gy + gz * gz <= R2box);
+            bool pass = valid && (ra.w != 0.0) && (gx * gx + gy * gy + gz * gz <= R2box);
             if (!LINEAR) pass = pass && !(px * px + py * py + pz * pz > thr2);
             const unsigned long long m = __ballot(pass);
             if (m == 0) continue;
@@ -1426,8 +1446,8 @@ __device__ __attribute__((always_inline)) inline void sf_gather_cart_target(cons
       const int s0 = __builtin_amdgcn_readlane(ms0, j), s1 = __builtin_amdgcn_readlane(ms1, j);
       const int si = __builtin_amdgcn_readlane(msi, j);
       const double ox = A.sh[si][0] - bx, oy = A.sh[si][1] - by, oz = A.sh[si][2] - bz;
-      // the run's records from a wave-uniform base, addressed by 32-bit offsets
-      const char *rr = reinterpret_cast<const char *>(A.rec + 2 * (size_t)s0);
+      // the run's records from a wave-uniform base
+      const double4 *ra_s = A.rec + s0, *rb_s = A.recB + s0;
       const int last = s1 - 1 - s0;
       // cull, stage and (64 pending) consume one chunk of 64 records
       auto chunk = [&](const double4 &ra, const double4 &rb, bool valid) {
@@ -1436,7 +1456,7 @@ __device__ __attribute__((always_inline)) inline void sf_gather_cart_target(cons
         const double gx = fmax(fabs(pu) - uh, 0.0);
         const double gy = fmax(fabs(pv) - vh, 0.0);
         const double gz = fmax(fabs(pw) - wh, 0.0);
-        const bool pass = valid && (rb.w != 0.0) && (gx * gx + gy * gy + gz * gz <= box2);
+        const bool pass = valid && (ra.w != 0.0) && (gx * gx + gy * gy + gz * gz <= box2);
         const unsigned long long m = __ballot(pass);
         if (m == 0) return;
         if (pass) {
@@ -1463,11 +1483,9 @@ __device__ __attribute__((always_inline)) inline void sf_gather_cart_target(cons
       // second at a clamped index when the run ends within the first)
       for (int base = 0; base <= last; base += 2 * kWave) {
         const int k0 = base + lane, k1 = base + kWave + lane;
-        const unsigned o0 = unsigned(min(k0, last)) * 64u, o1 = unsigned(min(k1, last)) * 64u;
-        const double4 ra0 = *reinterpret_cast<const double4 *>(rr + o0);
-        const double4 rb0 = *reinterpret_cast<const double4 *>(rr + o0 + 32u);
-        const double4 ra1 = *reinterpret_cast<const double4 *>(rr + o1);
-        const double4 rb1 = *reinterpret_cast<const double4 *>(rr + o1 + 32u);
+        const int c0 = min(k0, last), c1 = min(k1, last);
+        const double4 ra0 = rec_at(ra_s, c0), rb0 = rec_at(rb_s, c0);
+        const double4 ra1 = rec_at(ra_s, c1), rb1 = rec_at(rb_s, c1);
         chunk(ra0, rb0, k0 <= last);
         if (base + kWave <= last) chunk(ra1, rb1, k1 <= last);
       }
@@ -1599,20 +1617,19 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE_MX))
       const int s0 = __builtin_amdgcn_readlane(ms0, j), s1 = __builtin_amdgcn_readlane(ms1, j);
       const int si = __builtin_amdgcn_readlane(msi, j);
       const double ox = A.sh[si][0] - bx, oy = A.sh[si][1] - by, oz = A.sh[si][2] - bz;
-      // the run's records from a wave-uniform base, addressed by 32-bit offsets
-      const char *rr = reinterpret_cast<const char *>(A.rec + 2 * (size_t)s0);
+      // the run's records from a wave-uniform base
+      const double4 *ra_s = A.rec + s0, *rb_s = A.recB + s0;
       const int last = s1 - 1 - s0;
       for (int base = 0; base <= last; base += kWave) {
         const int k = base + lane;
         const bool valid = k <= last;
-        const unsigned o = unsigned(valid ? k : last) * 64u;
-        const double4 ra = *reinterpret_cast<const double4 *>(rr + o);
-        const double4 rb = *reinterpret_cast<const double4 *>(rr + o + 32u);
+        const int c = valid ? k : last;
+        const double4 ra = rec_at(ra_s, c), rb = rec_at(rb_s, c);
         const double px = ra.x + ox, py = ra.y + oy, pz = ra.z + oz;
         const double gx = fmax(0.0, fmax(blo[0] - px, px - bhi[0]));
         const double gy = fmax(0.0, fmax(blo[1] - py, py - bhi[1]));
         const double gz = fmax(0.0, fmax(blo[2] - pz, pz - bhi[2]));
-        bool pass = valid && (rb.w != 0.0) && (gx * gx + gy * gy + gz * gz <= box2);
+        bool pass = valid && (ra.w != 0.0) && (gx * gx + gy * gy + gz * gz <= box2);
         if (!A.linear) pass = pass && !(px * px + py * py + pz * pz > thr2);
         const unsigned long long m = __ballot(pass);
         if (m == 0) continue;
@@ -1754,19 +1771,18 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE_MX))
       const int s0 = __builtin_amdgcn_readlane(ms0, j), s1 = __builtin_amdgcn_readlane(ms1, j);
       const int si = __builtin_amdgcn_readlane(msi, j);
       const double ox = A.sh[si][0] - bx, oy = A.sh[si][1] - by, oz = A.sh[si][2] - bz;
-      const char *rr = reinterpret_cast<const char *>(A.rec + 2 * (size_t)s0);
+      const double4 *ra_s = A.rec + s0, *rb_s = A.recB + s0;
       const int last = s1 - 1 - s0;
       for (int base = 0; base <= last; base += kWave) {
         const int k = base + lane;
         const bool valid = k <= last;
-        const unsigned o = unsigned(valid ? k : last) * 64u;
-        const double4 ra = *reinterpret_cast<const double4 *>(rr + o);
-        const double4 rb = *reinterpret_cast<const double4 *>(rr + o + 32u);
+        const int c = valid ? k : last;
+        const double4 ra = rec_at(ra_s, c), rb = rec_at(rb_s, c);
         const double pu = fma(ra.x + ox, Ri, -uc), pv = fma(ra.y + oy, Ri, -vc), pw = fma(ra.z + oz, Ri, -wc);
         const double gx = fmax(fabs(pu) - uh, 0.0);
         const double gy = fmax(fabs(pv) - vh, 0.0);
         const double gz = fmax(fabs(pw) - wh, 0.0);
-        const bool pass = valid && (rb.w != 0.0) && (gx * gx + gy * gy + gz * gz <= box2);
+        const bool pass = valid && (ra.w != 0.0) && (gx * gx + gy * gy + gz * gz <= box2);
         const unsigned long long m = __ballot(pass);
         if (m == 0) continue;
         if (pass) {
@@ -2082,7 +2098,7 @@ int run_deposition(pic_handle *h, bool timed) {
       // particle arrays they are formed here
       if (!h->recFresh)
         hipLaunchKernelGGL(k_sf_records, dim3(grid_for(std::max<int64_t>(h->n, 1))), dim3(kBlock), 0, h->stream, h->P,
-                           h->start + ne, h->sf, const_cast<double4 *>(h->sf.rec));
+                           h->start + ne, h->sf, const_cast<double4 *>(h->sf.rec), const_cast<double4 *>(h->sf.recB));
       h->recFresh = true;
       if (timed) HIPCHK(hipEventRecord(h->ev[8], h->stream));
     }
@@ -2170,7 +2186,8 @@ int enqueue_sort(pic_handle *h, int64_t nGrid, bool withStaged) {
   SpeciesFactor sfac{};
   for (int k = 0; k < 8; ++k) sfac.fsp[k] = h->sf.fsp[k];
   hipLaunchKernelGGL(k_gather_state, dim3(grid_for(n)), dim3(kBlock), 0, h->stream, total, h->perm, h->P, h->Q, h->newElem,
-                     sfac, useRec ? const_cast<double4 *>(h->sf.rec) : nullptr);
+                     sfac, useRec ? const_cast<double4 *>(h->sf.rec) : nullptr,
+                     useRec ? const_cast<double4 *>(h->sf.recB) : nullptr);
   if (withStaged) {
     // keep LastPartPos / LastElem / FieldAtParticle in the new order
     hipLaunchKernelGGL(k_gather<double>, dim3(grid_for(n)), dim3(kBlock), 0, h->stream, total, h->perm, h->P.lx, h->Q.lx, 1);
@@ -2464,6 +2481,7 @@ int pic_create(const pic_mesh_desc *mesh, const pic_params *par, pic_handle **ou
     double4 *rec = nullptr;
     rc |= dalloc(h, &rec, size_t(cap) * 2);
     h->sf.rec = rec;
+    h->sf.recB = rec + cap;
     rc |= dalloc(h, &h->sf.work, 8);
     if (hipDeviceGetAttribute(&h->nCU, hipDeviceAttributeMultiprocessorCount, h->device) != hipSuccess || h->nCU < 1)
       h->nCU = 1;
@@ -2784,7 +2802,8 @@ int refresh_records(pic_handle *h) {
                        (h->sfMatrix || h->par.precision == PIC_PREC_MIXED) && h->sf.rec;
   if (!records || h->recFresh) return 0;
   hipLaunchKernelGGL(k_sf_records, dim3(grid_for(std::max<int64_t>(h->n, 1))), dim3(kBlock), 0, h->stream, h->P,
-                     h->start + h->hm.nElems, h->sf, const_cast<double4 *>(h->sf.rec));
+                     h->start + h->hm.nElems, h->sf, const_cast<double4 *>(h->sf.rec),
+                     const_cast<double4 *>(h->sf.recB));
   HIPCHK(hipGetLastError());
   h->recFresh = true;
   return 0;
