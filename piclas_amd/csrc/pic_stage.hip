@@ -523,9 +523,6 @@ __global__ __launch_bounds__(kBlock) void k_tria_locate(DevMesh M, int64_t n, co
 }
 
 // ---- sort by element --------------------------------------------------------
-// One wave per old element segment: stayers get their rank among the
-// stayers of the segment (order preserved); leavers take an arrival slot of
-// their new element (slot order fixed later by k_fix_arrivals).
 // One wave per old element segment: the particles that stay get their rank
 // among the element's stayers, ordered by 8 bins of their z across the
 // element's Gauss points and within a bin by their old order; arrivals take
