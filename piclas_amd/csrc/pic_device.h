@@ -531,7 +531,10 @@ __device__ __attribute__((noinline)) void em_field_dw(const DevMesh &M, int e, c
 
 // Mixed precision: LagrangeInterpolationPolys and EvaluateFieldAtRefPos in
 // FP32 (nodes equal to the FP32 machine epsilon count as hit)
-// (product form, as lagrange_prod; c: the factors in FP32)
+// (product form, as lagrange_prod; c: the factors in FP32).  The factors'
+// FP32 rounding is the same for every particle, so the basis is divided by
+// its sum: without that, sum L_i = 1 holds only to the factors' rounding and
+// every interpolated field carries the same relative bias.
 template <int NP>
 __device__ inline void lagrange_f32(float x, const float *xg, const float *c, float *L) {
   bool eq = false;
@@ -552,9 +555,12 @@ __device__ inline void lagrange_f32(float x, const float *xg, const float *c, fl
   float pre = 1.f;
 #pragma unroll
   for (int i = 0; i < NP; ++i) { L[i] = pre; pre = pre * d[i]; }
-  float suf = 1.f;
+  float suf = 1.f, sum = 0.f;
 #pragma unroll
-  for (int i = NP - 1; i >= 0; --i) { L[i] = L[i] * suf * c[i]; suf = suf * d[i]; }
+  for (int i = NP - 1; i >= 0; --i) { L[i] = L[i] * suf * c[i]; suf = suf * d[i]; sum = sum + L[i]; }
+  const float r = 1.f / sum;
+#pragma unroll
+  for (int i = 0; i < NP; ++i) L[i] = L[i] * r;
 }
 
 template <int N>
