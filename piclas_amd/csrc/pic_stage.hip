@@ -681,24 +681,54 @@ __global__ __launch_bounds__(kBlock) void k_gather(const int32_t *total, const i
 struct SpeciesFactor {
   double fsp[8];
 };
+// Bounding box of the records of one block of 64 (the wave's records j =
+// 64 b .. 64 b + 63; lanes past the end or with f = 0, which no gather
+// keeps, count as empty): blk[2 b] = (min x, min y, min z, 0), blk[2 b + 1] =
+// (max x, max y, max z, 0), +inf / -inf for a block without records.  The
+// gathers on general elements skip the blocks whose box misses a target.
+// Called by every lane of the wave.
+__device__ inline void record_block_box(double4 *blk, int64_t j, bool has, double x, double y, double z) {
+  const double inf = __builtin_inf();
+  double lo[3] = {has ? x : inf, has ? y : inf, has ? z : inf};
+  double hi[3] = {has ? x : -inf, has ? y : -inf, has ? z : -inf};
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      lo[c] = fmin(lo[c], __shfl_xor(lo[c], off, kWave));
+      hi[c] = fmax(hi[c], __shfl_xor(hi[c], off, kWave));
+    }
+  if ((j & (kWave - 1)) == 0) {
+    blk[2 * (j >> 6)] = make_double4(lo[0], lo[1], lo[2], 0.0);
+    blk[2 * (j >> 6) + 1] = make_double4(hi[0], hi[1], hi[2], 0.0);
+  }
+}
+
 __global__ __launch_bounds__(kBlock) void k_gather_state(const int32_t *total, const int32_t *perm, DevParts s, DevParts d,
                                                          const int32_t *newElem, SpeciesFactor sf, double4 *rec,
-                                                         double4 *recB) {
+                                                         double4 *recB, double4 *blk) {
   const int64_t j = int64_t(blockIdx.x) * kBlock + threadIdx.x;
-  if (j >= *total) return;
-  const int32_t i = perm[j];
-  const double x = s.x[i], y = s.y[i], z = s.z[i], vx = s.vx[i], vy = s.vy[i], vz = s.vz[i];
-  const uint8_t sp = s.spec[i];
-  d.x[j] = x; d.y[j] = y; d.z[j] = z;
-  d.vx[j] = vx; d.vy[j] = vy; d.vz[j] = vz;
-  d.id[j] = s.id[i]; d.spec[j] = sp; d.isNew[j] = s.isNew[i];
-  d.elem[j] = newElem[i];
-  if (s.rx) { d.rx[j] = s.rx[i]; d.ry[j] = s.ry[i]; d.rz[j] = s.rz[i]; }
-  if (rec) {
-    const double f = sf.fsp[sp];
-    rec[j] = make_double4(x, y, z, f);
-    recB[j] = make_double4(vx * f, vy * f, vz * f, f);
+  const int64_t n = *total;
+  if ((j & ~int64_t(kWave - 1)) >= n) return;   // the whole wave is past the end
+  const bool valid = j < n;
+  double x = 0.0, y = 0.0, z = 0.0, f = 0.0;
+  if (valid) {
+    const int32_t i = perm[j];
+    x = s.x[i]; y = s.y[i]; z = s.z[i];
+    const double vx = s.vx[i], vy = s.vy[i], vz = s.vz[i];
+    const uint8_t sp = s.spec[i];
+    d.x[j] = x; d.y[j] = y; d.z[j] = z;
+    d.vx[j] = vx; d.vy[j] = vy; d.vz[j] = vz;
+    d.id[j] = s.id[i]; d.spec[j] = sp; d.isNew[j] = s.isNew[i];
+    d.elem[j] = newElem[i];
+    if (s.rx) { d.rx[j] = s.rx[i]; d.ry[j] = s.ry[i]; d.rz[j] = s.rz[i]; }
+    if (rec) {
+      f = sf.fsp[sp];
+      rec[j] = make_double4(x, y, z, f);
+      recB[j] = make_double4(vx * f, vy * f, vz * f, f);
+    }
   }
+  if (blk) record_block_box(blk, j, valid && f != 0.0, x, y, z);
 }
 
 // ---- cell_volweight_mean ----------------------------------------------------
@@ -850,6 +880,8 @@ struct SFArgs {
   // read) and recB[i] = (f vx, f vy, f vz, f) (read for the staged records)
   const double4 *rec;
   const double4 *recB;
+  // per block of 64 records: the box of its records (record_block_box)
+  const double4 *blk;
   double r_inv;         // 1 / r_sf
   int nspec;
   int cartesian;        // every element's Gauss points form an axis-aligned tensor grid
@@ -1008,12 +1040,19 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
 // culls on one 32-byte record per particle and target and reads the second
 // only for the particles it keeps.
 __global__ __launch_bounds__(kBlock) void k_sf_records(DevParts P, const int32_t *count, SFArgs A, double4 *rec,
-                                                       double4 *recB) {
+                                                       double4 *recB, double4 *blk) {
   const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
-  if (i >= *count) return;
-  const double f = A.fsp[P.spec[i]];
-  rec[i] = make_double4(P.x[i], P.y[i], P.z[i], f);
-  recB[i] = make_double4(P.vx[i] * f, P.vy[i] * f, P.vz[i] * f, f);
+  const int64_t n = *count;
+  if ((i & ~int64_t(kWave - 1)) >= n) return;   // the whole wave is past the end
+  const bool valid = i < n;
+  double x = 0.0, y = 0.0, z = 0.0, f = 0.0;
+  if (valid) {
+    f = A.fsp[P.spec[i]];
+    x = P.x[i]; y = P.y[i]; z = P.z[i];
+    rec[i] = make_double4(x, y, z, f);
+    recB[i] = make_double4(P.vx[i] * f, P.vy[i] * f, P.vz[i] * f, f);
+  }
+  if (blk) record_block_box(blk, i, valid && f != 0.0, x, y, z);
 }
 
 // The next target of a persistent gather wave: targets are taken in
@@ -1071,6 +1110,7 @@ __device__ __attribute__((always_inline)) inline void sf_gather_mfma_target(cons
   // the reference's element cull |x - bary| > r_sf + ElemRadius, compared squared
   const double thr = A.r_sf + M.radius[t], thr2 = thr * thr;
   const double R2box = A.r2_sf * (1.0 + 1e-12);
+  const double R2skip = R2box * (1.0 + 1e-9), thr2skip = thr2 * (1.0 + 1e-9);
   const int kp = lane >> 4, jc = lane & 3;
   // groups read 64 consecutive ring records from a multiple of 64 (no wrap)
   auto group = [&](const double (*Lh)[8], int g) {
@@ -1141,22 +1181,53 @@ __device__ __attribute__((always_inline)) inline void sf_gather_mfma_target(cons
         head += kWave;
         __builtin_amdgcn_wave_barrier();
       };
-      // two chunks of 64 per pass: both position records loaded before
-      // either is culled (the second at a clamped index when the run ends
-      // within the first), then the current records of the kept particles only
-      for (int base = 0; base <= last; base += 2 * kWave) {
-        const int k0 = base + lane, k1 = base + kWave + lane;
-        const bool two = base + kWave <= last;
-        const double4 ra0 = rec_at(ra_s, min(k0, last));
-        const double4 ra1 = rec_at(ra_s, min(k1, last));
-        double px0, py0, pz0, px1, py1, pz1;
-        const bool p0 = cull(ra0, k0 <= last, px0, py0, pz0);
-        const bool p1 = cull(ra1, two && k1 <= last, px1, py1, pz1);
-        double4 rb0 = make_double4(0.0, 0.0, 0.0, 0.0), rb1 = rb0;
-        if (p0) rb0 = rec_at(rb_s, k0);
-        if (p1) rb1 = rec_at(rb_s, k1);
-        stage(p0, px0, py0, pz0, rb0);
-        if (two) stage(p1, px1, py1, pz1, rb1);
+      // The run is walked in the aligned blocks of 64 records that it
+      // overlaps.  A block whose box (record_block_box) lies farther than
+      // r_sf from the DOF box, or on curved elements farther than
+      // r_sf + ElemRadius from the barycentre, holds no record the cull
+      // keeps (each record's distances are at least the box's, and the
+      // skip thresholds carry a 1e-9 margin over the cull's for round-off):
+      // it is not read.  The blocks of up to 64 at a time are tested one
+      // per lane; the kept records are staged in record order, as without
+      // the skip.
+      const int bfirst = s0 >> 6, blast = s1 > s0 ? (s1 - 1) >> 6 : bfirst - 1;
+      for (int wb = bfirst; wb <= blast; wb += kWave) {
+        bool reach = false;
+        if (wb + lane <= blast) {
+          const double4 lo = A.blk[2 * (wb + lane)], hi = A.blk[2 * (wb + lane) + 1];
+          const double lx = lo.x + ox, ly = lo.y + oy, lz = lo.z + oz;
+          const double hx = hi.x + ox, hy = hi.y + oy, hz = hi.z + oz;
+          const double gx = fmax(0.0, fmax(blo[0] - hx, lx - bhi[0]));
+          const double gy = fmax(0.0, fmax(blo[1] - hy, ly - bhi[1]));
+          const double gz = fmax(0.0, fmax(blo[2] - hz, lz - bhi[2]));
+          reach = !(gx * gx + gy * gy + gz * gz > R2skip);
+          if (!LINEAR) {
+            const double nx = fmax(0.0, fmax(lx, -hx)), ny = fmax(0.0, fmax(ly, -hy)), nz = fmax(0.0, fmax(lz, -hz));
+            reach = reach && !(nx * nx + ny * ny + nz * nz > thr2skip);
+          }
+        }
+        unsigned long long km = __ballot(reach);
+        // two kept blocks per pass: both position records loaded before
+        // either is culled, then the current records of the kept particles
+        while (km) {
+          const int ia = __builtin_ctzll(km);
+          km &= km - 1;
+          const bool two = km != 0;
+          const int ib = two ? __builtin_ctzll(km) : ia;
+          if (two) km &= km - 1;
+          const int k0 = ((wb + ia) << 6) + lane - s0, k1 = ((wb + ib) << 6) + lane - s0;
+          const bool v0 = k0 >= 0 && k0 <= last, v1 = two && k1 >= 0 && k1 <= last;
+          const double4 ra0 = rec_at(ra_s, min(max(k0, 0), last));
+          const double4 ra1 = rec_at(ra_s, min(max(k1, 0), last));
+          double px0, py0, pz0, px1, py1, pz1;
+          const bool p0 = cull(ra0, v0, px0, py0, pz0);
+          const bool p1 = cull(ra1, v1, px1, py1, pz1);
+          double4 rb0 = make_double4(0.0, 0.0, 0.0, 0.0), rb1 = rb0;
+          if (p0) rb0 = rec_at(rb_s, k0);
+          if (p1) rb1 = rec_at(rb_s, k1);
+          stage(p0, px0, py0, pz0, rb0);
+          if (two) stage(p1, px1, py1, pz1, rb1);
+        }
       }
     }
   }
@@ -1978,6 +2049,12 @@ unsigned persist_grid(pic_handle *h, const void *kernel) {
   return g;
 }
 
+// the record block boxes, written with the records where the general-element
+// gather reads them (null on tensor grids, whose gather does not)
+inline double4 *block_boxes(pic_handle *h) {
+  return h->sf.cartesian ? nullptr : const_cast<double4 *>(h->sf.blk);
+}
+
 template <typename T>
 int dalloc(pic_handle *h, T **p, size_t count) {
   void *q = nullptr;
@@ -2095,7 +2172,8 @@ int run_deposition(pic_handle *h, bool timed) {
       // particle arrays they are formed here
       if (!h->recFresh)
         hipLaunchKernelGGL(k_sf_records, dim3(grid_for(std::max<int64_t>(h->n, 1))), dim3(kBlock), 0, h->stream, h->P,
-                           h->start + ne, h->sf, const_cast<double4 *>(h->sf.rec), const_cast<double4 *>(h->sf.recB));
+                           h->start + ne, h->sf, const_cast<double4 *>(h->sf.rec), const_cast<double4 *>(h->sf.recB),
+                           block_boxes(h));
       h->recFresh = true;
       if (timed) HIPCHK(hipEventRecord(h->ev[8], h->stream));
     }
@@ -2184,7 +2262,8 @@ int enqueue_sort(pic_handle *h, int64_t nGrid, bool withStaged) {
   for (int k = 0; k < 8; ++k) sfac.fsp[k] = h->sf.fsp[k];
   hipLaunchKernelGGL(k_gather_state, dim3(grid_for(n)), dim3(kBlock), 0, h->stream, total, h->perm, h->P, h->Q, h->newElem,
                      sfac, useRec ? const_cast<double4 *>(h->sf.rec) : nullptr,
-                     useRec ? const_cast<double4 *>(h->sf.recB) : nullptr);
+                     useRec ? const_cast<double4 *>(h->sf.recB) : nullptr,
+                     useRec ? block_boxes(h) : nullptr);
   if (withStaged) {
     // keep LastPartPos / LastElem / FieldAtParticle in the new order
     hipLaunchKernelGGL(k_gather<double>, dim3(grid_for(n)), dim3(kBlock), 0, h->stream, total, h->perm, h->P.lx, h->Q.lx, 1);
@@ -2479,6 +2558,9 @@ int pic_create(const pic_mesh_desc *mesh, const pic_params *par, pic_handle **ou
     rc |= dalloc(h, &rec, size_t(cap) * 2);
     h->sf.rec = rec;
     h->sf.recB = rec + cap;
+    double4 *blk = nullptr;
+    rc |= dalloc(h, &blk, size_t((cap + kWave - 1) / kWave) * 2);
+    h->sf.blk = blk;
     rc |= dalloc(h, &h->sf.work, 8);
     if (hipDeviceGetAttribute(&h->nCU, hipDeviceAttributeMultiprocessorCount, h->device) != hipSuccess || h->nCU < 1)
       h->nCU = 1;
@@ -2800,7 +2882,7 @@ int refresh_records(pic_handle *h) {
   if (!records || h->recFresh) return 0;
   hipLaunchKernelGGL(k_sf_records, dim3(grid_for(std::max<int64_t>(h->n, 1))), dim3(kBlock), 0, h->stream, h->P,
                      h->start + h->hm.nElems, h->sf, const_cast<double4 *>(h->sf.rec),
-                     const_cast<double4 *>(h->sf.recB));
+                     const_cast<double4 *>(h->sf.recB), block_boxes(h));
   HIPCHK(hipGetLastError());
   h->recFresh = true;
   return 0;
