@@ -332,6 +332,17 @@ def bench_annulus(args):
     s.synchronize()
     t = time.perf_counter() - t0
     st = s.stats()
+    nb = args.steps
+    if tuning.get("use_graph"):
+        # a replayed graph is timed as one piece: the per-stage times and the
+        # push kernel's events come from a few eager steps after the timed ones
+        nb = 3
+        s.set_tuning(**dict(tuning, use_graph=False))
+        s.reset_stats()
+        s.time_step(w.dt, nb)
+        s.synchronize()
+        sb = s.stats()
+        st = dict(sb, n_particles=st["n_particles"], n_lost=st["n_lost"])
     s.close()
     ms_step = t / args.steps * 1e3
     out = {"metric": METRIC, "value": n_in * args.steps / t, "unit": "particle-steps/s", "n_gpus": 1,
@@ -341,11 +352,13 @@ def bench_annulus(args):
                       "particles": n, "particles_placed": n_in, "deposition": "shape_function R=0.03 m alpha=2",
                       "tracking": "refmapping", "parallelism": "1 GPU"},
            "setup_s_locate_and_sort": round(t_locate, 3), "tuning": tuning,
-           "push_roofline": push_roofline(st, st["n_particles"], args.steps, "k_rm_push_inside<3, 3, false>",
+           "push_roofline": push_roofline(st, st["n_particles"], nb, "k_rm_push_inside<3, 3, false>",
                                           PUSH_BYTES_REFMAP, w.mesh.nElems * 5664),
-           "stage_ms_per_step": {"deposit": st["ms_deposit"] / args.steps,
-                                 "interp_push_track": st["ms_push_track"] / args.steps,
-                                 "sort": st["ms_sort"] / args.steps},
+           "stage_ms_per_step": {"deposit": st["ms_deposit"] / nb,
+                                 "interp_push_track": st["ms_push_track"] / nb,
+                                 "sort": st["ms_sort"] / nb,
+                                 "from": "the timed steps" if nb == args.steps else
+                                         "%d eager steps after the timed (replayed) ones" % nb},
            "n_removed_or_lost": {"lost": st["n_lost"], "inside_after": st["n_particles"]}}
     print(json.dumps(out), flush=True)
 
