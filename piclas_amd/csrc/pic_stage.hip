@@ -2479,7 +2479,6 @@ int pic_create(const pic_mesh_desc *mesh, const pic_params *par, pic_handle **ou
     for (int i = 0; i <= hm.N; ++i) {
       M.xGP32[i] = float(hm.xGP[i]);
       M.wBaryGP32[i] = float(hm.wBaryGP[i]);
-      M.cGP32[i] = float(M.cGP[i]);
     }
   }
   rc |= upload(h, &h->sJ, hm.sJ);
