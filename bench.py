@@ -217,7 +217,7 @@ def push_roofline(st, n, steps, kernel, bytes_per_particle, table_bytes):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--steps", type=int, default=100, help="timed steps (the cube and annulus workloads state 100)")
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--nx", type=int, default=48)
     ap.add_argument("--ppc", type=int, default=200)
