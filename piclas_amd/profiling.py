@@ -12,8 +12,12 @@
       that VALU issue (4 cycles per wave instruction on a 16-lane SIMD) and
       the matrix cores take together; GRBM_GUI_ACTIVE counts the cycles of
       each of the 8 XCDs (MI355X: 8 XCDs, 1024 SIMDs).
+  python -m piclas_amd.profiling perwave <counters.csv> <kernel> [kernel ...]
+      the wave-state counters of the named kernels (exact short names) per
+      wave: each counter summed over the kernel's dispatches, divided by
+      the waves they launched (Grid_Size / 64 per dispatch).
 
-Both print Markdown tables.
+All print Markdown tables.
 """
 import csv
 import sys
@@ -98,11 +102,37 @@ def issue(path, names=(), xcds=8, simds=1024):
     return "\n".join(lines)
 
 
+PERWAVE = ("SQ_WAVE_CYCLES", "SQ_ACTIVE_INST_VALU", "SQ_ACTIVE_INST_LDS", "SQ_ACTIVE_INST_MISC",
+           "SQ_WAIT_INST_ANY", "SQ_WAIT_INST_LDS", "SQ_LDS_BANK_CONFLICT")
+
+
+def perwave(path, names, wave=64):
+    agg, waves = OrderedDict((k, {}) for k in names), {k: {} for k in names}
+    for r in csv.DictReader(open(path)):
+        k = short(r["Kernel_Name"])
+        if k not in agg:
+            continue
+        agg[k][r["Counter_Name"]] = agg[k].get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
+        waves[k][r["Dispatch_Id"]] = int(r["Grid_Size"]) // wave
+
+    def fmt(v):
+        return "%.3e" % v if v >= 1e4 else ("%d" % round(v) if v >= 10 else "%.1f" % v)
+
+    lines = ["| kernel | " + " | ".join(PERWAVE) + " |", "|" + "---|" * (len(PERWAVE) + 1)]
+    for k, c in agg.items():
+        w = sum(waves[k].values())
+        if w:
+            lines.append("| `%s` | " % k + " | ".join(fmt(c.get(n, 0.0) / w) for n in PERWAVE) + " |")
+    return "\n".join(lines)
+
+
 def main(argv):
     if len(argv) >= 2 and argv[0] == "stats":
         print(kernel_stats(argv[1]))
     elif len(argv) >= 2 and argv[0] == "issue":
         print(issue(argv[1], tuple(argv[2:])))
+    elif len(argv) >= 3 and argv[0] == "perwave":
+        print(perwave(argv[1], tuple(argv[2:])))
     elif len(argv) >= 5 and argv[0] == "traffic":
         nb = int(argv[5]) if len(argv) > 5 else (1 << 30)
         print(traffic(argv[1], argv[2], argv[3], argv[4], nb))
