@@ -262,9 +262,10 @@ int  pic_adjoint(pic_handle *h, double dt, int32_t K, int32_t checkpoint_every, 
  *   sf_shared_loads  1: on meshes that are not axis-aligned tensor grids, the
  *              four waves of a workgroup take four consecutive target
  *              elements and stage each record of the union of their source
- *              lists once in LDS for the four (fewer record reads, more
- *              culls per wave); bit-identical as well.  Takes precedence
- *              over sf_persistent there. */
+ *              lists once in LDS for the four (more culls per wave, and no
+ *              skipping of the record blocks whose bounding box cannot reach
+ *              a target, which the default gather does); bit-identical as
+ *              well.  Takes precedence over sf_persistent there. */
 typedef struct {
   int32_t use_graph;
   int32_t sf_kernel;
